@@ -1,0 +1,160 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched PCR-Penta solve on one MI355X.
+
+Workload (default): m = 16384 systems of n = 1024, interleaved layout, plain
+FP64, seeded strictly dominant generator (off-diagonals U[-1,1], main
+U[4.5,5.5], RHS U[-1,1], seed 0). One "step" = one solve of the whole batch
+(y <- A^{-1} y in place). Every step gets its own pristine right-hand side,
+generated into HBM before the timed region, so nothing but solves is timed.
+
+Prints ONE JSON line: unknowns/s (value), ms per step, the dominant kernel's
+roofline (56 algorithmic bytes per unknown: a,b,c,d,e,y read + x written, all
+FP64) and the CPU baseline (the oracle's FP64 PCR-Penta, OpenMP over systems)
+timed on a sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pcr-penta_amd"))
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+BYTES_PER_UNKNOWN = 56     # 7 FP64 words
+
+WORKLOADS = {
+    # name: (n, m, precision, generator kind)
+    "lds_f64_m16384_n1024": (1024, 16384, "f64", 0),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="lds_f64_m16384_n1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-systems", type=int, default=2048,
+                    help="systems in the CPU-baseline sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys):
+    """The oracle's FP64 PCR-Penta over the first `nsys` systems of the same
+    workload, all host threads (OpenMP), median of 3 after one warm-up."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    oracle.load()
+    arrs = oracle.gen_batch(kind, seed, n, nsys, oracle.INTERLEAVED)
+    threads = oracle.max_threads()
+    times = []
+    x = None
+    for _ in range(4):
+        t0 = time.perf_counter()
+        x = oracle.pcr_batch(*arrs, n, nsys, oracle.INTERLEAVED, threads)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times[1:])[1]
+    agree = None
+    if gpu_x_sample is not None:
+        agree = bool(np.array_equal(x.reshape(n, nsys), gpu_x_sample))
+    return {
+        "value": n * nsys / t,
+        "unit": "unknowns/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {nsys} of the {m} systems (n={n}, same generator), FP64 PCR-Penta "
+                  f"restated from kernel.cu:32-154, gcc -O2 + OpenMP over systems, median of 3",
+        "seconds": t,
+        "bit_identical_to_gpu": agree,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(systems are independent: shard the batch across GPUs, "
+                                   "one process each)"}))
+        sys.exit(2)
+    import torch
+    import pcrpenta
+
+    n, m, prec, kind = WORKLOADS[args.workload]
+    layout = pcrpenta.INTERLEAVED
+    seed = 0
+    K, W = args.steps, args.warmup
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    a, b, c, d, e, y0 = pcrpenta.generate(kind, seed, n, m, layout, device=dev)
+    nbuf = max(1, min(K + W, int(8e9 // (8 * n * m))))
+    ys = [y0.clone() for _ in range(nbuf)]
+    y_check = y0.clone()
+    ws, nb = pcrpenta._workspace(n, m, pcrpenta.PREC_F64, dev)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+
+    def step(k):
+        return pcrpenta.solve(a, b, c, d, e, ys[k % nbuf], n, m, layout, stream=stream, ws=ws)
+
+    for w in range(W):
+        step(w)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(K)]
+    t0 = time.perf_counter()
+    for k in range(K):
+        evs[k][0].record(stream)
+        step(W + k)
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [s.elapsed_time(f) for s, f in evs]
+    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    ms_per_step = wall * 1e3 / K
+    unknowns = n * m
+    value = unknowns / (wall / K)
+
+    # one more solve on a pristine RHS for the CPU-baseline agreement check
+    pcrpenta.solve(a, b, c, d, e, y_check, n, m, layout, stream=stream, ws=ws)
+    torch.cuda.synchronize()
+
+    achieved = BYTES_PER_UNKNOWN * unknowns / avg_kernel_s
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(args.workload, {}).get("bytes_per_launch")
+    out = {
+        "metric": "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12",
+        "value": value,
+        "unit": "unknowns/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic (seeded counter-based generator, config-0 distribution)",
+        "config": {"workload": args.workload, "n": n, "m": m, "layout": "interleaved",
+                   "precision": prec, "seed": seed, "generator": "dominant"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel_ms": avg_kernel_s * 1e3},
+    }
+    if not args.no_cpu:
+        nsys = min(args.cpu_systems, m)
+        xs = y_check.view(n, m)[:, :nsys].cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(n, m, layout, kind, seed, xs, nsys)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

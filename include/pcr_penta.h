@@ -1,0 +1,111 @@
+/*
+ * pcr_penta.h — C ABI of the MI355X (gfx950) batched pentadiagonal solver.
+ *
+ * Method: PCR-Penta parallel cyclic reduction (Ghosh & Mishra, SIAM J. Sci.
+ * Comput., DOI 10.1137/20M1311053): floor(log2 n) reduction steps at stride
+ * 1, 2, 4, ..., each eliminating the +-1/+-2 couplings, then a 2x2 / 1x1 solve.
+ *
+ * Row i of a system reads
+ *     a_i x_{i-2} + b_i x_{i-1} + c_i x_i + d_i x_{i+1} + e_i x_{i+2} = y_i
+ * with row-aligned diagonals (a_0 = a_1 = b_0 = d_{n-1} = e_{n-2} = e_{n-1}
+ * are out of range and never affect the solution). This is the argument order
+ * (a,b,c,d,e,y) of the reference kernels, performanalysis.h:53-59.
+ *
+ * All array pointers are DEVICE pointers (hipMalloc'd or equivalent) unless
+ * noted. All entry points are asynchronous on `stream` (a hipStream_t; NULL
+ * means the null stream), never synchronise the device, never allocate device
+ * memory, and return a status code (PCRP_OK = 0) instead of exiting the
+ * process (the reference's checkCuda macro, performanalysis.h:12-20, exits).
+ * The diagonals a..e are never modified; y is overwritten by the solution
+ * (the reference overwrites a..e as well when n > 1024, kernel.cu:304-306).
+ */
+#ifndef PCR_PENTA_H
+#define PCR_PENTA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCRP_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+    PCRP_OK = 0,
+    PCRP_ERR_INVALID_SIZE = 1,   /* n < 1, m < 1, or sizes beyond 2^31 elements */
+    PCRP_ERR_NULL = 2,           /* a required pointer is NULL */
+    PCRP_ERR_LAYOUT = 3,         /* unknown layout code */
+    PCRP_ERR_WORKSPACE = 4,      /* ws_bytes < pcrp_workspace_size(...) */
+    PCRP_ERR_HIP = 5,            /* a HIP runtime call or kernel launch failed */
+    PCRP_ERR_ARG = 6,            /* other invalid argument (max_sweeps, tol, kind ...) */
+    PCRP_ERR_UNSUPPORTED = 7     /* valid request this build cannot serve */
+};
+
+/* Batch layouts. Element i of system j lives at
+ *   CONTIGUOUS:  j*n + i   (the reference's own layout, kernel.cu:43)
+ *   INTERLEAVED: i*m + j   (Annexure A p.2; cuSPARSE gpsvInterleavedBatch) */
+enum { PCRP_LAYOUT_CONTIGUOUS = 0, PCRP_LAYOUT_INTERLEAVED = 1 };
+
+/* Precision forms. */
+enum {
+    PCRP_PREC_F64 = 0,           /* plain FP64 PCR-Penta */
+    PCRP_PREC_F32_REFINE = 1     /* FP32 PCR + FP64 residual/update refinement */
+};
+
+int pcrp_abi_version(void);
+const char *pcrp_strerror(int status);
+
+/* Device workspace (bytes) a solve of this shape needs; 0 when none. */
+size_t pcrp_workspace_size(int n, int m, int precision);
+
+/*
+ * Plain FP64 batched solve.
+ * Replaces the solve half of the reference's host dispatch
+ * time_shared_cached_shared_tile_splitting (main.cu:14-89): pcr_penta for
+ * n <= 1024 (main.cu:46-54, kernel.cu:32-154) and the split pair
+ * shared_cached_shared_tile_splitting + pcr_penta_splitting_2nd_stage for
+ * n > 1024 (main.cu:57-86, kernel.cu:158-320, 665-791). Any n >= 1 is
+ * accepted (the reference needs n <= 1024 or n = 1024*2^s, main.cu:9-10).
+ * Results equal a host FP64 run of the same algorithm bit for bit.
+ */
+int pcrp_solve_f64(const double *a, const double *b, const double *c,
+                   const double *d, const double *e, double *y,
+                   int n, int m, int layout,
+                   void *ws, size_t ws_bytes, void *stream);
+
+/*
+ * FP32 PCR + FP64 iterative refinement (no counterpart in the reference,
+ * which is FP64-only). x0 = PCR32(y); then up to max_sweeps times:
+ * r = y - A x (FP64), dx = PCR32(r), x += dx (FP64), until
+ * ||dx||_inf <= tol * ||x||_inf for that system. Inputs and output are FP64.
+ * sweeps_out: optional DEVICE array of m ints receiving each system's sweep
+ * count (max_sweeps + 1 flags a system that did not meet tol).
+ */
+int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
+                        const double *d, const double *e, double *y,
+                        int n, int m, int layout, int max_sweeps, double tol,
+                        int *sweeps_out, void *ws, size_t ws_bytes, void *stream);
+
+/*
+ * Seeded input generator (device). kind 0: off-diagonals U[-1,1], main
+ * U[4.5,5.5]; kind 1: off-diagonals U[-0.5,0.5], main U[2.2,2.6]; RHS
+ * U[-1,1]; out-of-range diagonal entries zeroed. Values depend only on
+ * (seed, diagonal, system, row), not on the layout.
+ * Device counterpart of the reference's initialize kernel (kernel.cu:5-26).
+ */
+int pcrp_generate(int kind, uint64_t seed, int n, int m, int layout,
+                  double *a, double *b, double *c, double *d, double *e, double *y,
+                  void *stream);
+
+/* The reference's own test system T^2 (T = tridiag(-1,2,-1)), y = 1:
+ * kernel.cu:5-26 (initialize), including its out-of-range b_0 = d_{n-1} = 1. */
+int pcrp_init_t2(int n, int m, int layout,
+                 double *a, double *b, double *c, double *d, double *e, double *y,
+                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PCR_PENTA_H */

@@ -1,0 +1,162 @@
+"""Python host binding of libpcrpenta.so (the gfx950 PCR-Penta solver).
+
+Mirrors the reference's solve interface — six arrays (a, b, c, d, e, y) =
+(sub-2, sub-1, main, super-1, super-2, rhs), y overwritten by the solution
+(performanalysis.h:53, kernel.cu:32-39) — over torch device tensors. torch
+only provides device memory and the stream; every solve runs the hand-written
+HIP kernels through the C ABI of include/pcr_penta.h. There is no fallback:
+if the library is missing or a call fails, a PCRPError is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpcrpenta.so")
+
+CONTIGUOUS = 0   # element i of system j at j*n + i  (kernel.cu:43)
+INTERLEAVED = 1  # element i of system j at i*m + j  (Annexure A p.2)
+PREC_F64 = 0
+PREC_F32_REFINE = 1
+
+STATUS = {
+    0: "PCRP_OK", 1: "PCRP_ERR_INVALID_SIZE", 2: "PCRP_ERR_NULL", 3: "PCRP_ERR_LAYOUT",
+    4: "PCRP_ERR_WORKSPACE", 5: "PCRP_ERR_HIP", 6: "PCRP_ERR_ARG", 7: "PCRP_ERR_UNSUPPORTED",
+}
+
+
+class PCRPError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({strerror(status)})")
+
+
+_lib = None
+_vp = ctypes.c_void_p
+
+
+def build() -> str:
+    """Compile libpcrpenta.so for gfx950 with make (always recompiles stale objects)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"], check=True)
+    return LIB_PATH
+
+
+def _declare(lib):
+    lib.pcrp_abi_version.restype = ctypes.c_int
+    lib.pcrp_strerror.argtypes = [ctypes.c_int]
+    lib.pcrp_strerror.restype = ctypes.c_char_p
+    lib.pcrp_workspace_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.pcrp_workspace_size.restype = ctypes.c_size_t
+    lib.pcrp_solve_f64.argtypes = [_vp] * 6 + [ctypes.c_int] * 3 + [_vp, ctypes.c_size_t, _vp]
+    lib.pcrp_solve_f64.restype = ctypes.c_int
+    lib.pcrp_solve_f32r_f64.argtypes = ([_vp] * 6 + [ctypes.c_int] * 4 + [ctypes.c_double, _vp, _vp,
+                                        ctypes.c_size_t, _vp])
+    lib.pcrp_solve_f32r_f64.restype = ctypes.c_int
+    lib.pcrp_generate.argtypes = ([ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int] + [_vp] * 6 + [_vp])
+    lib.pcrp_generate.restype = ctypes.c_int
+    lib.pcrp_init_t2.argtypes = [ctypes.c_int] * 3 + [_vp] * 6 + [_vp]
+    lib.pcrp_init_t2.restype = ctypes.c_int
+    return lib
+
+
+def load():
+    """Load the in-tree libpcrpenta.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PCRPError(7, f"{LIB_PATH} missing (run __graft_entry__.build())")
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def strerror(status: int) -> str:
+    return load().pcrp_strerror(int(status)).decode()
+
+
+def _check(st: int, what: str):
+    if st != 0:
+        raise PCRPError(st, what)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _validate(arrs, n, m, dtype_name="float64"):
+    import torch
+    for t in arrs:
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError("pcrpenta: arrays must be CUDA (HIP) tensors")
+        if str(t.dtype) != f"torch.{dtype_name}":
+            raise TypeError(f"pcrpenta: arrays must be {dtype_name}")
+        if not t.is_contiguous() or t.numel() != n * m:
+            raise ValueError(f"pcrpenta: arrays must be contiguous with n*m = {n * m} elements")
+
+
+def workspace_size(n: int, m: int, precision: int = PREC_F64) -> int:
+    return int(load().pcrp_workspace_size(n, m, precision))
+
+
+def _workspace(n, m, precision, device):
+    import torch
+    nb = workspace_size(n, m, precision)
+    ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=device)
+    return ws, nb
+
+
+def solve(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, stream=None, ws=None):
+    """Plain FP64 PCR-Penta: y <- A^{-1} y for m systems of size n (in place)."""
+    _validate([a, b, c, d, e, y], n, m)
+    if ws is None:
+        ws, nb = _workspace(n, m, PREC_F64, y.device)
+    else:
+        nb = ws.numel()
+    st = load().pcrp_solve_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], n, m, layout,
+                               _ptr(ws), nb, _stream(stream))
+    _check(st, "pcrp_solve_f64")
+    return y
+
+
+def solve_refined(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, max_sweeps: int = 8,
+                  tol: float = 1e-13, sweeps=None, stream=None, ws=None):
+    """FP32 PCR-Penta + FP64 iterative refinement (in place on y).
+    ``sweeps``: optional int32 device tensor of m entries for per-system sweep counts."""
+    _validate([a, b, c, d, e, y], n, m)
+    if ws is None:
+        ws, nb = _workspace(n, m, PREC_F32_REFINE, y.device)
+    else:
+        nb = ws.numel()
+    sp = _ptr(sweeps) if sweeps is not None else None
+    st = load().pcrp_solve_f32r_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], n, m, layout,
+                                    max_sweeps, tol, sp, _ptr(ws), nb, _stream(stream))
+    _check(st, "pcrp_solve_f32r_f64")
+    return y
+
+
+def generate(kind: int, seed: int, n: int, m: int, layout: int = INTERLEAVED, device="cuda",
+             stream=None):
+    """Seeded batch on the device (same values as the oracle's generator)."""
+    import torch
+    arrs = [torch.empty(n * m, dtype=torch.float64, device=device) for _ in range(6)]
+    st = load().pcrp_generate(kind, seed, n, m, layout, *[_ptr(t) for t in arrs], _stream(stream))
+    _check(st, "pcrp_generate")
+    return arrs
+
+
+def init_t2(n: int, m: int, layout: int = INTERLEAVED, device="cuda", stream=None):
+    """The reference's T^2 test batch (kernel.cu:5-26) on the device."""
+    import torch
+    arrs = [torch.empty(n * m, dtype=torch.float64, device=device) for _ in range(6)]
+    st = load().pcrp_init_t2(n, m, layout, *[_ptr(t) for t in arrs], _stream(stream))
+    _check(st, "pcrp_init_t2")
+    return arrs

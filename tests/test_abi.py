@@ -1,0 +1,89 @@
+"""CPU-side checks of the C ABI: the library loads, exports every entry point
+declared in include/pcr_penta.h, and rejects invalid requests with a status
+code before touching the device (the reference's checkCuda exits instead,
+performanalysis.h:12-20)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pcr_penta.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pcrp_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import pcrpenta
+    if not os.path.exists(pcrpenta.LIB_PATH):
+        pcrpenta.build()
+    return pcrpenta.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert "pcrp_solve_f64" in names and len(names) >= 6
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_abi_version_and_strerror(lib):
+    import pcrpenta
+    assert lib.pcrp_abi_version() == 1
+    for st, name in pcrpenta.STATUS.items():
+        assert pcrpenta.strerror(st)
+    assert pcrpenta.strerror(999) == "unknown status"
+
+
+def _fake(n):
+    # never dereferenced: validation fails before any launch
+    return [ctypes.c_void_p(0x1000 * (k + 1)) for k in range(n)]
+
+
+@pytest.mark.parametrize("n,m", [(0, 4), (4, 0), (-1, 1), (1 << 16, 1 << 16)])
+def test_invalid_sizes(lib, n, m):
+    st = lib.pcrp_solve_f64(*_fake(6), n, m, 1, None, 0, None)
+    assert st == 1
+    st = lib.pcrp_solve_f32r_f64(*_fake(6), n, m, 1, 4, 1e-13, None, None, 0, None)
+    assert st == 1
+
+
+def test_null_pointer(lib):
+    p = _fake(6)
+    p[2] = None
+    assert lib.pcrp_solve_f64(*p, 16, 4, 1, None, 0, None) == 2
+    assert lib.pcrp_generate(0, 1, 16, 4, 1, *p, None) == 2
+
+
+def test_bad_layout(lib):
+    assert lib.pcrp_solve_f64(*_fake(6), 16, 4, 7, None, 0, None) == 3
+    assert lib.pcrp_init_t2(16, 4, -1, *_fake(6), None) == 3
+
+
+def test_bad_generator_kind(lib):
+    assert lib.pcrp_generate(5, 1, 16, 4, 1, *_fake(6), None) == 6
+
+
+def test_refine_argument_checks(lib):
+    assert lib.pcrp_solve_f32r_f64(*_fake(6), 16, 4, 1, -1, 1e-13, None, None, 0, None) == 6
+    assert lib.pcrp_solve_f32r_f64(*_fake(6), 16, 4, 1, 4, -1.0, None, None, 0, None) == 6
+
+
+def test_workspace_is_reported(lib):
+    for prec in (0, 1):
+        for n in (1, 512, 1024, 2048, 4096, 1 << 20):
+            assert lib.pcrp_workspace_size(n, 8, prec) >= 0
+
+
+def test_workspace_too_small_is_rejected(lib):
+    n, m = 1 << 14, 4
+    need = lib.pcrp_workspace_size(n, m, 0)
+    if need == 0:
+        pytest.skip("no workspace needed at this size")
+    assert lib.pcrp_solve_f64(*_fake(6), n, m, 1, _fake(1)[0], need - 1, None) == 4

@@ -1,0 +1,109 @@
+"""GPU parity: the HIP kernels against the CPU oracle on the same seeded inputs.
+
+Plain FP64 must equal the oracle's FP64 PCR-Penta (kernel.cu:32-154 restated)
+bit for bit (signed zeros compare equal); the FP32+refine form must reach
+normwise relative error <= 1e-12 against the FP64 banded-LU (dgbsv) oracle.
+Each test prints its case before touching the GPU."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C, I = 0, 1  # layouts
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _say(*args):
+    print("[case]", *args, flush=True)
+
+
+def _run_f64(gpu, arrs_dev, n, m, layout):
+    a, b, c, d, e, y = arrs_dev
+    gpu.solve(a, b, c, d, e, y, n, m, layout)
+    import torch
+    torch.cuda.synchronize()
+    return _np(y)
+
+
+def test_generator_matches_oracle(gpu, oracle_lib):
+    for kind, seed, n, m, layout in ((0, 0, 512, 256, I), (1, 5, 1000, 3, C), (0, 1, 37, 129, I)):
+        _say("generate", kind, seed, n, m, layout)
+        dev = gpu.generate(kind, seed, n, m, layout)
+        host = oracle_lib.gen_batch(kind, seed, n, m, layout)
+        for u, v in zip(dev, host):
+            assert np.array_equal(_np(u), v)
+
+
+@pytest.mark.parametrize("layout", [I, C])
+def test_config0_bit_exact(gpu, oracle_lib, layout):
+    """Config 0: m=256 systems of n=512, seed 0, strictly dominant."""
+    n, m = 512, 256
+    _say("config0", layout)
+    dev = gpu.generate(0, 0, n, m, layout)
+    ins = [_np(t).copy() for t in dev]
+    x = _run_f64(gpu, dev, n, m, layout)
+    ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+    assert np.array_equal(x, ref)
+    # a..e untouched
+    for t, h in zip(dev[:5], ins[:5]):
+        assert np.array_equal(_np(t), h)
+    # and within FP64 PCR noise of the banded-LU solution
+    xg = oracle_lib.gbsv_batch(*ins, n, m, layout)
+    assert oracle_lib.normwise_relerr(x, xg, n, m, layout) < 1e-10
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 9, 31, 33, 63, 64, 65, 100, 127, 128, 129,
+                               255, 256, 257, 511, 512, 513, 777, 1000, 1023, 1024])
+def test_lds_sizes_bit_exact(gpu, oracle_lib, n):
+    m = 37
+    for kind, layout in ((0, I), (1, C)):
+        _say("lds", n, m, kind, layout)
+        dev = gpu.generate(kind, 100 + n, n, m, layout)
+        ins = [_np(t).copy() for t in dev]
+        x = _run_f64(gpu, dev, n, m, layout)
+        ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+        assert np.array_equal(x, ref), (n, kind, layout)
+
+
+def test_reference_t2_system(gpu, oracle_lib):
+    """The reference's own input (kernel.cu:5-26), incl. its out-of-range b_0 = d_{n-1} = 1."""
+    for n in (8, 64, 256, 512, 1000, 1024):
+        m = 3
+        _say("t2", n)
+        dev = gpu.init_t2(n, m, I)
+        host = oracle_lib.init_t2(n, m, I)
+        for u, v in zip(dev, host):
+            assert np.array_equal(_np(u), v)
+        x = _run_f64(gpu, dev, n, m, I)
+        assert np.array_equal(x, oracle_lib.pcr_batch(*host, n, m, I))
+
+
+def test_layouts_agree_bit_for_bit(gpu, oracle_lib):
+    n, m = 300, 20
+    _say("layouts", n, m)
+    xi = _run_f64(gpu, gpu.generate(0, 3, n, m, I), n, m, I).reshape(n, m).T
+    xc = _run_f64(gpu, gpu.generate(0, 3, n, m, C), n, m, C).reshape(m, n)
+    assert np.array_equal(xi, xc)
+
+
+def test_zero_pivot_rule(gpu, oracle_lib):
+    """Exact zero pivots (c1 == 0 / c2 == 0, kernel.cu:122,129) take the
+    multiplier the reference would reuse; results match the oracle."""
+    n, m = 16, 4
+    _say("zero-pivot", n, m)
+    import torch
+    arrs = oracle_lib.gen_batch(0, 2, n, m, C)
+    # make the first pair equation's pivot exactly zero in system 1: c'_j = c_j + al*d_{j-1} + ar*b_{j+1}
+    for k in range(6):
+        arrs[k] = arrs[k].copy()
+    j0 = 1 * n
+    arrs[0][j0:j0 + n] = 0.0
+    arrs[4][j0:j0 + n] = 0.0     # tridiagonal system: E(j).C = c_j when a=e=0
+    arrs[2][j0 + 5] = 0.0        # zero main diagonal entry -> zero pivot at step 0
+    dev = [torch.from_numpy(v).cuda() for v in arrs]
+    x = _run_f64(gpu, dev, n, m, C)
+    ref = oracle_lib.pcr_batch(*arrs, n, m, C)
+    assert np.array_equal(np.nan_to_num(x, nan=7.0), np.nan_to_num(ref, nan=7.0))
