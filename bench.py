@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-systems", type=int, default=2048,
                     help="systems in the CPU-baseline sample")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner; use the built-in default variant")
     return ap.parse_args()
 
 
@@ -98,6 +100,16 @@ def main():
     ws, nb = pcrpenta._workspace(n, m, pcrpenta.PREC_F64, dev)
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_tune:
+        # the autotuner times every LDS variant on this very batch (outside the timed region)
+        best, best_ms, times = pcrpenta.autotune(a, b, c, d, e, y0, n, m, layout, reps=3)
+        vs = pcrpenta.variants()
+        tuned = {"variant": vs[best], "ms": best_ms,
+                 "all_ms": {f"{v['kind']}-{v['nt']}x{v['rpt']}": round(t, 4)
+                            for v, t in zip(vs, times) if t >= 0}}
+        print(json.dumps({"autotune": tuned}), file=sys.stderr, flush=True)
+    variant = pcrpenta.variants()[pcrpenta.tune_get(n)]
 
     def step(k):
         return pcrpenta.solve(a, b, c, d, e, ys[k % nbuf], n, m, layout, stream=stream, ws=ws)
@@ -144,7 +156,8 @@ def main():
         "dtype": prec,
         "data": "synthetic (seeded counter-based generator, config-0 distribution)",
         "config": {"workload": args.workload, "n": n, "m": m, "layout": "interleaved",
-                   "precision": prec, "seed": seed, "generator": "dominant"},
+                   "precision": prec, "seed": seed, "generator": "dominant",
+                   "kernel_variant": f"{variant['kind']}-{variant['nt']}x{variant['rpt']}"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": avg_kernel_s * 1e3},

@@ -105,6 +105,32 @@ int pcrp_init_t2(int n, int m, int layout,
                  double *a, double *b, double *c, double *d, double *e, double *y,
                  void *stream);
 
+/* ---- Autotuner --------------------------------------------------------
+ * The reference hard-codes its launch shapes (tile_size = 512, one thread per
+ * row, the n <= 1024 threshold: main.cu:16-22, 46-49). Here each size bucket
+ * ceil(log2 n) maps to one compiled LDS-kernel variant (kind, threads NT, rows
+ * per thread RPT); the table starts at a built-in default and is set by
+ * timing every variant on the device. */
+int pcrp_lds_variant_count(int precision);
+int pcrp_lds_variant_info(int precision, int id, int *kind, int *nt, int *rpt,
+                          int *max_n, int *lds_bytes);
+/* Pin the bucket containing n to variant_id (-1: back to the default). */
+int pcrp_tune_set(int precision, int n, int variant_id);
+/* Variant the next solve of size n would use (-1 if n is not an LDS size). */
+int pcrp_tune_get(int precision, int n);
+/* Time every variant that fits n on the caller's batch (y is copied into
+ * y_work before each timed launch; a..e, y unchanged), keep the fastest for
+ * n's bucket. Synchronises `stream`. times_ms: optional host array of
+ * pcrp_lds_variant_count() entries (-1 for variants that do not fit). */
+int pcrp_autotune(int precision, int n, int m, int layout,
+                  const double *a, const double *b, const double *c,
+                  const double *d, const double *e, const double *y,
+                  double *y_work, int reps, void *stream,
+                  int *best_id, float *best_ms, float *times_ms);
+/* Persist / restore the table as text (host file). */
+int pcrp_tune_save(const char *path);
+int pcrp_tune_load(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,8 @@
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
+#include <stdio.h>
+
 using namespace pcrp;
 
 namespace {
@@ -73,6 +75,29 @@ uint64_t host_splitmix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+
+// ---- autotuner state: per precision, per size bucket ceil(log2 n) -> variant
+constexpr int kBuckets = 12;   // n in [1, 2048]
+int g_tuned[2][kBuckets] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
+                            {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+
+int bucket_of(int n)
+{
+    int b = 0;
+    while ((1 << b) < n) ++b;
+    return b;
+}
+
+int tuned_variant(int prec, int n)
+{
+    const int b = bucket_of(n);
+    if (b < kBuckets && g_tuned[prec][b] >= 0) {
+        const LdsVariantF64* v = lds_f64_variant(g_tuned[prec][b]);
+        if (v && v->max_n >= n) return g_tuned[prec][b];
+    }
+    return lds_f64_default_variant(n);
+}
+
 }  // namespace
 
 extern "C" {
@@ -111,7 +136,10 @@ int pcrp_solve_f64(const double* a, const double* b, const double* c, const doub
     SolveArgsF64 A{a, b, c, d, e, y, n, m, 0, 0, pcr_steps(n)};
     if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (n <= lds_f64_max_n()) return lds_f64_solve(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    if (n <= lds_f64_max_n()) {
+        const LdsVariantF64* v = lds_f64_variant(tuned_variant(PCRP_PREC_F64, n));
+        return v->launch(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    }
     return PCRP_ERR_UNSUPPORTED;
 }
 
@@ -127,6 +155,130 @@ int pcrp_solve_f32r_f64(const double* a, const double* b, const double* c, const
     long long rs, ss;
     if (!layout_strides(n, m, layout, &rs, &ss)) return PCRP_ERR_LAYOUT;
     return PCRP_ERR_UNSUPPORTED;
+}
+
+int pcrp_lds_variant_count(int precision)
+{
+    return precision == PCRP_PREC_F64 ? lds_f64_variant_count() : 0;
+}
+
+int pcrp_lds_variant_info(int precision, int id, int* kind, int* nt, int* rpt, int* max_n,
+                          int* lds_bytes)
+{
+    if (precision != PCRP_PREC_F64) return PCRP_ERR_ARG;
+    const LdsVariantF64* v = lds_f64_variant(id);
+    if (!v) return PCRP_ERR_ARG;
+    if (kind) *kind = v->kind;
+    if (nt) *nt = v->nt;
+    if (rpt) *rpt = v->rpt;
+    if (max_n) *max_n = v->max_n;
+    if (lds_bytes) *lds_bytes = lds_f64_lds_bytes(*v);
+    return PCRP_OK;
+}
+
+int pcrp_tune_set(int precision, int n, int variant_id)
+{
+    if (precision != PCRP_PREC_F64 || n < 1) return PCRP_ERR_ARG;
+    const int b = bucket_of(n);
+    if (b >= kBuckets) return PCRP_ERR_ARG;
+    if (variant_id >= 0) {
+        const LdsVariantF64* v = lds_f64_variant(variant_id);
+        if (!v || v->max_n < n) return PCRP_ERR_ARG;
+    }
+    g_tuned[precision][b] = variant_id < 0 ? -1 : variant_id;
+    return PCRP_OK;
+}
+
+int pcrp_tune_get(int precision, int n)
+{
+    if (precision != PCRP_PREC_F64 || n < 1 || n > lds_f64_max_n()) return -1;
+    return tuned_variant(precision, n);
+}
+
+int pcrp_autotune(int precision, int n, int m, int layout, const double* a, const double* b,
+                  const double* c, const double* d, const double* e, const double* y,
+                  double* y_work, int reps, void* stream, int* best_id, float* best_ms,
+                  float* times_ms)
+{
+    int st = check_shape(n, m);
+    if (st != PCRP_OK) return st;
+    if (!a || !b || !c || !d || !e || !y || !y_work) return PCRP_ERR_NULL;
+    if (precision != PCRP_PREC_F64 || reps < 1) return PCRP_ERR_ARG;
+    if (n > lds_f64_max_n()) return PCRP_ERR_UNSUPPORTED;
+    SolveArgsF64 A{a, b, c, d, e, y_work, n, m, 0, 0, pcr_steps(n)};
+    if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return PCRP_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return PCRP_ERR_HIP; }
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)m;
+    int best = -1;
+    float bt = 0.f;
+    st = PCRP_OK;
+    for (int id = 0; id < lds_f64_variant_count() && st == PCRP_OK; ++id) {
+        const LdsVariantF64* v = lds_f64_variant(id);
+        float tmin = -1.f;
+        if (v->max_n >= n) {
+            for (int r = 0; r < reps && st == PCRP_OK; ++r) {
+                float t = 0.f;
+                if (hipMemcpyAsync(y_work, y, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    hipEventRecord(e0, s) != hipSuccess || v->launch(A, s) != 0 ||
+                    hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+                    st = PCRP_ERR_HIP;
+                    break;
+                }
+                if (tmin < 0.f || t < tmin) tmin = t;
+            }
+            if (st == PCRP_OK && (best < 0 || tmin < bt)) { best = id; bt = tmin; }
+        }
+        if (times_ms) times_ms[id] = tmin;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != PCRP_OK) return st;
+    g_tuned[precision][bucket_of(n)] = best;
+    if (best_id) *best_id = best;
+    if (best_ms) *best_ms = bt;
+    return PCRP_OK;
+}
+
+int pcrp_tune_save(const char* path)
+{
+    if (!path) return PCRP_ERR_NULL;
+    FILE* f = fopen(path, "w");
+    if (!f) return PCRP_ERR_ARG;
+    fprintf(f, "# pcrp tuning table: precision bucket(ceil log2 n) variant kind nt rpt\n");
+    for (int p = 0; p < 2; ++p)
+        for (int b = 0; b < kBuckets; ++b) {
+            const LdsVariantF64* v = g_tuned[p][b] >= 0 ? lds_f64_variant(g_tuned[p][b]) : nullptr;
+            if (v) fprintf(f, "%d %d %d %d %d %d\n", p, b, g_tuned[p][b], v->kind, v->nt, v->rpt);
+        }
+    fclose(f);
+    return PCRP_OK;
+}
+
+int pcrp_tune_load(const char* path)
+{
+    if (!path) return PCRP_ERR_NULL;
+    FILE* f = fopen(path, "r");
+    if (!f) return PCRP_ERR_ARG;
+    char line[256];
+    int st = PCRP_OK;
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        int p, b, id, kind, nt, rpt;
+        if (sscanf(line, "%d %d %d %d %d %d", &p, &b, &id, &kind, &nt, &rpt) != 6) continue;
+        const LdsVariantF64* v = (p == PCRP_PREC_F64) ? lds_f64_variant(id) : nullptr;
+        if (!v || b < 0 || b >= kBuckets || v->kind != kind || v->nt != nt || v->rpt != rpt ||
+            v->max_n < (1 << b)) {
+            st = PCRP_ERR_ARG;   // table from another build: ignore that entry
+            continue;
+        }
+        g_tuned[p][b] = id;
+    }
+    fclose(f);
+    return st;
 }
 
 int pcrp_generate(int kind, uint64_t seed, int n, int m, int layout, double* a, double* b,

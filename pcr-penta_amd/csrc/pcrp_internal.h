@@ -17,8 +17,18 @@ struct SolveArgsF64 {
     int steps;
 };
 
+// A compiled LDS-kernel variant: KIND (0 shared pair equations, 1 redundant),
+// NT threads x RPT rows per thread, capacity max_n = NT*RPT rows.
+struct LdsVariantF64 {
+    int kind, nt, rpt, max_n;
+    int (*launch)(const SolveArgsF64&, hipStream_t);
+};
+
+int lds_f64_variant_count();
+const LdsVariantF64* lds_f64_variant(int id);
+int lds_f64_default_variant(int n);
+int lds_f64_lds_bytes(const LdsVariantF64& v);
 int lds_f64_max_n();
-int lds_f64_solve(const SolveArgsF64& A, hipStream_t st);
 
 inline int pcr_steps(long long n)
 {

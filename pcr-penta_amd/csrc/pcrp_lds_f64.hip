@@ -3,33 +3,84 @@
 // Counterpart of pcr_penta (kernel.cu:32-154) and of the second split stage
 // pcr_penta_splitting_2nd_stage (kernel.cu:665-791), which is the same solve
 // on a strided sub-system. Differences by design:
-//   * NT threads each own RPT rows (row i = t + NT*k), so n is not tied to
-//     the workgroup size and the rows of one thread stay in VGPRs;
-//   * Reduction-I/II are shared per centre j (pair equations, pcrp_device.h):
-//     4 FP64 divisions per row and step instead of 6;
-//   * rows live in VGPRs; LDS holds the rows (6 doubles) and pair equations
-//     (8 doubles) that neighbours read: 2 barriers per step, as kernel.cu:135,142;
-//   * any element addressing (row stride, system stride), so both the
-//     reference's contiguous layout and the interleaved layout work;
+//   * NT threads each own RPT rows (row i = t + NT*k), so n is not tied to the
+//     workgroup size, the rows of a thread stay in VGPRs and the RPT rows give
+//     each thread independent division chains to interleave;
+//   * steps with stride >= NT couple only rows of the same thread: they run in
+//     registers with no LDS traffic and no barrier;
+//   * any element addressing (row stride, system stride, base offset), so the
+//     reference's contiguous layout, the interleaved layout and the strided
+//     sub-systems of the split path all use the same kernel;
 //   * a..e are read-only; y is overwritten by x.
-// A sub-system view (used by the split path) is expressed through
-// (base offset, row stride, n_sub, steps, final delta) — the reference's
-// stage-2 index map g = sys*n + id*G + r (kernel.cu:680) is the special case
-// row stride G, offset r.
+// Two ways to run a step through LDS (chosen per size by the autotuner):
+//   SHARED  : each centre j builds its pair equation E(j) once (pcrp_device.h),
+//             4 FP64 divisions per row-step, LDS = rows + pair equations
+//             (14 doubles per row);
+//   REDUNDANT: each row builds E(i-d) and E(i+d) itself from four neighbour
+//             rows, exactly as kernel.cu:66-117 writes it: 6 divisions per
+//             row-step but only the rows in LDS (6 doubles per row), so three
+//             times as many workgroups fit per CU.
+// Both give the same bits (the pair equation is the same expression on the
+// same operands either way).
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
 namespace pcrp {
 
-template <int NT, int RPT>
+enum { KIND_SHARED = 0, KIND_REDUNDANT = 1 };
+
+template <int KIND>
+struct LdsWords {
+    static constexpr int value = KIND == KIND_SHARED ? 14 : 6;
+};
+
+// Register-resident step at stride delta = DK*NT: rows k-DK, k, k+DK of the
+// same thread are the neighbours. new[k] from E(k-DK), row k, E(k+DK).
+template <int NT, int RPT, int DK>
+__device__ __forceinline__ void register_step(Row<double> (&r)[RPT], int t, int n)
+{
+    const int delta = DK * NT;
+    Pair<double> E[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int j = t + NT * k;
+        const bool hasL = k - DK >= 0;                 // j - delta >= 0
+        const bool aC = k - 2 * DK >= 0;               // j - 2 delta >= 0
+        const bool hasR = k + DK < RPT && j + delta <= n - 1;
+        const bool arC = k + 2 * DK < RPT && j + 2 * delta <= n - 1;
+        const Row<double>& L = r[hasL ? k - DK : k];
+        const Row<double>& R = r[k + DK < RPT ? k + DK : k];
+        E[k] = make_pair(L, r[k], hasR ? R : r[k], hasL, aC, hasR, arC);
+    }
+    Row<double> N[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = t + NT * k;
+        const bool hp1 = k - DK >= 0, hp2 = k - 2 * DK >= 0;
+        const bool hq1 = k + DK < RPT && i + delta <= n - 1;
+        const bool hq2 = k + 2 * DK < RPT && i + 2 * delta <= n - 1;
+        const Pair<double>& EL = E[hp1 ? k - DK : k];
+        const Pair<double>& ER = E[k + DK < RPT ? k + DK : k];
+        const Pair<double>& ERs = hq1 ? ER : E[k];
+        N[k] = reduce_row(r[k], EL, ERs, hp1, hq1, hp2, hq2,
+                          [&](double& alL, double& arL, double& alR, double& arR) {
+                              alL = EL.al; arL = EL.ar; alR = ERs.al; arR = ERs.ar;
+                          });
+    }
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+        if (t + NT * k < n) r[k] = N[k];
+}
+
+template <int KIND, int NT, int RPT>
 __global__ __launch_bounds__(NT) void lds_f64_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
     int n, int m, long long rs, long long ss, int steps)
 {
     constexpr int NMAX = NT * RPT;
-    __shared__ double sR[6][NMAX];   // current rows a,b,c,d,e,y
-    __shared__ double sE[8][NMAX];   // pair equations A,B,C,D,E,K,al,ar
+    __shared__ double sR[6][NMAX];                               // rows a,b,c,d,e,y
+    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
 
     const int t = threadIdx.x;
     const int sys = xcd_remap(blockIdx.x, m);
@@ -43,61 +94,110 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
             const long long g = base + (long long)i * rs;
             r[k].a = ga[g]; r[k].b = gb[g]; r[k].c = gc[g];
             r[k].d = gd[g]; r[k].e = ge[g]; r[k].y = gy[g];
-            sR[0][i] = r[k].a; sR[1][i] = r[k].b; sR[2][i] = r[k].c;
-            sR[3][i] = r[k].d; sR[4][i] = r[k].e; sR[5][i] = r[k].y;
+        } else {
+            r[k] = Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
         }
     }
-    __syncthreads();
 
+    auto writeR = [&](int i, const Row<double>& q) {
+        sR[0][i] = q.a; sR[1][i] = q.b; sR[2][i] = q.c;
+        sR[3][i] = q.d; sR[4][i] = q.e; sR[5][i] = q.y;
+    };
     auto readR = [&](int i) {
         Row<double> q;
         q.a = sR[0][i]; q.b = sR[1][i]; q.c = sR[2][i];
         q.d = sR[3][i]; q.e = sR[4][i]; q.y = sR[5][i];
         return q;
     };
-    auto readE = [&](int i) {
-        Pair<double> q;
-        q.A = sE[0][i]; q.B = sE[1][i]; q.C = sE[2][i];
-        q.D = sE[3][i]; q.E = sE[4][i]; q.K = sE[5][i];
-        return q;
-    };
 
-    int delta = 1;
-    for (int s = 0; s < steps; ++s, delta <<= 1) {
-        // Reduction-I/II: pair equation centred on each own row j.
+    // Steps with stride < NT exchange rows through LDS.
+    int s = 0, delta = 1;
+    const bool any_lds = steps > 0 && NT > 1;
+    if (any_lds) {
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int j = t + NT * k;
-            if (j < n) {
-                const bool hasL = j - delta >= 0, hasR = j + delta <= n - 1;
-                const bool aC = j - 2 * delta >= 0, arC = j + 2 * delta <= n - 1;
-                const Row<double> L = readR(hasL ? j - delta : j);
-                const Row<double> R = readR(hasR ? j + delta : j);
-                const Pair<double> E = make_pair(L, r[k], R, hasL, aC, hasR, arC);
-                sE[0][j] = E.A; sE[1][j] = E.B; sE[2][j] = E.C; sE[3][j] = E.D;
-                sE[4][j] = E.E; sE[5][j] = E.K; sE[6][j] = E.al; sE[7][j] = E.ar;
-            }
-        }
+        for (int k = 0; k < RPT; ++k)
+            if (t + NT * k < n) writeR(t + NT * k, r[k]);
         __syncthreads();
-        // Reduction-III: new row from E(i-d), row i, E(i+d).
+    }
+    for (; s < steps && delta < NT; ++s, delta <<= 1) {
+        if constexpr (KIND == KIND_SHARED) {
+            // Reduction-I/II once per centre j (kernel.cu:69-117).
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int i = t + NT * k;
-            if (i < n) {
-                const bool hp1 = i - delta >= 0, hq1 = i + delta <= n - 1;
-                const bool hp2 = i - 2 * delta >= 0, hq2 = i + 2 * delta <= n - 1;
-                const int il = hp1 ? i - delta : i, ir = hq1 ? i + delta : i;
-                const Pair<double> EL = readE(il), ER = readE(ir);
-                r[k] = reduce_row(r[k], EL, ER, hp1, hq1, hp2, hq2,
-                                  [&](double& alL, double& arL, double& alR, double& arR) {
-                                      alL = sE[6][il]; arL = sE[7][il];
-                                      alR = sE[6][ir]; arR = sE[7][ir];
-                                  });
-                sR[0][i] = r[k].a; sR[1][i] = r[k].b; sR[2][i] = r[k].c;
-                sR[3][i] = r[k].d; sR[4][i] = r[k].e; sR[5][i] = r[k].y;
+            for (int k = 0; k < RPT; ++k) {
+                const int j = t + NT * k;
+                if (j < n) {
+                    const bool hasL = j - delta >= 0, hasR = j + delta <= n - 1;
+                    const bool aC = j - 2 * delta >= 0, arC = j + 2 * delta <= n - 1;
+                    const Row<double> L = readR(hasL ? j - delta : j);
+                    const Row<double> R = readR(hasR ? j + delta : j);
+                    const Pair<double> E = make_pair(L, r[k], R, hasL, aC, hasR, arC);
+                    sE[0][j] = E.A; sE[1][j] = E.B; sE[2][j] = E.C; sE[3][j] = E.D;
+                    sE[4][j] = E.E; sE[5][j] = E.K; sE[6][j] = E.al; sE[7][j] = E.ar;
+                }
             }
+            __syncthreads();
+            // Reduction-III (kernel.cu:120-140).
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int i = t + NT * k;
+                if (i < n) {
+                    const bool hp1 = i - delta >= 0, hq1 = i + delta <= n - 1;
+                    const bool hp2 = i - 2 * delta >= 0, hq2 = i + 2 * delta <= n - 1;
+                    const int il = hp1 ? i - delta : i, ir = hq1 ? i + delta : i;
+                    Pair<double> EL, ER;
+                    EL.A = sE[0][il]; EL.B = sE[1][il]; EL.C = sE[2][il];
+                    EL.D = sE[3][il]; EL.E = sE[4][il]; EL.K = sE[5][il];
+                    ER.A = sE[0][ir]; ER.B = sE[1][ir]; ER.C = sE[2][ir];
+                    ER.D = sE[3][ir]; ER.E = sE[4][ir]; ER.K = sE[5][ir];
+                    r[k] = reduce_row(r[k], EL, ER, hp1, hq1, hp2, hq2,
+                                      [&](double& alL, double& arL, double& alR, double& arR) {
+                                          alL = sE[6][il]; arL = sE[7][il];
+                                          alR = sE[6][ir]; arR = sE[7][ir];
+                                      });
+                    writeR(i, r[k]);
+                }
+            }
+            __syncthreads();
+        } else {
+            // kernel.cu:66-140 as written: each row builds eq1 and eq2 itself.
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int i = t + NT * k;
+                if (i < n) {
+                    const bool hp1 = i - delta >= 0, hq1 = i + delta <= n - 1;
+                    const bool hp2 = i - 2 * delta >= 0, hq2 = i + 2 * delta <= n - 1;
+                    const bool hp3 = i - 3 * delta >= 0, hq3 = i + 3 * delta <= n - 1;
+                    const Row<double> P1 = readR(hp1 ? i - delta : i);
+                    const Row<double> P2 = readR(hp2 ? i - 2 * delta : i);
+                    const Row<double> Q1 = readR(hq1 ? i + delta : i);
+                    const Row<double> Q2 = readR(hq2 ? i + 2 * delta : i);
+                    // eq1 = E(i-d) from rows i-2d, i-d, i; eq2 = E(i+d) from rows i, i+d, i+2d
+                    const Pair<double> EL = make_pair(P2, P1, r[k], hp2, hp3, true, hq1);
+                    const Pair<double> ER = make_pair(r[k], Q1, Q2, true, hp1, hq2, hq3);
+                    r[k] = reduce_row(r[k], EL, ER, hp1, hq1, hp2, hq2,
+                                      [&](double& alL, double& arL, double& alR, double& arR) {
+                                          alL = EL.al; arL = EL.ar; alR = ER.al; arR = ER.ar;
+                                      });
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (t + NT * k < n) writeR(t + NT * k, r[k]);
+            __syncthreads();
         }
-        __syncthreads();
+    }
+    // Steps with stride >= NT: all neighbours are rows of this thread.
+#pragma unroll
+    for (int dk = 1; dk < RPT; dk <<= 1) {
+        if (s < steps) {
+            if (dk == 1) register_step<NT, RPT, (RPT > 1 ? 1 : 1)>(r, t, n);
+            if constexpr (RPT > 2) if (dk == 2) register_step<NT, RPT, 2>(r, t, n);
+            if constexpr (RPT > 4) if (dk == 4) register_step<NT, RPT, 4>(r, t, n);
+            if constexpr (RPT > 8) if (dk == 8) register_step<NT, RPT, 8>(r, t, n);
+            ++s;
+            delta <<= 1;
+        }
     }
 
     // Final 2x2 / 1x1 solve (kernel.cu:145-153).
@@ -107,8 +207,18 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
         if (i < n) {
             const long long g = base + (long long)i * rs;
             if (i < n - delta) {
-                const int ii = i + delta;
-                const double cii = sR[2][ii], bii = sR[1][ii], yii = sR[5][ii];
+                double cii, bii, yii;
+                if (delta >= NT) {
+                    // partner is row k + delta/NT of this thread
+                    const int kk = k + delta / NT;
+                    Row<double> P = r[0];
+#pragma unroll
+                    for (int q = 1; q < RPT; ++q)
+                        if (q == kk) P = r[q];
+                    cii = P.c; bii = P.b; yii = P.y;
+                } else {
+                    cii = sR[2][i + delta]; bii = sR[1][i + delta]; yii = sR[5][i + delta];
+                }
                 const double deno = __builtin_fma(r[k].c, cii, -(bii * r[k].d));
                 gy[g] = __builtin_fma(cii, r[k].y, -(r[k].d * yii)) / deno;
                 gy[g + (long long)delta * rs] = __builtin_fma(r[k].c, yii, -(bii * r[k].y)) / deno;
@@ -119,26 +229,64 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
     }
 }
 
-template <int NT, int RPT>
-static int launch_lds_f64(const SolveArgsF64& A, hipStream_t st)
+// ---- variant table ---------------------------------------------------------
+
+template <int KIND, int NT, int RPT>
+static int launch(const SolveArgsF64& A, hipStream_t st)
 {
-    hipLaunchKernelGGL((lds_f64_kernel<NT, RPT>), dim3(A.m), dim3(NT), 0, st,
+    hipLaunchKernelGGL((lds_f64_kernel<KIND, NT, RPT>), dim3(A.m), dim3(NT), 0, st,
                        A.a, A.b, A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.steps);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+#define PCRP_V(KIND, NT, RPT) {KIND, NT, RPT, NT * RPT, &launch<KIND, NT, RPT>}
+
+static const LdsVariantF64 kVariants[] = {
+    // shared pair equations, 14 doubles of LDS per row
+    PCRP_V(KIND_SHARED, 64, 1),   PCRP_V(KIND_SHARED, 128, 1),  PCRP_V(KIND_SHARED, 256, 1),
+    PCRP_V(KIND_SHARED, 512, 1),  PCRP_V(KIND_SHARED, 1024, 1), PCRP_V(KIND_SHARED, 64, 2),
+    PCRP_V(KIND_SHARED, 128, 2),  PCRP_V(KIND_SHARED, 256, 2),  PCRP_V(KIND_SHARED, 512, 2),
+    PCRP_V(KIND_SHARED, 64, 4),   PCRP_V(KIND_SHARED, 128, 4),  PCRP_V(KIND_SHARED, 256, 4),
+    // reference-style redundant pairs, 6 doubles of LDS per row
+    PCRP_V(KIND_REDUNDANT, 64, 1),   PCRP_V(KIND_REDUNDANT, 128, 1),
+    PCRP_V(KIND_REDUNDANT, 256, 1),  PCRP_V(KIND_REDUNDANT, 512, 1),
+    PCRP_V(KIND_REDUNDANT, 1024, 1), PCRP_V(KIND_REDUNDANT, 64, 2),
+    PCRP_V(KIND_REDUNDANT, 128, 2),  PCRP_V(KIND_REDUNDANT, 256, 2),
+    PCRP_V(KIND_REDUNDANT, 512, 2),  PCRP_V(KIND_REDUNDANT, 64, 4),
+    PCRP_V(KIND_REDUNDANT, 128, 4),  PCRP_V(KIND_REDUNDANT, 256, 4),
+};
+
+#undef PCRP_V
+
+int lds_f64_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
+
+const LdsVariantF64* lds_f64_variant(int id)
+{
+    return (id >= 0 && id < lds_f64_variant_count()) ? &kVariants[id] : nullptr;
+}
+
 int lds_f64_max_n() { return 1024; }
 
-// Workgroup shape per size class (NT threads x RPT rows per thread).
-int lds_f64_solve(const SolveArgsF64& A, hipStream_t st)
+int lds_f64_lds_bytes(const LdsVariantF64& v)
 {
-    const int n = A.n;
-    if (n <= 64) return launch_lds_f64<64, 1>(A, st);
-    if (n <= 128) return launch_lds_f64<128, 1>(A, st);
-    if (n <= 256) return launch_lds_f64<256, 1>(A, st);
-    if (n <= 512) return launch_lds_f64<512, 1>(A, st);
-    if (n <= 1024) return launch_lds_f64<1024, 1>(A, st);
-    return -2;
+    return 8 * v.max_n * (v.kind == KIND_SHARED ? LdsWords<KIND_SHARED>::value
+                                                : LdsWords<KIND_REDUNDANT>::value);
+}
+
+// Default choice before tuning: the smallest workgroup of 256 threads or
+// fewer (RPT up to 4) whose capacity covers n, redundant form.
+int lds_f64_default_variant(int n)
+{
+    int best = -1;
+    for (int id = 0; id < lds_f64_variant_count(); ++id) {
+        const LdsVariantF64& v = kVariants[id];
+        if (v.kind != KIND_REDUNDANT || v.max_n < n) continue;
+        if (best < 0) { best = id; continue; }
+        const LdsVariantF64& b = kVariants[best];
+        // prefer fewer total rows (tighter fit), then more threads
+        if (v.max_n < b.max_n || (v.max_n == b.max_n && v.nt > b.nt && v.nt <= 256)) best = id;
+    }
+    return best;
 }
 
 }  // namespace pcrp

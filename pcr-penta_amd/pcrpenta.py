@@ -59,6 +59,22 @@ def _declare(lib):
     lib.pcrp_generate.restype = ctypes.c_int
     lib.pcrp_init_t2.argtypes = [ctypes.c_int] * 3 + [_vp] * 6 + [_vp]
     lib.pcrp_init_t2.restype = ctypes.c_int
+    lib.pcrp_lds_variant_count.argtypes = [ctypes.c_int]
+    lib.pcrp_lds_variant_count.restype = ctypes.c_int
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.pcrp_lds_variant_info.argtypes = [ctypes.c_int, ctypes.c_int] + [ip] * 5
+    lib.pcrp_lds_variant_info.restype = ctypes.c_int
+    lib.pcrp_tune_set.argtypes = [ctypes.c_int] * 3
+    lib.pcrp_tune_set.restype = ctypes.c_int
+    lib.pcrp_tune_get.argtypes = [ctypes.c_int] * 2
+    lib.pcrp_tune_get.restype = ctypes.c_int
+    fp = ctypes.POINTER(ctypes.c_float)
+    lib.pcrp_autotune.argtypes = [ctypes.c_int] * 4 + [_vp] * 7 + [ctypes.c_int, _vp, ip, fp, fp]
+    lib.pcrp_autotune.restype = ctypes.c_int
+    lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
+    lib.pcrp_tune_save.restype = ctypes.c_int
+    lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
+    lib.pcrp_tune_load.restype = ctypes.c_int
     return lib
 
 
@@ -160,3 +176,53 @@ def init_t2(n: int, m: int, layout: int = INTERLEAVED, device="cuda", stream=Non
     st = load().pcrp_init_t2(n, m, layout, *[_ptr(t) for t in arrs], _stream(stream))
     _check(st, "pcrp_init_t2")
     return arrs
+
+
+# ---- autotuner -------------------------------------------------------------
+
+def variants(precision: int = PREC_F64):
+    """List of dicts describing the compiled LDS-kernel variants."""
+    lib = load()
+    out = []
+    for vid in range(lib.pcrp_lds_variant_count(precision)):
+        vals = [ctypes.c_int() for _ in range(5)]
+        _check(lib.pcrp_lds_variant_info(precision, vid, *[ctypes.byref(v) for v in vals]),
+               "pcrp_lds_variant_info")
+        kind, nt, rpt, max_n, lds = (v.value for v in vals)
+        out.append({"id": vid, "kind": ("shared", "redundant")[kind], "nt": nt, "rpt": rpt,
+                    "max_n": max_n, "lds_bytes": lds})
+    return out
+
+
+def tune_set(n: int, variant_id: int, precision: int = PREC_F64):
+    _check(load().pcrp_tune_set(precision, n, variant_id), "pcrp_tune_set")
+
+
+def tune_get(n: int, precision: int = PREC_F64) -> int:
+    return load().pcrp_tune_get(precision, n)
+
+
+def autotune(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, reps: int = 3,
+             precision: int = PREC_F64, stream=None):
+    """Time every fitting variant on this batch; returns (best_id, best_ms, times_ms)."""
+    import torch
+    _validate([a, b, c, d, e, y], n, m)
+    lib = load()
+    work = torch.empty_like(y)
+    nv = lib.pcrp_lds_variant_count(precision)
+    times = (ctypes.c_float * max(nv, 1))()
+    best = ctypes.c_int(-1)
+    best_ms = ctypes.c_float(0)
+    st = lib.pcrp_autotune(precision, n, m, layout, *[_ptr(t) for t in (a, b, c, d, e, y)],
+                           _ptr(work), reps, _stream(stream), ctypes.byref(best),
+                           ctypes.byref(best_ms), times)
+    _check(st, "pcrp_autotune")
+    return best.value, best_ms.value, [times[i] for i in range(nv)]
+
+
+def tune_save(path: str):
+    _check(load().pcrp_tune_save(path.encode()), "pcrp_tune_save")
+
+
+def tune_load(path: str):
+    _check(load().pcrp_tune_load(path.encode()), "pcrp_tune_load")

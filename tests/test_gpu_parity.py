@@ -107,3 +107,51 @@ def test_zero_pivot_rule(gpu, oracle_lib):
     x = _run_f64(gpu, dev, n, m, C)
     ref = oracle_lib.pcr_batch(*arrs, n, m, C)
     assert np.array_equal(np.nan_to_num(x, nan=7.0), np.nan_to_num(ref, nan=7.0))
+
+
+def test_every_lds_variant_bit_exact(gpu, oracle_lib):
+    """Every compiled LDS variant (shared/redundant pair equations, NT threads
+    x RPT rows, register-resident large strides) gives the oracle's bits."""
+    import torch
+    vs = gpu.variants()
+    assert len(vs) >= 10
+    try:
+        for n, layout in ((1, I), (5, C), (64, I), (100, I), (255, C), (256, I), (300, I),
+                          (512, I), (513, C), (777, I), (1024, I), (1024, C)):
+            m = 21
+            _say("variants", n, layout)
+            dev = gpu.generate(0, 900 + n, n, m, layout)
+            ins = [_np(t).copy() for t in dev]
+            ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+            for v in vs:
+                if v["max_n"] < n:
+                    continue
+                gpu.tune_set(n, v["id"])
+                assert gpu.tune_get(n) == v["id"]
+                y = torch.from_numpy(ins[5]).cuda()
+                gpu.solve(*dev[:5], y, n, m, layout)
+                torch.cuda.synchronize()
+                assert np.array_equal(_np(y), ref), (n, layout, v)
+    finally:
+        for n in (1, 5, 64, 100, 255, 256, 300, 512, 513, 777, 1024):
+            gpu.tune_set(n, -1)
+
+
+def test_autotune_picks_a_fitting_variant(gpu, oracle_lib, tmp_path):
+    import torch
+    n, m = 300, 64
+    _say("autotune", n, m)
+    dev = gpu.generate(0, 1, n, m, I)
+    y0 = _np(dev[5]).copy()
+    best, ms, times = gpu.autotune(*dev, n, m, I, reps=2)
+    assert best >= 0 and ms > 0 and times[best] == ms
+    assert gpu.variants()[best]["max_n"] >= n
+    assert gpu.tune_get(n) == best
+    assert np.array_equal(_np(dev[5]), y0)   # autotune leaves y untouched
+    path = str(tmp_path / "tune.txt")
+    gpu.tune_save(path)
+    gpu.tune_set(n, -1)
+    gpu.tune_load(path)
+    assert gpu.tune_get(n) == best
+    gpu.tune_set(n, -1)
+    torch.cuda.synchronize()
