@@ -43,6 +43,9 @@ def parse():
                     help="systems in the CPU-baseline sample")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner; use the built-in default variant")
+    ap.add_argument("--tune-save", default=None, help="write the tuning table to this file")
+    ap.add_argument("--tune-load", default=None,
+                    help="load a tuning table (implies --no-tune)")
     return ap.parse_args()
 
 
@@ -101,7 +104,9 @@ def main():
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
     tuned = None
-    if not args.no_tune:
+    if args.tune_load:
+        pcrpenta.tune_load(args.tune_load)
+    elif not args.no_tune:
         # the autotuner times every LDS variant on this very batch (outside the timed region)
         best, best_ms, times = pcrpenta.autotune(a, b, c, d, e, y0, n, m, layout, reps=3)
         vs = pcrpenta.variants()
@@ -109,6 +114,8 @@ def main():
                  "all_ms": {f"{v['kind']}-{v['nt']}x{v['rpt']}": round(t, 4)
                             for v, t in zip(vs, times) if t >= 0}}
         print(json.dumps({"autotune": tuned}), file=sys.stderr, flush=True)
+        if args.tune_save:
+            pcrpenta.tune_save(args.tune_save)
     variant = pcrpenta.variants()[pcrpenta.tune_get(n)]
 
     def step(k):
