@@ -77,11 +77,15 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
 
 /*
  * FP32 PCR + FP64 iterative refinement (no counterpart in the reference,
- * which is FP64-only). x0 = PCR32(y); then up to max_sweeps times:
- * r = y - A x (FP64), dx = PCR32(r), x += dx (FP64), until
- * ||dx||_inf <= tol * ||x||_inf for that system. Inputs and output are FP64.
- * sweeps_out: optional DEVICE array of m ints receiving each system's sweep
- * count (max_sweeps + 1 flags a system that did not meet tol).
+ * which is FP64-only). The FP32 factorisation is computed once per system;
+ * x0 = PCR32(y); then up to max_sweeps times: r = y - A x (FP64),
+ * dx = PCR32(r), x += dx (FP64). A system stops when ||dx_k|| <= tol ||x||,
+ * or when the error left after the update, estimated from the observed
+ * contraction as ||dx_k||^2 / ||dx_{k-1}||, is <= tol ||x|| (and the
+ * contraction is <= 0.1). Norms are max-norms. Inputs and output are FP64.
+ * sweeps_out: optional DEVICE array of m ints receiving each system's number
+ * of refinement sweeps (max_sweeps + 1 flags a system that did not stop).
+ * Sizes n <= 2048 run in one LDS-resident kernel per system.
  */
 int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
                         const double *d, const double *e, double *y,

@@ -147,13 +147,15 @@ int pcrp_solve_f32r_f64(const double* a, const double* b, const double* c, const
                         const double* e, double* y, int n, int m, int layout, int max_sweeps,
                         double tol, int* sweeps_out, void* ws, size_t ws_bytes, void* stream)
 {
-    (void)sweeps_out; (void)ws; (void)ws_bytes; (void)stream;
+    (void)ws; (void)ws_bytes;
     int st = check_shape(n, m);
     if (st != PCRP_OK) return st;
     if (!a || !b || !c || !d || !e || !y) return PCRP_ERR_NULL;
     if (max_sweeps < 0 || !(tol >= 0.0)) return PCRP_ERR_ARG;
-    long long rs, ss;
-    if (!layout_strides(n, m, layout, &rs, &ss)) return PCRP_ERR_LAYOUT;
+    SolveArgsF32R A{a, b, c, d, e, y, n, m, 0, 0, pcr_steps(n), max_sweeps, tol, sweeps_out};
+    if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n <= lds_f32r_max_n()) return lds_f32r_solve(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
     return PCRP_ERR_UNSUPPORTED;
 }
 

@@ -30,6 +30,21 @@ int lds_f64_default_variant(int n);
 int lds_f64_lds_bytes(const LdsVariantF64& v);
 int lds_f64_max_n();
 
+// FP32 PCR + FP64 refinement over the same strided view.
+struct SolveArgsF32R {
+    const double *a, *b, *c, *d, *e;
+    double* y;
+    int n, m;
+    long long rs, ss;
+    int steps;
+    int max_sweeps;
+    double tol;
+    int* sweeps_out;
+};
+
+int lds_f32r_max_n();
+int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);
+
 inline int pcr_steps(long long n)
 {
     int s = 0;

@@ -155,3 +155,57 @@ def test_autotune_picks_a_fitting_variant(gpu, oracle_lib, tmp_path):
     assert gpu.tune_get(n) == best
     gpu.tune_set(n, -1)
     torch.cuda.synchronize()
+
+
+# ---------------- FP32 PCR + FP64 refinement ----------------
+
+def _refined(gpu, dev, n, m, layout, max_sweeps=8, tol=1e-12):
+    import torch
+    sweeps = torch.full((m,), -1, dtype=torch.int32, device="cuda")
+    a, b, c, d, e, y = dev
+    gpu.solve_refined(a, b, c, d, e, y, n, m, layout, max_sweeps=max_sweeps, tol=tol, sweeps=sweeps)
+    torch.cuda.synchronize()
+    return _np(y), _np(sweeps)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 33, 100, 255, 256, 512, 700, 1000, 1024, 1500, 2048])
+def test_refined_meets_1e12_vs_dgbsv(gpu, oracle_lib, n):
+    """FP32 PCR + FP64 refinement: normwise relative error <= 1e-12 against the
+    FP64 banded-LU (dgbsv) oracle, both generators, both layouts."""
+    m = 24
+    for kind, layout in ((0, I), (1, C)):
+        _say("refined", n, m, kind, layout)
+        dev = gpu.generate(kind, 300 + n, n, m, layout)
+        ins = [_np(t).copy() for t in dev]
+        x, sw = _refined(gpu, dev, n, m, layout)
+        ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+        err = oracle_lib.normwise_relerr(x, ref, n, m, layout)
+        assert err <= 1e-12, (n, kind, layout, err, sw)
+        assert sw.min() >= 0 and sw.max() <= 8, sw
+        for t, h in zip(dev[:5], ins[:5]):      # a..e untouched
+            assert np.array_equal(_np(t), h)
+
+
+def test_refined_reports_nonconvergence(gpu, oracle_lib):
+    """max_sweeps = 0 stops after the bare FP32 solve and flags it (sweeps = 1)."""
+    n, m = 512, 8
+    _say("refined-flag", n, m)
+    dev = gpu.generate(0, 5, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    x, sw = _refined(gpu, dev, n, m, I, max_sweeps=0, tol=1e-15)
+    assert (sw == 1).all(), sw
+    ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+    err = oracle_lib.normwise_relerr(x, ref, n, m, I)
+    assert 1e-9 < err < 1e-1, err   # a plain FP32 PCR solve
+
+
+def test_refined_config1_shape(gpu, oracle_lib):
+    """Config-1 FP32+refine shape (n = 2048) on a slice of the batch."""
+    n, m = 2048, 256
+    _say("refined-c1", n, m)
+    dev = gpu.generate(0, 0, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    x, sw = _refined(gpu, dev, n, m, I)
+    ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+    assert oracle_lib.normwise_relerr(x, ref, n, m, I) <= 1e-12
+    print("sweeps histogram", np.bincount(sw), flush=True)
