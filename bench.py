@@ -27,9 +27,14 @@ HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_UNKNOWN = 56     # 7 FP64 words
 
 WORKLOADS = {
-    # name: (n, m, precision, generator kind)
+    # name: (n, m, precision, generator kind); kind 0 = strictly dominant
+    # (config 0/1 distribution), kind 1 = weakly dominant (config 2)
     "lds_f64_m16384_n1024": (1024, 16384, "f64", 0),
+    "lds_f32r_m16384_n2048": (2048, 16384, "f32r", 0),
+    "lds_f64_m256_n512": (512, 256, "f64", 0),
+    "hbm_f64_m32_n1048576": (1 << 20, 32, "f64", 1),
 }
+MAX_SWEEPS, TOL = 8, 1e-12
 
 
 def parse():
@@ -49,9 +54,11 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys):
+def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys, prec):
     """The oracle's FP64 PCR-Penta over the first `nsys` systems of the same
-    workload, all host threads (OpenMP), median of 3 after one warm-up."""
+    workload, all host threads (OpenMP), median of 3 after one warm-up; also
+    checks the GPU's result on those systems (bit-identical for FP64, error
+    against the FP64 banded-LU oracle for FP32+refine)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle
@@ -65,9 +72,11 @@ def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys):
         x = oracle.pcr_batch(*arrs, n, nsys, oracle.INTERLEAVED, threads)
         times.append(time.perf_counter() - t0)
     t = sorted(times[1:])[1]
-    agree = None
+    agree = err = None
     if gpu_x_sample is not None:
         agree = bool(np.array_equal(x.reshape(n, nsys), gpu_x_sample))
+        xg = oracle.gbsv_batch(*arrs, n, nsys, oracle.INTERLEAVED)
+        err = oracle.normwise_relerr(gpu_x_sample.reshape(-1), xg, n, nsys, oracle.INTERLEAVED)
     return {
         "value": n * nsys / t,
         "unit": "unknowns/s",
@@ -77,6 +86,7 @@ def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys):
                   f"restated from kernel.cu:32-154, gcc -O2 + OpenMP over systems, median of 3",
         "seconds": t,
         "bit_identical_to_gpu": agree,
+        "gpu_relerr_vs_dgbsv": err,
     }
 
 
@@ -100,13 +110,15 @@ def main():
     nbuf = max(1, min(K + W, int(8e9 // (8 * n * m))))
     ys = [y0.clone() for _ in range(nbuf)]
     y_check = y0.clone()
-    ws, nb = pcrpenta._workspace(n, m, pcrpenta.PREC_F64, dev)
+    pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
+    ws, nb = pcrpenta._workspace(n, m, pid, dev)
+    sweeps = torch.zeros(m, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
     tuned = None
     if args.tune_load:
         pcrpenta.tune_load(args.tune_load)
-    elif not args.no_tune:
+    elif not args.no_tune and prec == "f64" and n <= 1024:
         # the autotuner times every LDS variant on this very batch (outside the timed region)
         best, best_ms, times = pcrpenta.autotune(a, b, c, d, e, y0, n, m, layout, reps=3)
         vs = pcrpenta.variants()
@@ -116,10 +128,15 @@ def main():
         print(json.dumps({"autotune": tuned}), file=sys.stderr, flush=True)
         if args.tune_save:
             pcrpenta.tune_save(args.tune_save)
-    variant = pcrpenta.variants()[pcrpenta.tune_get(n)]
+    vid = pcrpenta.tune_get(min(n, 1024)) if prec == "f64" else -1
+    variant = pcrpenta.variants()[vid] if vid >= 0 else None
 
-    def step(k):
-        return pcrpenta.solve(a, b, c, d, e, ys[k % nbuf], n, m, layout, stream=stream, ws=ws)
+    def step(k, y=None):
+        y = ys[k % nbuf] if y is None else y
+        if prec == "f64":
+            return pcrpenta.solve(a, b, c, d, e, y, n, m, layout, stream=stream, ws=ws)
+        return pcrpenta.solve_refined(a, b, c, d, e, y, n, m, layout, max_sweeps=MAX_SWEEPS,
+                                      tol=TOL, sweeps=sweeps, stream=stream, ws=ws)
 
     for w in range(W):
         step(w)
@@ -140,8 +157,11 @@ def main():
     value = unknowns / (wall / K)
 
     # one more solve on a pristine RHS for the CPU-baseline agreement check
-    pcrpenta.solve(a, b, c, d, e, y_check, n, m, layout, stream=stream, ws=ws)
+    step(0, y_check)
     torch.cuda.synchronize()
+    sweep_hist = None
+    if prec != "f64":
+        sweep_hist = torch.bincount(sweeps.long()).tolist()
 
     achieved = BYTES_PER_UNKNOWN * unknowns / avg_kernel_s
     traffic = None
@@ -161,10 +181,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": prec,
-        "data": "synthetic (seeded counter-based generator, config-0 distribution)",
+        "data": "synthetic (seeded counter-based generator, "
+                + ("config-0 distribution)" if kind == 0 else "config-2 weakly dominant distribution)"),
         "config": {"workload": args.workload, "n": n, "m": m, "layout": "interleaved",
-                   "precision": prec, "seed": seed, "generator": "dominant",
-                   "kernel_variant": f"{variant['kind']}-{variant['nt']}x{variant['rpt']}"},
+                   "precision": prec, "seed": seed,
+                   "generator": "dominant" if kind == 0 else "weak",
+                   "kernel_variant": (f"{variant['kind']}-{variant['nt']}x{variant['rpt']}"
+                                      if variant else None),
+                   "split": list(pcrpenta.split_tune_get()[:3]) if n > 1024 else None,
+                   "refine": ({"max_sweeps": MAX_SWEEPS, "tol": TOL, "sweeps_hist": sweep_hist}
+                              if prec != "f64" else None)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel_ms": avg_kernel_s * 1e3},
@@ -172,7 +198,7 @@ def main():
     if not args.no_cpu:
         nsys = min(args.cpu_systems, m)
         xs = y_check.view(n, m)[:, :nsys].cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(n, m, layout, kind, seed, xs, nsys)
+        out["cpu_baseline"] = cpu_baseline(n, m, layout, kind, seed, xs, nsys, prec)
     print(json.dumps(out), flush=True)
 
 

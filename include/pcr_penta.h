@@ -131,6 +131,15 @@ int pcrp_autotune(int precision, int n, int m, int layout,
                   const double *d, const double *e, const double *y,
                   double *y_work, int reps, void *stream,
                   int *best_id, float *best_ms, float *times_ms);
+/* Split path (n above the LDS threshold): sub-systems of at most lds_max rows
+ * (a power of two <= 1024) go to the LDS kernel, HBM passes fuse up to
+ * k_fuse (1..6) reduction steps each, pass tile shape `shape` (0 ..
+ * n_shapes-1). The reference fixes these at 1024, 1 and 512-row tiles
+ * (main.cu:16-22, kernel.cu:182). Changing lds_max changes
+ * pcrp_workspace_size(). */
+int pcrp_split_tune_set(int precision, int lds_max, int k_fuse, int shape);
+int pcrp_split_tune_get(int precision, int *lds_max, int *k_fuse, int *shape,
+                        int *n_shapes);
 /* Persist / restore the table as text (host file). */
 int pcrp_tune_save(const char *path);
 int pcrp_tune_load(const char *path);

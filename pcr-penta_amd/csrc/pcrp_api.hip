@@ -88,7 +88,10 @@ int bucket_of(int n)
     return b;
 }
 
-int tuned_variant(int prec, int n)
+}  // namespace
+
+namespace pcrp {
+int lds_tuned_variant(int prec, int n)
 {
     const int b = bucket_of(n);
     if (b < kBuckets && g_tuned[prec][b] >= 0) {
@@ -97,8 +100,7 @@ int tuned_variant(int prec, int n)
     }
     return lds_f64_default_variant(n);
 }
-
-}  // namespace
+}  // namespace pcrp
 
 extern "C" {
 
@@ -121,7 +123,8 @@ const char* pcrp_strerror(int status)
 
 size_t pcrp_workspace_size(int n, int m, int precision)
 {
-    (void)n; (void)m; (void)precision;
+    if (check_shape(n, m) != PCRP_OK) return 0;
+    if (precision == PCRP_PREC_F64) return split_f64_workspace(n, m);
     return 0;
 }
 
@@ -129,18 +132,19 @@ int pcrp_solve_f64(const double* a, const double* b, const double* c, const doub
                    const double* e, double* y, int n, int m, int layout, void* ws,
                    size_t ws_bytes, void* stream)
 {
-    (void)ws; (void)ws_bytes;
     int st = check_shape(n, m);
     if (st != PCRP_OK) return st;
     if (!a || !b || !c || !d || !e || !y) return PCRP_ERR_NULL;
-    SolveArgsF64 A{a, b, c, d, e, y, n, m, 0, 0, pcr_steps(n)};
+    SolveArgsF64 A{a, b, c, d, e, y, n, m, 0, 0, pcr_steps(n), SplitInfo{0, n, m, layout, y}};
     if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (n <= lds_f64_max_n()) {
-        const LdsVariantF64* v = lds_f64_variant(tuned_variant(PCRP_PREC_F64, n));
+    if (split_f64_levels(n, split_f64_get().lds_max) == 0) {
+        const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, n));
         return v->launch(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
     }
-    return PCRP_ERR_UNSUPPORTED;
+    if (!ws || ws_bytes < split_f64_workspace(n, m)) return PCRP_ERR_WORKSPACE;
+    const int rc = split_f64_solve(A, layout, ws, ws_bytes, s);
+    return rc == 0 ? PCRP_OK : (rc == -3 ? PCRP_ERR_WORKSPACE : PCRP_ERR_HIP);
 }
 
 int pcrp_solve_f32r_f64(const double* a, const double* b, const double* c, const double* d,
@@ -194,7 +198,7 @@ int pcrp_tune_set(int precision, int n, int variant_id)
 int pcrp_tune_get(int precision, int n)
 {
     if (precision != PCRP_PREC_F64 || n < 1 || n > lds_f64_max_n()) return -1;
-    return tuned_variant(precision, n);
+    return lds_tuned_variant(precision, n);
 }
 
 int pcrp_autotune(int precision, int n, int m, int layout, const double* a, const double* b,
@@ -207,7 +211,7 @@ int pcrp_autotune(int precision, int n, int m, int layout, const double* a, cons
     if (!a || !b || !c || !d || !e || !y || !y_work) return PCRP_ERR_NULL;
     if (precision != PCRP_PREC_F64 || reps < 1) return PCRP_ERR_ARG;
     if (n > lds_f64_max_n()) return PCRP_ERR_UNSUPPORTED;
-    SolveArgsF64 A{a, b, c, d, e, y_work, n, m, 0, 0, pcr_steps(n)};
+    SolveArgsF64 A{a, b, c, d, e, y_work, n, m, 0, 0, pcr_steps(n), SplitInfo{0, n, m, layout, y_work}};
     if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t e0, e1;
@@ -242,6 +246,23 @@ int pcrp_autotune(int precision, int n, int m, int layout, const double* a, cons
     g_tuned[precision][bucket_of(n)] = best;
     if (best_id) *best_id = best;
     if (best_ms) *best_ms = bt;
+    return PCRP_OK;
+}
+
+int pcrp_split_tune_set(int precision, int lds_max, int k_fuse, int shape)
+{
+    if (precision != PCRP_PREC_F64) return PCRP_ERR_ARG;
+    return split_f64_set(SplitTune{lds_max, k_fuse, shape}) == 0 ? PCRP_OK : PCRP_ERR_ARG;
+}
+
+int pcrp_split_tune_get(int precision, int* lds_max, int* k_fuse, int* shape, int* n_shapes)
+{
+    if (precision != PCRP_PREC_F64) return PCRP_ERR_ARG;
+    const SplitTune t = split_f64_get();
+    if (lds_max) *lds_max = t.lds_max;
+    if (k_fuse) *k_fuse = t.k_fuse;
+    if (shape) *shape = t.shape;
+    if (n_shapes) *n_shapes = split_f64_shape_count();
     return PCRP_OK;
 }
 

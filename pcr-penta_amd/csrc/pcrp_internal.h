@@ -9,12 +9,24 @@ namespace pcrp {
 
 // One batched FP64 solve over a strided view: element i of system j at
 // a[j*ss + i*rs]; `steps` reduction steps, final delta = 2^steps.
+// Split-path view of the final LDS stage: the batch is the m0 * 2^S residue
+// sub-systems of m0 original systems of size n0, stored interleaved in a
+// workspace (sub-system q = R*m0 + j holds rows R + 2^S*l of system j, element
+// l at q + l*m); x goes back to the caller's y in the caller's layout.
+struct SplitInfo {
+    int S;            // steps already done in HBM passes (0: not a split solve)
+    int n0, m0;       // original system size and count
+    int out_layout;   // caller's layout (PCRP_LAYOUT_*) for x
+    double* x;        // caller's y (receives x)
+};
+
 struct SolveArgsF64 {
     const double *a, *b, *c, *d, *e;
     double* y;
     int n, m;
     long long rs, ss;
     int steps;
+    SplitInfo split;
 };
 
 // A compiled LDS-kernel variant: KIND (0 shared pair equations, 1 redundant),
@@ -41,6 +53,21 @@ struct SolveArgsF32R {
     double tol;
     int* sweeps_out;
 };
+
+// Variant the tuning table assigns to an LDS solve of size n (pcrp_api.hip).
+int lds_tuned_variant(int prec, int n);
+
+// Split path tunables: sub-systems up to lds_max rows go to the LDS kernel;
+// HBM passes fuse up to k_fuse steps; shape indexes the pass tile shapes.
+struct SplitTune {
+    int lds_max, k_fuse, shape;
+};
+SplitTune split_f64_get();
+int split_f64_set(const SplitTune& t);
+int split_f64_shape_count();
+int split_f64_levels(int n, int lds_max);
+size_t split_f64_workspace(int n, int m);
+int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes, hipStream_t st);
 
 int lds_f32r_max_n();
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);

@@ -76,7 +76,7 @@ template <int KIND, int NT, int RPT>
 __global__ __launch_bounds__(NT) void lds_f64_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
-    int n, int m, long long rs, long long ss, int steps)
+    int n, int m, long long rs, long long ss, int steps, SplitInfo split)
 {
     constexpr int NMAX = NT * RPT;
     __shared__ double sR[6][NMAX];                               // rows a,b,c,d,e,y
@@ -85,6 +85,19 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
     const int t = threadIdx.x;
     const int sys = xcd_remap(blockIdx.x, m);
     const long long base = (long long)sys * ss;
+    // where x goes: in place, or (split path) back to the caller's layout
+    double* __restrict__ gx = gy;
+    long long xbase = base, xrs = rs;
+    if (split.S > 0) {
+        // sub-system sys = R*m0 + j: rows R + 2^S*l of original system j
+        const int R = sys / split.m0, j = sys - R * split.m0;
+        n = (split.n0 - R + (1 << split.S) - 1) >> split.S;
+        gx = split.x;
+        if (split.out_layout == 0) {   // contiguous j*n0 + i
+            xbase = (long long)j * split.n0 + R;
+            xrs = 1LL << split.S;
+        }
+    }
 
     Row<double> r[RPT];
 #pragma unroll
@@ -205,7 +218,7 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
     for (int k = 0; k < RPT; ++k) {
         const int i = t + NT * k;
         if (i < n) {
-            const long long g = base + (long long)i * rs;
+            const long long g = xbase + (long long)i * xrs;
             if (i < n - delta) {
                 double cii, bii, yii;
                 if (delta >= NT) {
@@ -220,10 +233,10 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
                     cii = sR[2][i + delta]; bii = sR[1][i + delta]; yii = sR[5][i + delta];
                 }
                 const double deno = __builtin_fma(r[k].c, cii, -(bii * r[k].d));
-                gy[g] = __builtin_fma(cii, r[k].y, -(r[k].d * yii)) / deno;
-                gy[g + (long long)delta * rs] = __builtin_fma(r[k].c, yii, -(bii * r[k].y)) / deno;
+                gx[g] = __builtin_fma(cii, r[k].y, -(r[k].d * yii)) / deno;
+                gx[g + (long long)delta * xrs] = __builtin_fma(r[k].c, yii, -(bii * r[k].y)) / deno;
             } else if (i < delta) {
-                gy[g] = r[k].y / r[k].c;
+                gx[g] = r[k].y / r[k].c;
             }
         }
     }
@@ -235,7 +248,7 @@ template <int KIND, int NT, int RPT>
 static int launch(const SolveArgsF64& A, hipStream_t st)
 {
     hipLaunchKernelGGL((lds_f64_kernel<KIND, NT, RPT>), dim3(A.m), dim3(NT), 0, st,
-                       A.a, A.b, A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.steps);
+                       A.a, A.b, A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.steps, A.split);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -71,6 +71,10 @@ def _declare(lib):
     fp = ctypes.POINTER(ctypes.c_float)
     lib.pcrp_autotune.argtypes = [ctypes.c_int] * 4 + [_vp] * 7 + [ctypes.c_int, _vp, ip, fp, fp]
     lib.pcrp_autotune.restype = ctypes.c_int
+    lib.pcrp_split_tune_set.argtypes = [ctypes.c_int] * 4
+    lib.pcrp_split_tune_set.restype = ctypes.c_int
+    lib.pcrp_split_tune_get.argtypes = [ctypes.c_int] + [ip] * 4
+    lib.pcrp_split_tune_get.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -226,3 +230,15 @@ def tune_save(path: str):
 
 def tune_load(path: str):
     _check(load().pcrp_tune_load(path.encode()), "pcrp_tune_load")
+
+
+def split_tune_get(precision: int = PREC_F64):
+    """(lds_max, k_fuse, shape, n_shapes) of the split path."""
+    vals = [ctypes.c_int() for _ in range(4)]
+    _check(load().pcrp_split_tune_get(precision, *[ctypes.byref(v) for v in vals]),
+           "pcrp_split_tune_get")
+    return tuple(v.value for v in vals)
+
+
+def split_tune_set(lds_max: int, k_fuse: int, shape: int, precision: int = PREC_F64):
+    _check(load().pcrp_split_tune_set(precision, lds_max, k_fuse, shape), "pcrp_split_tune_set")

@@ -209,3 +209,62 @@ def test_refined_config1_shape(gpu, oracle_lib):
     ref = oracle_lib.gbsv_batch(*ins, n, m, I)
     assert oracle_lib.normwise_relerr(x, ref, n, m, I) <= 1e-12
     print("sweeps histogram", np.bincount(sw), flush=True)
+
+
+# ---------------- HBM split path (n above the LDS threshold) ----------------
+
+@pytest.mark.parametrize("n", [1025, 1500, 2048, 2049, 3000, 4096, 5000, 8191, 16384])
+def test_split_path_bit_exact(gpu, oracle_lib, n):
+    """n > 1024: fused HBM passes + LDS stage give the unsplit oracle's bits
+    (kernel.cu:158-320 + 665-791 are the same arithmetic as kernel.cu:32-154)."""
+    m = 6
+    for layout in (I, C):
+        _say("split", n, m, layout)
+        dev = gpu.generate(0, 40 + n, n, m, layout)
+        ins = [_np(t).copy() for t in dev]
+        x = _run_f64(gpu, dev, n, m, layout)
+        ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+        assert np.array_equal(x, ref), (n, layout)
+        for t, h in zip(dev[:5], ins[:5]):
+            assert np.array_equal(_np(t), h)
+
+
+def test_split_tunables_bit_exact(gpu, oracle_lib):
+    """Deep splits on small systems: every LDS threshold, steps-per-pass and
+    tile shape gives the same bits."""
+    import torch
+    saved = gpu.split_tune_get()
+    n_shapes = saved[3]
+    try:
+        for n, m in ((40, 5), (301, 3), (1000, 4), (2500, 2)):
+            dev = gpu.generate(1, 7 + n, n, m, I)
+            ins = [_np(t).copy() for t in dev]
+            ref = oracle_lib.pcr_batch(*ins, n, m, I)
+            for lds_max in (4, 16, 64, 256):
+                for k in (1, 2, 3, 5):
+                    for shape in range(n_shapes):
+                        try:
+                            gpu.split_tune_set(lds_max, k, shape)
+                        except gpu.PCRPError:
+                            continue   # halo too wide for this tile shape
+                        _say("split-tune", n, m, lds_max, k, shape)
+                        y = torch.from_numpy(ins[5]).cuda()
+                        gpu.solve(*dev[:5], y, n, m, I)
+                        torch.cuda.synchronize()
+                        assert np.array_equal(_np(y), ref), (n, lds_max, k, shape)
+    finally:
+        gpu.split_tune_set(*saved[:3])
+
+
+def test_split_config2_shape_sample(gpu, oracle_lib):
+    """Config-2 system size (n = 2^20, weakly dominant generator), 2 systems:
+    bit-identical to the oracle and within FP64 PCR noise of dgbsv."""
+    n, m = 1 << 20, 2
+    _say("split-c2", n, m)
+    dev = gpu.generate(1, 0, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    x = _run_f64(gpu, dev, n, m, I)
+    ref = oracle_lib.pcr_batch(*ins, n, m, I)
+    assert np.array_equal(x, ref)
+    xg = oracle_lib.gbsv_batch(*ins, n, m, I)
+    print("c2 pcr64 vs dgbsv", oracle_lib.normwise_relerr(x, xg, n, m, I), flush=True)
