@@ -29,6 +29,13 @@ struct SolveArgsF64 {
     SplitInfo split;
 };
 
+// One job of the persistent LDS queue: rows base + (i << stride_log2),
+// i < n, of array set `src`; x written to the same indices of the output.
+struct LdsJob {
+    long long base;
+    int stride_log2, n, steps, src;
+};
+
 // A compiled LDS-kernel variant: KIND (0 shared pair equations, 1 redundant),
 // NT threads x RPT rows per thread, capacity max_n = NT*RPT rows.
 struct LdsVariantF64 {

@@ -72,33 +72,18 @@ __device__ __forceinline__ void register_step(Row<double> (&r)[RPT], int t, int 
         if (t + NT * k < n) r[k] = N[k];
 }
 
+// Solve one system (or sub-system) held at ga..gy[base + i*rs], i < n, with
+// `steps` reduction steps; x goes to gx[xbase + i*xrs]. All threads of the
+// workgroup call this together (it contains barriers).
 template <int KIND, int NT, int RPT>
-__global__ __launch_bounds__(NT) void lds_f64_kernel(
+__device__ __forceinline__ void lds_solve_one(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
-    const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
-    int n, int m, long long rs, long long ss, int steps, SplitInfo split)
+    const double* __restrict__ gd, const double* __restrict__ ge, const double* gy,
+    double* gx, long long base, long long rs, long long xbase, long long xrs, int n,
+    int steps, double (&sR)[6][NT * RPT],
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1])
 {
-    constexpr int NMAX = NT * RPT;
-    __shared__ double sR[6][NMAX];                               // rows a,b,c,d,e,y
-    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
-
     const int t = threadIdx.x;
-    const int sys = xcd_remap(blockIdx.x, m);
-    const long long base = (long long)sys * ss;
-    // where x goes: in place, or (split path) back to the caller's layout
-    double* __restrict__ gx = gy;
-    long long xbase = base, xrs = rs;
-    if (split.S > 0) {
-        // sub-system sys = R*m0 + j: rows R + 2^S*l of original system j
-        const int R = sys / split.m0, j = sys - R * split.m0;
-        n = (split.n0 - R + (1 << split.S) - 1) >> split.S;
-        gx = split.x;
-        if (split.out_layout == 0) {   // contiguous j*n0 + i
-            xbase = (long long)j * split.n0 + R;
-            xrs = 1LL << split.S;
-        }
-    }
-
     Row<double> r[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
@@ -239,6 +224,62 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
                 gx[g] = r[k].y / r[k].c;
             }
         }
+    }
+}
+
+template <int KIND, int NT, int RPT>
+__global__ __launch_bounds__(NT) void lds_f64_kernel(
+    const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
+    const double* __restrict__ gd, const double* __restrict__ ge, double* gy,
+    int n, int m, long long rs, long long ss, int steps, SplitInfo split)
+{
+    constexpr int NMAX = NT * RPT;
+    __shared__ double sR[6][NMAX];                               // rows a,b,c,d,e,y
+    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
+
+    const int sys = xcd_remap(blockIdx.x, m);
+    const long long base = (long long)sys * ss;
+    // where x goes: in place, or (split path) back to the caller's layout
+    double* gx = gy;
+    long long xbase = base, xrs = rs;
+    if (split.S > 0) {
+        // sub-system sys = R*m0 + j: rows R + 2^S*l of original system j
+        const int R = sys / split.m0, j = sys - R * split.m0;
+        n = (split.n0 - R + (1 << split.S) - 1) >> split.S;
+        gx = split.x;
+        if (split.out_layout == 0) {   // contiguous j*n0 + i
+            xbase = (long long)j * split.n0 + R;
+            xrs = 1LL << split.S;
+        }
+    }
+    lds_solve_one<KIND, NT, RPT>(ga, gb, gc, gd, ge, gy, gx, base, rs, xbase, xrs, n, steps, sR, sE);
+}
+
+// Persistent work-queue form: each workgroup takes the next job from an
+// atomic counter until the list is exhausted (every wave reaches the exit).
+// Jobs are whole systems or residue sub-systems of split systems; src picks
+// the array set a job reads (the caller's arrays or a pass workspace).
+template <int KIND, int NT, int RPT>
+__global__ __launch_bounds__(NT) void lds_f64_queue_kernel(
+    const double* const* __restrict__ sets, double* __restrict__ gx, const LdsJob* __restrict__ jobs,
+    int njobs, unsigned int* __restrict__ counter)
+{
+    constexpr int NMAX = NT * RPT;
+    __shared__ double sR[6][NMAX];
+    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
+    __shared__ int sJob;
+    for (;;) {
+        if (threadIdx.x == 0) sJob = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int jid = sJob;
+        __syncthreads();
+        if (jid >= njobs) break;
+        const LdsJob J = jobs[jid];
+        const double* const* S = sets + 6 * J.src;
+        const long long rs = 1LL << J.stride_log2;
+        lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base, rs, J.base,
+                                     rs, J.n, J.steps, sR, sE);
+        __syncthreads();
     }
 }
 
