@@ -253,9 +253,11 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
         xmax = block_max<NT>(xmax, sRed[1]);
         // Stop when the correction is below tol, or when the error left after
         // applying it, estimated from the observed contraction
-        // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k|, is below tol (rho <= 0.1).
+        // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k| with a safety factor of 10,
+        // is below tol (and rho <= 0.1).
         const bool small = !(dmax > tol * xmax);   // also stops on NaN
-        const bool contracting = sweep > 0 && dmax <= 0.1 * dprev && dmax * dmax <= tol * xmax * dprev;
+        const bool contracting = sweep > 0 && dmax <= 0.1 * dprev &&
+                                 10.0 * dmax * dmax <= tol * xmax * dprev;
         const bool done = small || contracting;
         dprev = dmax;
         if (done || sweep >= max_sweeps) {
