@@ -81,8 +81,8 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
  * x0 = PCR32(y); then up to max_sweeps times: r = y - A x (FP64),
  * dx = PCR32(r), x += dx (FP64). A system stops when ||dx_k|| <= tol ||x||,
  * or when the error left after the update, estimated from the observed
- * contraction as 10 ||dx_k||^2 / ||dx_{k-1}||, is <= tol ||x|| (and the
- * contraction is <= 0.1). Norms are max-norms. Inputs and output are FP64.
+ * contraction as 10 ||dx_k||^2 / ||dx_{k-1}|| (k >= 2), is <= tol ||x|| (and
+ * the contraction is <= 0.1). Norms are max-norms. Inputs and output are FP64.
  * sweeps_out: optional DEVICE array of m ints receiving each system's number
  * of refinement sweeps (max_sweeps + 1 flags a system that did not stop).
  * Sizes n <= 2048 run in one LDS-resident kernel per system.

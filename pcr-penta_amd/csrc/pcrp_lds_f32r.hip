@@ -254,9 +254,10 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
         // Stop when the correction is below tol, or when the error left after
         // applying it, estimated from the observed contraction
         // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k| with a safety factor of 10,
-        // is below tol (and rho <= 0.1).
+        // is below tol (and rho <= 0.1). rho needs two refinement corrections:
+        // |dx_1| / |x_0| measures the first solve's error, not the contraction.
         const bool small = !(dmax > tol * xmax);   // also stops on NaN
-        const bool contracting = sweep > 0 && dmax <= 0.1 * dprev &&
+        const bool contracting = sweep > 1 && dmax <= 0.1 * dprev &&
                                  10.0 * dmax * dmax <= tol * xmax * dprev;
         const bool done = small || contracting;
         dprev = dmax;
