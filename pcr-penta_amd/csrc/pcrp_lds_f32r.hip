@@ -10,8 +10,8 @@
 //      x += dx (FP64), r = y - A x (FP64, A and y exact), until the
 //      relative error left is estimated below tol (see the stopping rule
 //      below) or max_sweeps refinement sweeps have run.
-// A zero FP32 pivot gives a zero multiplier (the FP64 path keeps the
-// reference's rule, kernel.cu:122,129): the refinement corrects either way.
+// A zero (or below 2^-100) FP32 pivot gives a zero multiplier (the FP64 path
+// keeps the reference's rule, kernel.cu:122,129): the refinement corrects it.
 //
 // Reciprocals are approximate (v_rcp_f32): the refinement absorbs the error.
 #include "pcrp_device.h"
@@ -22,6 +22,13 @@ namespace pcrp {
 namespace {
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Off-diagonal couplings of a dominant system shrink doubly exponentially
+// with the level (squaring per step), so deep-level FP32 pivots reach the
+// denormal range, where v_rcp_f32 returns infinity. Pivots below 2^-100 are
+// treated as zero: the coupling they would eliminate is below FP32
+// resolution, and the FP64 refinement corrects what is left.
+constexpr float kTiny = 0x1p-100f;
 
 struct RowF { float a, b, c, d, e; };
 struct PairF { float A, B, C, D, E, rC; };   // rC = 1/C (approximate)
@@ -106,14 +113,14 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
                     const bool aC = j - 2 * delta >= 0, arC = j + 2 * delta <= n - 1;
                     const int jl = hasL ? j - delta : j, jr = hasR ? j + delta : j;
                     const float Lb = sF[1][jl], Rd = sF[3][jr];
-                    al = (aC && Lb != 0.f) ? -r[k].a * frcp(Lb) : 0.f;
-                    ar = (arC && Rd != 0.f) ? -r[k].e * frcp(Rd) : 0.f;
+                    al = (aC && fabsf(Lb) >= kTiny) ? -r[k].a * frcp(Lb) : 0.f;
+                    ar = (arC && fabsf(Rd) >= kTiny) ? -r[k].e * frcp(Rd) : 0.f;
                     E[k].A = al * sF[0][jl];
                     E[k].B = hasL ? fmaf(ar, sF[0][jr], fmaf(al, sF[2][jl], r[k].b)) : 0.f;
                     E[k].C = fmaf(ar, sF[1][jr], fmaf(al, sF[3][jl], r[k].c));
                     E[k].D = hasR ? fmaf(ar, sF[2][jr], fmaf(al, sF[4][jl], r[k].d)) : 0.f;
                     E[k].E = ar * sF[4][jr];
-                    E[k].rC = E[k].C != 0.f ? frcp(E[k].C) : 0.f;
+                    E[k].rC = fabsf(E[k].C) >= kTiny ? frcp(E[k].C) : 0.f;
                 }
                 Mal[s][k] = al;
                 Mar[s][k] = ar;
@@ -168,10 +175,11 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
         if (i < n) {
             if (i < n - D) {
                 const float cii = sF[2][i + D], bii = sF[1][i + D];
-                const float rd = frcp(fmaf(r[k].c, cii, -(bii * r[k].d)));
+                const float det = fmaf(r[k].c, cii, -(bii * r[k].d));
+                const float rd = fabsf(det) >= kTiny ? frcp(det) : 0.f;
                 F1[k] = cii * rd; F2[k] = -r[k].d * rd; F3[k] = r[k].c * rd; F4[k] = -bii * rd;
             } else if (i < D) {
-                F1[k] = frcp(r[k].c);
+                F1[k] = fabsf(r[k].c) >= kTiny ? frcp(r[k].c) : 0.f;
             }
         }
     }
