@@ -93,6 +93,32 @@ int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
                         int *sweeps_out, void *ws, size_t ws_bytes, void *stream);
 
 /*
+ * Uneven batch, plain FP64: m systems of different sizes in one call. System
+ * j occupies rows h_offsets[j] .. h_offsets[j] + h_n[j] - 1 of every array
+ * (contiguous, non-overlapping). h_offsets / h_n are HOST arrays, used to
+ * plan the routing: systems up to the LDS threshold are solved whole, larger
+ * ones run their first reduction steps as HBM pass rounds shared by all large
+ * systems, and every system or residue sub-system then goes through one
+ * persistent work-queue LDS launch. Results equal solving each system alone
+ * with pcrp_solve_f64 (bit for bit). No counterpart in the reference, whose
+ * driver runs one (m, n) shape per launch (main.cu:245-260).
+ * The plan tables are copied to the workspace on `stream` each call.
+ */
+size_t pcrp_uneven_workspace_size(const int64_t *h_offsets, const int *h_n, int m,
+                                  int precision);
+int pcrp_solve_uneven_f64(const double *a, const double *b, const double *c,
+                          const double *d, const double *e, double *y,
+                          const int64_t *h_offsets, const int *h_n, int m,
+                          void *ws, size_t ws_bytes, void *stream);
+
+/* Seeded generator for an uneven batch laid out as above (DEVICE arrays
+ * d_offsets / d_n): the same values as pcrp_generate gives system j of a
+ * batch, row i at d_offsets[j] + i. */
+int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t *d_offsets, const int *d_n,
+                         int m, double *a, double *b, double *c, double *d, double *e,
+                         double *y, void *stream);
+
+/*
  * Seeded input generator (device). kind 0: off-diagonals U[-1,1], main
  * U[4.5,5.5]; kind 1: off-diagonals U[-0.5,0.5], main U[2.2,2.6]; RHS
  * U[-1,1]; out-of-range diagonal entries zeroed. Values depend only on

@@ -49,6 +49,27 @@ __global__ void generate_kernel(int kind, uint64_t key, int n, int m, long long 
     y[g] = gen_uniform(key, 5, j, i, -1.0, 1.0);
 }
 
+// one workgroup per system of an uneven batch
+__global__ void generate_uneven_kernel(int kind, uint64_t key, const int64_t* off, const int* ns,
+                                       double* a, double* b, double* c, double* d, double* e,
+                                       double* y)
+{
+    const int j = blockIdx.x;
+    const int n = ns[j];
+    const long long o = off[j];
+    const double offv = kind == 1 ? 0.5 : 1.0;
+    const double clo = kind == 1 ? 2.2 : 4.5, chi = kind == 1 ? 2.6 : 5.5;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const long long g = o + i;
+        a[g] = i >= 2 ? gen_uniform(key, 0, j, i, -offv, offv) : 0.0;
+        b[g] = i >= 1 ? gen_uniform(key, 1, j, i, -offv, offv) : 0.0;
+        c[g] = gen_uniform(key, 2, j, i, clo, chi);
+        d[g] = i <= n - 2 ? gen_uniform(key, 3, j, i, -offv, offv) : 0.0;
+        e[g] = i <= n - 3 ? gen_uniform(key, 4, j, i, -offv, offv) : 0.0;
+        y[g] = gen_uniform(key, 5, j, i, -1.0, 1.0);
+    }
+}
+
 // kernel.cu:5-26 (initialize): T^2 with y = 1.
 __global__ void init_t2_kernel(int n, int m, long long rs, long long ss,
                                double* a, double* b, double* c, double* d, double* e, double* y)
@@ -318,6 +339,19 @@ int pcrp_generate(int kind, uint64_t seed, int n, int m, int layout, double* a, 
     hipLaunchKernelGGL(generate_kernel, dim3((unsigned)((total + bs - 1) / bs)), dim3(bs), 0,
                        static_cast<hipStream_t>(stream), kind, host_splitmix64(seed), n, m, rs,
                        ss, a, b, c, d, e, y);
+    return hipGetLastError() == hipSuccess ? PCRP_OK : PCRP_ERR_HIP;
+}
+
+int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t* d_offsets, const int* d_n,
+                         int m, double* a, double* b, double* c, double* d, double* e, double* y,
+                         void* stream)
+{
+    if (m < 1) return PCRP_ERR_INVALID_SIZE;
+    if (!d_offsets || !d_n || !a || !b || !c || !d || !e || !y) return PCRP_ERR_NULL;
+    if (kind != 0 && kind != 1) return PCRP_ERR_ARG;
+    hipLaunchKernelGGL(generate_uneven_kernel, dim3(m), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), kind, host_splitmix64(seed), d_offsets,
+                       d_n, a, b, c, d, e, y);
     return hipGetLastError() == hipSuccess ? PCRP_OK : PCRP_ERR_HIP;
 }
 

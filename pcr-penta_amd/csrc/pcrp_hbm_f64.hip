@@ -31,32 +31,22 @@
 
 namespace pcrp {
 
-template <int TB, int G>
-__global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
-    const double* __restrict__ ia, const double* __restrict__ ib, const double* __restrict__ ic,
-    const double* __restrict__ id, const double* __restrict__ ie, const double* __restrict__ iy,
-    double* __restrict__ oa, double* __restrict__ ob, double* __restrict__ oc,
-    double* __restrict__ od, double* __restrict__ oe, double* __restrict__ oy,
-    int n0, int m0, int Sin, int k, long long in_rs, long long in_ss, int Qin, int tiles)
+// One tile of one HBM pass: rows l = tile*T - H + lr of current sub-system q
+// (valid when 0 <= l < L), k fused steps in LDS, interior rows stored.
+// in_at(l) / out_at(l) give the element index of row l of this sub-system in
+// the input arrays and of its destination in the output arrays.
+template <int TB, int G, typename InAt, typename OutAt>
+__device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* const (&out)[6],
+                                          bool qvalid, int l, int L, int k, int H, int T,
+                                          InAt in_at, OutAt out_at, double (&sR)[6][TB * G])
 {
-    constexpr int NTH = TB * G;
-    __shared__ double sR[6][NTH];
-
-    const int H = 2 * ((1 << k) - 1);
-    const int T = TB - 2 * H;
-    const int grp = blockIdx.x / tiles, tile = blockIdx.x - grp * tiles;
     const int tid = threadIdx.x, lr = tid / G, g = tid - lr * G;
-    const int q = grp * G + g;
-    const int l = tile * T - H + lr;
-    // current sub-system q = R*m0 + j holds rows R + 2^Sin * l of system j
-    const int R = q / m0;
-    const int L = (n0 - R + (1 << Sin) - 1) >> Sin;
-    const bool valid = q < Qin && l >= 0 && l < L;
-
+    const bool valid = qvalid && l >= 0 && l < L;
     Row<double> r{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
     if (valid) {
-        const long long gi = (long long)q * in_ss + (long long)l * in_rs;
-        r.a = ia[gi]; r.b = ib[gi]; r.c = ic[gi]; r.d = id[gi]; r.e = ie[gi]; r.y = iy[gi];
+        const long long gi = in_at(l);
+        r.a = in[0][gi]; r.b = in[1][gi]; r.c = in[2][gi];
+        r.d = in[3][gi]; r.e = in[4][gi]; r.y = in[5][gi];
     }
     auto readR = [&](int i) {
         Row<double> w;
@@ -95,11 +85,79 @@ __global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
         __syncthreads();
     }
     if (valid && lr >= H && lr < H + T) {
-        const int rr = l & ((1 << k) - 1), lp = l >> k;
-        const long long Qout = (long long)Qin << k;
-        const long long go = (long long)lp * Qout + (long long)rr * Qin + q;
-        oa[go] = r.a; ob[go] = r.b; oc[go] = r.c; od[go] = r.d; oe[go] = r.e; oy[go] = r.y;
+        const long long go = out_at(l);
+        out[0][go] = r.a; out[1][go] = r.b; out[2][go] = r.c;
+        out[3][go] = r.d; out[4][go] = r.e; out[5][go] = r.y;
     }
+}
+
+// Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
+template <int TB, int G>
+__global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
+    const double* __restrict__ ia, const double* __restrict__ ib, const double* __restrict__ ic,
+    const double* __restrict__ id, const double* __restrict__ ie, const double* __restrict__ iy,
+    double* __restrict__ oa, double* __restrict__ ob, double* __restrict__ oc,
+    double* __restrict__ od, double* __restrict__ oe, double* __restrict__ oy,
+    int n0, int m0, int Sin, int k, long long in_rs, long long in_ss, int Qin, int tiles)
+{
+    __shared__ double sR[6][TB * G];
+    const int H = 2 * ((1 << k) - 1);
+    const int T = TB - 2 * H;
+    const int grp = blockIdx.x / tiles, tile = blockIdx.x - grp * tiles;
+    const int lr = threadIdx.x / G, g = threadIdx.x - lr * G;
+    const int q = grp * G + g;
+    const int l = tile * T - H + lr;
+    // current sub-system q = R*m0 + j holds rows R + 2^Sin * l of system j
+    const int R = q / m0;
+    const int L = (n0 - R + (1 << Sin) - 1) >> Sin;
+    const long long Qout = (long long)Qin << k;
+    const double* const in[6] = {ia, ib, ic, id, ie, iy};
+    double* const out[6] = {oa, ob, oc, od, oe, oy};
+    pass_tile<TB, G>(
+        in, out, q < Qin, l, L, k, H, T,
+        [&](int ll) { return (long long)q * in_ss + (long long)ll * in_rs; },
+        [&](int ll) {
+            return (long long)(ll >> k) * Qout + (long long)(ll & ((1 << k) - 1)) * Qin + q;
+        },
+        sR);
+}
+
+// Uneven batch: systems stored contiguously at their own offsets. For a single
+// system the residue-interleaved index of row l of sub-system q (q < 2^Sin)
+// is q + l*2^Sin, i.e. the row's own index, so each pass reads and writes
+// the same element positions (in the caller's arrays or a same-shaped
+// workspace). tab lists the systems that still need passes in this round.
+template <int TB, int G>
+__global__ __launch_bounds__(TB* G) void hbm_pass_uneven_f64_kernel(
+    const double* const* __restrict__ sets, int src, int dst, const PassSys* __restrict__ tab,
+    int ntab, int Sin)
+{
+    __shared__ double sR[6][TB * G];
+    // locate this block's system: last entry with block0 <= blockIdx.x
+    const long long b = blockIdx.x;
+    int lo = 0, hi = ntab - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    }
+    const PassSys P = tab[lo];
+    const int local = (int)(b - P.block0);
+    const int grp = local / P.tiles, tile = local - grp * P.tiles;
+    const int k = P.k;
+    const int H = 2 * ((1 << k) - 1);
+    const int T = TB - 2 * H;
+    const int lr = threadIdx.x / G, g = threadIdx.x - lr * G;
+    const int Qin = 1 << Sin;
+    const int q = grp * G + g;
+    const int L = q < Qin ? (P.n - q + Qin - 1) >> Sin : 0;
+    const int l = tile * T - H + lr;
+    const double* const in[6] = {sets[6 * src + 0], sets[6 * src + 1], sets[6 * src + 2],
+                                 sets[6 * src + 3], sets[6 * src + 4], sets[6 * src + 5]};
+    double* const out[6] = {const_cast<double*>(sets[6 * dst + 0]), const_cast<double*>(sets[6 * dst + 1]),
+                            const_cast<double*>(sets[6 * dst + 2]), const_cast<double*>(sets[6 * dst + 3]),
+                            const_cast<double*>(sets[6 * dst + 4]), const_cast<double*>(sets[6 * dst + 5])};
+    auto at = [&](int ll) { return P.off + q + ((long long)ll << Sin); };
+    pass_tile<TB, G>(in, out, q < Qin, l, L, k, H, T, at, at, sR);
 }
 
 template <int TB, int G>
@@ -120,20 +178,44 @@ static int launch_pass(const double* const in[6], double* const out[6], int n0, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int TB, int G>
+static int launch_pass_uneven_t(const double* const* sets, int src, int dst, const PassSys* tab,
+                                int ntab, long long blocks, int Sin, hipStream_t st)
+{
+    if (blocks <= 0) return 0;
+    if (blocks > 0x7fffffffLL) return -2;
+    hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G>), dim3((unsigned)blocks), dim3(TB * G),
+                       0, st, sets, src, dst, tab, ntab, Sin);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 using PassFn = int (*)(const double* const*, double* const*, int, int, int, int, long long,
                        long long, int, hipStream_t);
+
+using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, int, long long, int,
+                             hipStream_t);
 
 struct PassShape {
     int tb, g;
     PassFn fn;
+    PassUnevenFn uneven;
 };
 
 static const PassShape kPassShapes[] = {
-    {256, 4, &launch_pass<256, 4>},
-    {512, 2, &launch_pass<512, 2>},
-    {128, 8, &launch_pass<128, 8>},
-    {1024, 1, &launch_pass<1024, 1>},
+    {256, 4, &launch_pass<256, 4>, &launch_pass_uneven_t<256, 4>},
+    {512, 2, &launch_pass<512, 2>, &launch_pass_uneven_t<512, 2>},
+    {128, 8, &launch_pass<128, 8>, &launch_pass_uneven_t<128, 8>},
+    {1024, 1, &launch_pass<1024, 1>, &launch_pass_uneven_t<1024, 1>},
 };
+
+int pass_shape_tb(int shape) { return kPassShapes[shape].tb; }
+int pass_shape_g(int shape) { return kPassShapes[shape].g; }
+
+int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
+                       int ntab, long long blocks, int Sin, hipStream_t st)
+{
+    return kPassShapes[shape].uneven(sets, src, dst, tab, ntab, blocks, Sin, st);
+}
 
 // Tunables of the split path: LDS/HBM threshold (sub-systems at most this
 // long go to the LDS kernel), steps fused per HBM pass, pass tile shape.

@@ -36,11 +36,23 @@ struct LdsJob {
     int stride_log2, n, steps, src;
 };
 
+// One system of an uneven-batch HBM pass round: rows at off + i, i < n;
+// k steps this round; the round's blocks for it start at block0.
+struct PassSys {
+    long long off;
+    long long block0;
+    int n, k, tiles, pad;
+};
+
 // A compiled LDS-kernel variant: KIND (0 shared pair equations, 1 redundant),
 // NT threads x RPT rows per thread, capacity max_n = NT*RPT rows.
 struct LdsVariantF64 {
     int kind, nt, rpt, max_n;
     int (*launch)(const SolveArgsF64&, hipStream_t);
+    // persistent work-queue form over a job list (jobs read array set
+    // sets[6*src .. 6*src+5], write x into x at the job's indices)
+    int (*launch_queue)(const double* const* sets, double* x, const LdsJob* jobs, int njobs,
+                        unsigned int* counter, hipStream_t);
 };
 
 int lds_f64_variant_count();
@@ -75,6 +87,10 @@ int split_f64_shape_count();
 int split_f64_levels(int n, int lds_max);
 size_t split_f64_workspace(int n, int m);
 int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes, hipStream_t st);
+int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
+                       int ntab, long long blocks, int Sin, hipStream_t st);
+int pass_shape_tb(int shape);
+int pass_shape_g(int shape);
 
 int lds_f32r_max_n();
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);

@@ -293,7 +293,31 @@ static int launch(const SolveArgsF64& A, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-#define PCRP_V(KIND, NT, RPT) {KIND, NT, RPT, NT * RPT, &launch<KIND, NT, RPT>}
+// Persistent launch: one workgroup per resident slot (CUs x blocks per CU).
+template <int KIND, int NT, int RPT>
+static int launch_queue(const double* const* sets, double* x, const LdsJob* jobs, int njobs,
+                        unsigned int* counter, hipStream_t st)
+{
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void*>(&lds_f64_queue_kernel<KIND, NT, RPT>), NT, 0) !=
+                hipSuccess)
+            return -1;
+        grid = cus * (per > 0 ? per : 1);
+    }
+    const int g = njobs < grid ? njobs : grid;
+    if (g <= 0) return 0;
+    hipLaunchKernelGGL((lds_f64_queue_kernel<KIND, NT, RPT>), dim3(g), dim3(NT), 0, st, sets, x,
+                       jobs, njobs, counter);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+#define PCRP_V(KIND, NT, RPT) \
+    {KIND, NT, RPT, NT * RPT, &launch<KIND, NT, RPT>, &launch_queue<KIND, NT, RPT>}
 
 static const LdsVariantF64 kVariants[] = {
     // shared pair equations, 14 doubles of LDS per row

@@ -1,0 +1,196 @@
+// Uneven batches: thousands of systems of different n in one call.
+//
+// No counterpart in the reference (its driver sweeps one (m, n) at a time,
+// main.cu:245-260, and requires n = 1024*2^s). Systems are stored
+// contiguously, system j at rows off[j] .. off[j]+n[j]-1 of each array (the
+// reference's contiguous layout per system, kernel.cu:43).
+//
+// Routing (planned on the host from the host copies of off/n):
+//   * n <= lds_max: the whole system is one job of the LDS work queue;
+//   * n > lds_max: S = ceil(log2(n / lds_max)) reduction steps run as HBM
+//     passes (k_fuse steps per pass); all large systems share each pass round
+//     (one launch per round, a table maps blocks to systems); then every one of
+//     its 2^S residue sub-systems is one LDS job.
+// All LDS jobs go through one persistent kernel: each workgroup pulls the
+// next job from an atomic counter, largest systems first (longest-processing-
+// time order), sub-systems of one system adjacent so their interleaved rows
+// share cache lines.
+// Results are bit-identical to solving each system alone (same arithmetic,
+// same guards; the pass rounds are the split path with one system per batch).
+#include "pcr_penta.h"
+#include "pcrp_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace pcrp {
+
+namespace {
+
+__global__ void reset_counter_kernel(unsigned int* c) { *c = 0u; }
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct UnevenPlan {
+    std::vector<LdsJob> jobs;
+    std::vector<std::vector<PassSys>> rounds;   // per pass round
+    std::vector<long long> round_blocks;
+    std::vector<int> round_shape;
+    long long total = 0;                        // elements per array (max off + n)
+    bool any_split = false;
+    // workspace byte offsets
+    size_t off_counter = 0, off_sets = 0, off_jobs = 0, off_rounds = 0, off_arrays = 0, bytes = 0;
+    std::vector<size_t> round_off;
+};
+
+int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
+{
+    const SplitTune tune = split_f64_get();
+    struct Sys { int j, S, nsub; };
+    std::vector<Sys> order;
+    order.reserve(m);
+    for (int j = 0; j < m; ++j) {
+        if (h_n[j] < 1 || h_off[j] < 0) return PCRP_ERR_INVALID_SIZE;
+        const long long end = h_off[j] + (long long)h_n[j];
+        if (end > P.total) P.total = end;
+        const int S = split_f64_levels(h_n[j], tune.lds_max);
+        order.push_back({j, S, (h_n[j] + (1 << S) - 1) >> S});
+        if (S > 0) P.any_split = true;
+    }
+    if (P.total > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
+    std::stable_sort(order.begin(), order.end(),
+                     [](const Sys& x, const Sys& y) { return x.nsub > y.nsub; });
+    // LDS jobs
+    for (const Sys& s : order) {
+        const int n = h_n[s.j];
+        const int steps = pcr_steps(n);
+        if (s.S == 0) {
+            P.jobs.push_back({h_off[s.j], 0, n, steps, 0});
+        } else {
+            const int rounds = (s.S + tune.k_fuse - 1) / tune.k_fuse;
+            const int src = 1 + ((rounds - 1) & 1);
+            for (int q = 0; q < (1 << s.S); ++q)
+                P.jobs.push_back({h_off[s.j] + q, s.S, (n - q + (1 << s.S) - 1) >> s.S,
+                                  steps - s.S, src});
+        }
+    }
+    // HBM pass rounds: round r covers steps [r*k_fuse, min(S, (r+1)*k_fuse))
+    for (int r = 0;; ++r) {
+        const int Sin = r * tune.k_fuse;
+        // first round: one sub-system per system, whole-row tiles; later rounds:
+        // several residue sub-systems side by side for coalesced row loads
+        const int shape = (r == 0) ? 3 : tune.shape;
+        const int TB = pass_shape_tb(shape), G = pass_shape_g(shape);
+        std::vector<PassSys> tab;
+        long long blocks = 0;
+        for (int j = 0; j < m; ++j) {
+            const int S = split_f64_levels(h_n[j], tune.lds_max);
+            if (S <= Sin) continue;
+            const int k = std::min(tune.k_fuse, S - Sin);
+            const int H = 2 * ((1 << k) - 1), T = TB - 2 * H;
+            if (T < 1) return PCRP_ERR_ARG;
+            const int L = (h_n[j] + (1 << Sin) - 1) >> Sin;
+            const int tiles = (L + T - 1) / T;
+            const int groups = ((1 << Sin) + G - 1) / G;
+            tab.push_back({h_off[j], blocks, h_n[j], k, tiles, 0});
+            blocks += (long long)groups * tiles;
+        }
+        if (tab.empty()) break;
+        P.rounds.push_back(std::move(tab));
+        P.round_blocks.push_back(blocks);
+        P.round_shape.push_back(shape);
+    }
+    // workspace layout
+    size_t o = 0;
+    P.off_counter = o; o = align_up(o + sizeof(unsigned int));
+    P.off_sets = o;    o = align_up(o + 18 * sizeof(double*));
+    P.off_jobs = o;    o = align_up(o + P.jobs.size() * sizeof(LdsJob));
+    for (const auto& t : P.rounds) {
+        P.round_off.push_back(o);
+        o = align_up(o + t.size() * sizeof(PassSys));
+    }
+    P.off_arrays = o;
+    if (P.any_split) o += 2 * 6 * (size_t)P.total * sizeof(double);
+    P.bytes = o;
+    return PCRP_OK;
+}
+
+}  // namespace
+
+size_t uneven_f64_workspace(const int64_t* h_off, const int* h_n, int m)
+{
+    UnevenPlan P;
+    if (plan_uneven(h_off, h_n, m, P) != PCRP_OK) return 0;
+    return P.bytes;
+}
+
+int uneven_f64_solve(const double* a, const double* b, const double* c, const double* d,
+                     const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
+                     void* ws, size_t ws_bytes, hipStream_t st)
+{
+    UnevenPlan P;
+    int rc = plan_uneven(h_off, h_n, m, P);
+    if (rc != PCRP_OK) return rc;
+    if (!ws || ws_bytes < P.bytes) return PCRP_ERR_WORKSPACE;
+    char* w = static_cast<char*>(ws);
+    // array sets: 0 = caller's arrays, 1 and 2 = pass ping-pong buffers
+    const double* sets[18] = {a, b, c, d, e, y};
+    for (int v = 0; v < 12; ++v)
+        sets[6 + v] = P.any_split
+                          ? reinterpret_cast<double*>(w + P.off_arrays) + (size_t)v * P.total
+                          : nullptr;
+    std::vector<char> host(P.off_arrays);
+    std::copy(reinterpret_cast<const char*>(sets), reinterpret_cast<const char*>(sets) + sizeof sets,
+              host.begin() + P.off_sets);
+    std::copy(reinterpret_cast<const char*>(P.jobs.data()),
+              reinterpret_cast<const char*>(P.jobs.data() + P.jobs.size()), host.begin() + P.off_jobs);
+    for (size_t r = 0; r < P.rounds.size(); ++r)
+        std::copy(reinterpret_cast<const char*>(P.rounds[r].data()),
+                  reinterpret_cast<const char*>(P.rounds[r].data() + P.rounds[r].size()),
+                  host.begin() + P.round_off[r]);
+    if (hipMemcpyAsync(w, host.data(), host.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+        return PCRP_ERR_HIP;
+    const double* const* dsets = reinterpret_cast<const double* const*>(w + P.off_sets);
+    // pass rounds: round r reads set (r == 0 ? caller : 1 + (r-1)%2), writes 1 + r%2
+    for (size_t r = 0; r < P.rounds.size(); ++r) {
+        const int src = r == 0 ? 0 : 1 + (int)((r - 1) & 1);
+        const int dst = 1 + (int)(r & 1);
+        rc = launch_pass_uneven(P.round_shape[r], dsets, src, dst,
+                                reinterpret_cast<const PassSys*>(w + P.round_off[r]),
+                                (int)P.rounds[r].size(), P.round_blocks[r],
+                                (int)r * split_f64_get().k_fuse, st);
+        if (rc != 0) return PCRP_ERR_HIP;
+    }
+    // one persistent LDS launch over every job
+    unsigned int* counter = reinterpret_cast<unsigned int*>(w + P.off_counter);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, counter);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, split_f64_get().lds_max));
+    rc = v->launch_queue(dsets, y, reinterpret_cast<const LdsJob*>(w + P.off_jobs),
+                         (int)P.jobs.size(), counter, st);
+    return rc == 0 ? PCRP_OK : PCRP_ERR_HIP;
+}
+
+}  // namespace pcrp
+
+extern "C" {
+
+size_t pcrp_uneven_workspace_size(const int64_t* h_offsets, const int* h_n, int m, int precision)
+{
+    if (!h_offsets || !h_n || m < 1 || precision != PCRP_PREC_F64) return 0;
+    return pcrp::uneven_f64_workspace(h_offsets, h_n, m);
+}
+
+int pcrp_solve_uneven_f64(const double* a, const double* b, const double* c, const double* d,
+                          const double* e, double* y, const int64_t* h_offsets, const int* h_n,
+                          int m, void* ws, size_t ws_bytes, void* stream)
+{
+    if (m < 1) return PCRP_ERR_INVALID_SIZE;
+    if (!a || !b || !c || !d || !e || !y || !h_offsets || !h_n) return PCRP_ERR_NULL;
+    return pcrp::uneven_f64_solve(a, b, c, d, e, y, h_offsets, h_n, m, ws, ws_bytes,
+                                  static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -75,6 +75,13 @@ def _declare(lib):
     lib.pcrp_split_tune_set.restype = ctypes.c_int
     lib.pcrp_split_tune_get.argtypes = [ctypes.c_int] + [ip] * 4
     lib.pcrp_split_tune_get.restype = ctypes.c_int
+    lib.pcrp_uneven_workspace_size.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int]
+    lib.pcrp_uneven_workspace_size.restype = ctypes.c_size_t
+    lib.pcrp_solve_uneven_f64.argtypes = [_vp] * 8 + [ctypes.c_int, _vp, ctypes.c_size_t, _vp]
+    lib.pcrp_solve_uneven_f64.restype = ctypes.c_int
+    lib.pcrp_generate_uneven.argtypes = [ctypes.c_int, ctypes.c_uint64] + [_vp] * 2 + \
+        [ctypes.c_int] + [_vp] * 6 + [_vp]
+    lib.pcrp_generate_uneven.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -242,3 +249,76 @@ def split_tune_get(precision: int = PREC_F64):
 
 def split_tune_set(lds_max: int, k_fuse: int, shape: int, precision: int = PREC_F64):
     _check(load().pcrp_split_tune_set(precision, lds_max, k_fuse, shape), "pcrp_split_tune_set")
+
+
+# ---- uneven batches ----------------------------------------------------------
+
+def uneven_layout(sizes):
+    """(offsets int64, n int32, total rows) for systems stored back to back."""
+    import numpy as np
+    n = np.ascontiguousarray(sizes, dtype=np.int32)
+    off = np.zeros(len(n), dtype=np.int64)
+    if len(n) > 1:
+        off[1:] = np.cumsum(n[:-1], dtype=np.int64)
+    return off, n, int(n.astype(np.int64).sum())
+
+
+def uneven_workspace_size(offsets, sizes) -> int:
+    import numpy as np
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = np.ascontiguousarray(sizes, dtype=np.int32)
+    return int(load().pcrp_uneven_workspace_size(off.ctypes.data, n.ctypes.data, len(n), PREC_F64))
+
+
+def solve_uneven(a, b, c, d, e, y, offsets, sizes, stream=None, ws=None):
+    """Plain FP64 solve of an uneven batch (system j at rows offsets[j] ..
+    offsets[j]+sizes[j]-1 of flat device arrays), y overwritten in place."""
+    import numpy as np
+    import torch
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = np.ascontiguousarray(sizes, dtype=np.int32)
+    for t in (a, b, c, d, e, y):
+        if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+            raise TypeError("pcrpenta: arrays must be contiguous float64 device tensors")
+    if ws is None:
+        nb = uneven_workspace_size(off, n)
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y.device)
+    st = load().pcrp_solve_uneven_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], off.ctypes.data,
+                                      n.ctypes.data, len(n), _ptr(ws), ws.numel(), _stream(stream))
+    _check(st, "pcrp_solve_uneven_f64")
+    return y
+
+
+def generate_uneven(kind: int, seed: int, offsets, sizes, device="cuda", stream=None):
+    """Seeded uneven batch on the device (values as pcrp_generate for each system)."""
+    import numpy as np
+    import torch
+    off = torch.from_numpy(np.ascontiguousarray(offsets, dtype=np.int64)).to(device)
+    n = torch.from_numpy(np.ascontiguousarray(sizes, dtype=np.int32)).to(device)
+    total = int(np.asarray(offsets, dtype=np.int64)[-1] + np.asarray(sizes, dtype=np.int64)[-1])
+    arrs = [torch.empty(total, dtype=torch.float64, device=device) for _ in range(6)]
+    st = load().pcrp_generate_uneven(kind, seed, _ptr(off), _ptr(n), len(n),
+                                     *[_ptr(t) for t in arrs], _stream(stream))
+    _check(st, "pcrp_generate_uneven")
+    torch.cuda.current_stream().synchronize()
+    return arrs
+
+
+def config4_sizes(m: int = 10000, seed: int = 1, lo: int = 64, hi: int = 131072):
+    """Config-4 sizes: n_j = floor(exp(U[ln lo, ln hi))), U drawn by the counter
+    generator keyed (seed, 7, j, 0)."""
+    import numpy as np
+    M64 = (1 << 64) - 1
+
+    def mix(z):
+        z = (z + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(M64)
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(M64)
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(M64)
+        return z ^ (z >> np.uint64(31))
+    with np.errstate(over="ignore"):
+        key = mix(np.array([seed], dtype=np.uint64))[0]
+        ctr = (np.uint64(7) << np.uint64(56)) | (np.arange(m, dtype=np.uint64) << np.uint64(24))
+        u = mix((key + ctr) & np.uint64(M64))
+    f = (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    ln_lo, ln_hi = np.log(lo), np.log(hi)
+    return np.floor(np.exp(ln_lo + (ln_hi - ln_lo) * f)).astype(np.int32)

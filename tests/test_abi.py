@@ -87,3 +87,21 @@ def test_workspace_too_small_is_rejected(lib):
     if need == 0:
         pytest.skip("no workspace needed at this size")
     assert lib.pcrp_solve_f64(*_fake(6), n, m, 1, _fake(1)[0], need - 1, None) == 4
+
+
+def test_uneven_argument_checks(lib):
+    import numpy as np
+    off = np.array([0, 10], dtype=np.int64)
+    n = np.array([10, 0], dtype=np.int32)
+    p = _fake(6)
+    assert lib.pcrp_solve_uneven_f64(*p, off.ctypes.data, n.ctypes.data, 2, None, 0, None) == 1
+    n[1] = 5
+    assert lib.pcrp_solve_uneven_f64(*p, None, n.ctypes.data, 2, None, 0, None) == 2
+    assert lib.pcrp_solve_uneven_f64(*p, off.ctypes.data, n.ctypes.data, 0, None, 0, None) == 1
+    # small systems only: no pass arrays, but the plan tables still need workspace
+    need = lib.pcrp_uneven_workspace_size(off.ctypes.data, n.ctypes.data, 2, 0)
+    assert need > 0
+    assert lib.pcrp_solve_uneven_f64(*p, off.ctypes.data, n.ctypes.data, 2, _fake(1)[0],
+                                     need - 1, None) == 4
+    big = np.array([5000], dtype=np.int32)
+    assert lib.pcrp_uneven_workspace_size(off.ctypes.data, big.ctypes.data, 1, 0) >= 2 * 6 * 8 * 5000

@@ -268,3 +268,62 @@ def test_split_config2_shape_sample(gpu, oracle_lib):
     assert np.array_equal(x, ref)
     xg = oracle_lib.gbsv_batch(*ins, n, m, I)
     print("c2 pcr64 vs dgbsv", oracle_lib.normwise_relerr(x, xg, n, m, I), flush=True)
+
+
+# ---------------- uneven batches (persistent work queue) ----------------
+
+def _uneven_case(gpu, oracle_lib, sizes, kind, seed):
+    import torch
+    off, n, total = gpu.uneven_layout(sizes)
+    dev = gpu.generate_uneven(kind, seed, off, n)
+    ins = [_np(t).copy() for t in dev]
+    gpu.solve_uneven(*dev, off, n)
+    torch.cuda.synchronize()
+    x = _np(dev[5])
+    ref = oracle_lib.pcr_uneven(*ins, off, n)
+    return x, ref, ins, dev
+
+
+def test_uneven_generator_matches_batch_generator(gpu, oracle_lib):
+    sizes = [5, 64, 300]
+    off, n, _ = gpu.uneven_layout(sizes)
+    dev = gpu.generate_uneven(0, 3, off, n)
+    for j, (o, nj) in enumerate(zip(off, n)):
+        ref = oracle_lib.gen_batch(0, 3, int(nj), j + 1, oracle_lib.CONTIGUOUS)
+        sysj = oracle_lib.system(ref, j, int(nj), j + 1, oracle_lib.CONTIGUOUS)
+        for u, v in zip(dev, sysj):
+            assert np.array_equal(_np(u)[o:o + nj], v)
+
+
+def test_uneven_bit_exact_mixed_sizes(gpu, oracle_lib):
+    rng = np.random.default_rng(11)
+    sizes = [1, 2, 3, 5, 64, 100, 1000, 1024, 1025, 2048, 3000, 5000, 16385]
+    sizes += list(rng.integers(1, 4000, 60))
+    _say("uneven-mixed", len(sizes))
+    x, ref, ins, dev = _uneven_case(gpu, oracle_lib, sizes, 0, 21)
+    assert np.array_equal(x, ref)
+    for t, h in zip(dev[:5], ins[:5]):
+        assert np.array_equal(_np(t), h)
+
+
+def test_uneven_deep_splits_bit_exact(gpu, oracle_lib):
+    """Small LDS threshold: several pass rounds with different k per system."""
+    saved = gpu.split_tune_get()
+    try:
+        for lds_max, k, shape in ((16, 2, 0), (8, 3, 2), (64, 1, 1), (32, 4, 0)):
+            gpu.split_tune_set(lds_max, k, shape)
+            rng = np.random.default_rng(lds_max + k)
+            sizes = list(rng.integers(1, 3000, 40)) + [4096, 777]
+            _say("uneven-deep", lds_max, k, shape)
+            x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 1, 5 + k)
+            assert np.array_equal(x, ref), (lds_max, k, shape)
+    finally:
+        gpu.split_tune_set(*saved[:3])
+
+
+def test_uneven_config4_sample(gpu, oracle_lib):
+    """First 400 systems of the config-4 size draw (n log-uniform in [64, 131072], seed 1)."""
+    sizes = gpu.config4_sizes()[:400]
+    _say("uneven-c4", len(sizes), int(sizes.sum()))
+    x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 0, 1)
+    assert np.array_equal(x, ref)
