@@ -119,6 +119,34 @@ int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t *d_offsets, cons
                          double *y, void *stream);
 
 /*
+ * Heston ADI driver (the application of the PCR-Penta paper; the reference
+ * ships no ADI code). European call under Heston, Douglas scheme, uniform
+ * grid: Ns unknowns in S (S = i Smax/(Ns+1), i = 1..Ns; u = 0 at S = 0,
+ * u = Smax - K e^{-r tau} at S = Smax) and Nv unknowns in v (v = j vmax/Nv,
+ * j = 0..Nv-1; u = S at v = vmax). Fourth-order 5-point central differences
+ * in S (second order next to the S boundaries), second-order upwinding plus
+ * central diffusion in v, explicit mixed term. Each step solves Nv S-lines
+ * (contiguous, n = Ns) and Ns v-lines (interleaved, n = Nv) with
+ * pcrp_solve_f64 or pcrp_solve_f32r_f64.
+ * u: DEVICE array of Ns*Nv doubles, u[j*Ns + i-1], receives the price at
+ * tau = steps*dt. use_graph = 1 captures one time step in a hipGraph and
+ * replays it (needs a non-NULL stream; the call then synchronises `stream`
+ * before releasing the graph); use_graph = 0 enqueues every step.
+ */
+typedef struct {
+    int ns, nv;                 /* unknowns in S and in v */
+    double smax, vmax;          /* grid extents */
+    double kappa, theta_v, sigma, rho, r, strike;
+    double dt;                  /* time step */
+    int steps;                  /* number of steps */
+    double theta_scheme;        /* Douglas theta (0.5) */
+} pcrp_heston_params;
+
+size_t pcrp_heston_workspace_size(const pcrp_heston_params *p, int precision);
+int pcrp_heston_solve(const pcrp_heston_params *p, int precision, int use_graph, double *u,
+                      void *ws, size_t ws_bytes, void *stream);
+
+/*
  * Seeded input generator (device). kind 0: off-diagonals U[-1,1], main
  * U[4.5,5.5]; kind 1: off-diagonals U[-0.5,0.5], main U[2.2,2.6]; RHS
  * U[-1,1]; out-of-range diagonal entries zeroed. Values depend only on

@@ -1,0 +1,369 @@
+// Heston ADI driver: the pentadiagonal solver's application in the PCR-Penta
+// paper (convection-dominated Heston PDE). The reference repository ships no
+// ADI code; this caller builds the line systems and steps the Douglas scheme.
+//
+// PDE in time-to-maturity tau for a European call:
+//   u_tau = 1/2 v S^2 u_SS + rho sigma v S u_Sv + 1/2 sigma^2 v u_vv
+//           + r S u_S + kappa (theta_v - v) u_v - r u
+// Unknowns: S_i = i hS, i = 1..Ns (hS = Smax/(Ns+1); u = 0 at S = 0 and
+// u = Smax - K e^{-r tau} at S = Smax), v_j = j hv, j = 0..Nv-1 (hv = vmax/Nv;
+// the PDE holds at v = 0; u = S at v = vmax). Storage U[j*Ns + (i-1)]:
+// S-lines are contiguous (m = Nv systems of n = Ns), v-lines interleaved
+// (m = Ns systems of n = Nv).
+//   A1 (S): 1/2 v S^2 u_SS + r S u_S - r u/2, fourth-order 5-point central
+//           differences (3-point second order at i = 1 and i = Ns);
+//   A2 (v): 1/2 sigma^2 v u_vv (3-point) + kappa(theta_v - v) u_v with
+//           second-order upwinding (first order next to v = vmax / at j = 1),
+//           - r u/2;
+//   A0    : rho sigma v S u_Sv, 4-point central cross difference (explicit).
+// Douglas (theta = 1/2), boundary terms b_k(tau) split off each A_k:
+//   Y0 = U + dt (A0 U + A1 U + A2 U + b(tau_n))
+//   (I - th dt A1) Y1 = Y0 - th dt (A1 U + b1(tau_n)) + th dt b1(tau_n+1)
+//   (I - th dt A2) Y2 = Y1 - th dt (A2 U + b2(tau_n)) + th dt b2(tau_n+1)
+//   U_{n+1} = Y2.
+// A step is four launches on one stream with fixed pointers (explicit
+// kernel, S-line solve, combine kernel, v-line solve); the step index lives
+// in device memory, so one captured hipGraph replays every step.
+#include "pcr_penta.h"
+#include "pcrp_internal.h"
+
+#include <math.h>
+
+namespace pcrp {
+
+namespace {
+
+struct Grid {
+    int ns, nv;
+    double hS, hv, smax, vmax, kappa, thv, sigma, rho, r, K, dt, th;
+};
+
+__host__ __device__ inline double S_at(const Grid& g, int i) { return i * g.hS; }
+__host__ __device__ inline double v_at(const Grid& g, int j) { return j * g.hv; }
+
+// value of the grid function at node (i, j), i in 0..ns+1, j in 0..nv
+// (boundary nodes from the boundary conditions at time-to-maturity tau)
+__device__ inline double node(const Grid& g, const double* U, int i, int j, double tau)
+{
+    if (i == 0) return 0.0;
+    if (i == g.ns + 1) return g.smax - g.K * exp(-g.r * tau);
+    if (j == g.nv) return S_at(g, i);
+    return U[(long long)j * g.ns + (i - 1)];
+}
+
+// stencil coefficients of A1 at row i (offsets -2..2)
+__host__ __device__ inline void a1_coef(const Grid& g, int i, int j, double w[5])
+{
+    const double S = S_at(g, i), v = v_at(g, j);
+    const double D = 0.5 * v * S * S, C = g.r * S;
+    if (i >= 2 && i <= g.ns - 1) {
+        const double d2 = D / (12.0 * g.hS * g.hS), d1 = C / (12.0 * g.hS);
+        w[0] = -d2 + d1;
+        w[1] = 16.0 * d2 - 8.0 * d1;
+        w[2] = -30.0 * d2 - 0.5 * g.r;
+        w[3] = 16.0 * d2 + 8.0 * d1;
+        w[4] = -d2 - d1;
+    } else {
+        const double d2 = D / (g.hS * g.hS), d1 = C / (2.0 * g.hS);
+        w[0] = 0.0;
+        w[1] = d2 - d1;
+        w[2] = -2.0 * d2 - 0.5 * g.r;
+        w[3] = d2 + d1;
+        w[4] = 0.0;
+    }
+}
+
+// stencil coefficients of A2 at row j (offsets -2..2)
+__host__ __device__ inline void a2_coef(const Grid& g, int j, double w[5])
+{
+    const double v = v_at(g, j);
+    const double diff = 0.5 * g.sigma * g.sigma * v / (g.hv * g.hv);
+    const double mu = g.kappa * (g.thv - v);
+    w[0] = w[1] = w[3] = w[4] = 0.0;
+    w[2] = -0.5 * g.r;
+    if (j >= 1) {   // diffusion (zero at v = 0)
+        w[1] += diff;
+        w[2] += -2.0 * diff;
+        w[3] += diff;
+    }
+    if (mu >= 0.0) {
+        if (j + 2 <= g.nv) {   // second-order forward
+            const double f = mu / (2.0 * g.hv);
+            w[2] += -3.0 * f; w[3] += 4.0 * f; w[4] += -f;
+        } else {               // first-order forward next to v = vmax
+            const double f = mu / g.hv;
+            w[2] += -f; w[3] += f;
+        }
+    } else {
+        if (j >= 2) {          // second-order backward
+            const double f = mu / (2.0 * g.hv);
+            w[2] += 3.0 * f; w[1] += -4.0 * f; w[0] += f;
+        } else {               // first-order backward at j = 1
+            const double f = mu / g.hv;
+            w[2] += f; w[1] += -f;
+        }
+    }
+}
+
+// (I - th dt A) row coefficients into the five diagonals, row-aligned:
+// entries that would couple to a boundary node or fall outside are zero.
+__global__ void setup_lines_kernel(Grid g, double* sa, double* sb, double* sc, double* sd,
+                                   double* se, double* va, double* vb, double* vc, double* vd,
+                                   double* ve)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.ns * g.nv) return;
+    const int j = (int)(idx / g.ns), i = (int)(idx % g.ns) + 1;
+    const double f = -g.th * g.dt;
+    double w[5];
+    a1_coef(g, i, j, w);
+    // S-line j, row i-1 (contiguous: element (i-1) of system j at idx)
+    sa[idx] = (i - 2 >= 1) ? f * w[0] : 0.0;
+    sb[idx] = (i - 1 >= 1) ? f * w[1] : 0.0;
+    sc[idx] = 1.0 + f * w[2];
+    sd[idx] = (i + 1 <= g.ns) ? f * w[3] : 0.0;
+    se[idx] = (i + 2 <= g.ns) ? f * w[4] : 0.0;
+    a2_coef(g, j, w);
+    // v-line i-1, row j (interleaved: element j of system i-1 at j*ns + i-1 = idx)
+    va[idx] = (j - 2 >= 0) ? f * w[0] : 0.0;
+    vb[idx] = (j - 1 >= 0) ? f * w[1] : 0.0;
+    vc[idx] = 1.0 + f * w[2];
+    vd[idx] = (j + 1 <= g.nv - 1) ? f * w[3] : 0.0;
+    ve[idx] = (j + 2 <= g.nv - 1) ? f * w[4] : 0.0;
+}
+
+__global__ void payoff_kernel(Grid g, double* U, int* step)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx == 0) *step = 0;
+    if (idx >= (long long)g.ns * g.nv) return;
+    const int i = (int)(idx % g.ns) + 1;
+    const double s = S_at(g, i) - g.K;
+    U[idx] = s > 0.0 ? s : 0.0;
+}
+
+// A1 U (with boundary nodes at tau) at unknown (i, j)
+__device__ inline double apply_a1(const Grid& g, const double* U, int i, int j, double tau)
+{
+    double w[5];
+    a1_coef(g, i, j, w);
+    double acc = w[2] * node(g, U, i, j, tau);
+    acc += w[1] * node(g, U, i - 1, j, tau);
+    acc += w[3] * node(g, U, i + 1, j, tau);
+    if (w[0] != 0.0) acc += w[0] * node(g, U, i - 2, j, tau);
+    if (w[4] != 0.0) acc += w[4] * node(g, U, i + 2, j, tau);
+    return acc;
+}
+
+// boundary part b1(tau) of A1 at (i, j): terms on boundary nodes only
+__device__ inline double bnd_a1(const Grid& g, int i, int j, double tau)
+{
+    double w[5];
+    a1_coef(g, i, j, w);
+    double acc = 0.0;
+    for (int o = -2; o <= 2; ++o) {
+        const int ii = i + o;
+        if (w[o + 2] != 0.0 && (ii == 0 || ii == g.ns + 1))
+            acc += w[o + 2] * node(g, nullptr, ii, j, tau);
+    }
+    return acc;
+}
+
+__device__ inline double apply_a2(const Grid& g, const double* U, int i, int j, double tau)
+{
+    double w[5];
+    a2_coef(g, j, w);
+    double acc = w[2] * node(g, U, i, j, tau);
+    for (int o = -2; o <= 2; ++o) {
+        if (o == 0 || w[o + 2] == 0.0) continue;
+        acc += w[o + 2] * node(g, U, i, j + o, tau);
+    }
+    return acc;
+}
+
+__device__ inline double bnd_a2(const Grid& g, int i, int j, double tau)
+{
+    double w[5];
+    a2_coef(g, j, w);
+    double acc = 0.0;
+    for (int o = 1; o <= 2; ++o)
+        if (w[o + 2] != 0.0 && j + o == g.nv) acc += w[o + 2] * node(g, nullptr, i, j + o, tau);
+    return acc;
+}
+
+__device__ inline double apply_a0(const Grid& g, const double* U, int i, int j, double tau)
+{
+    if (j == 0) return 0.0;   // v = 0
+    const double c = g.rho * g.sigma * v_at(g, j) * S_at(g, i) / (4.0 * g.hS * g.hv);
+    const double x = node(g, U, i + 1, j + 1, tau) - node(g, U, i + 1, j - 1, tau) -
+                     node(g, U, i - 1, j + 1, tau) + node(g, U, i - 1, j - 1, tau);
+    return c * x;
+}
+
+// explicit stage: R1 = RHS of the S-line systems, W = v-line correction
+__global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, const int* step)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)g.ns * g.nv) return;
+    const int j = (int)(idx / g.ns), i = (int)(idx % g.ns) + 1;
+    const double tau0 = (*step) * g.dt, tau1 = tau0 + g.dt;
+    const double u = U[idx];
+    const double l0 = apply_a0(g, U, i, j, tau0);
+    const double l1 = apply_a1(g, U, i, j, tau0);
+    const double l2 = apply_a2(g, U, i, j, tau0);
+    const double y0 = u + g.dt * (l0 + l1 + l2);
+    const double th = g.th * g.dt;
+    R1[idx] = y0 - th * l1 + th * bnd_a1(g, i, j, tau1);
+    W[idx] = -th * l2 + th * bnd_a2(g, i, j, tau1);
+}
+
+__global__ void combine_kernel(Grid g, const double* Y1, const double* W, double* X, int* step)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx == 0) *step += 1;   // read by the next step's explicit kernel only
+    if (idx >= (long long)g.ns * g.nv) return;
+    X[idx] = Y1[idx] + W[idx];
+}
+
+Grid make_grid(const pcrp_heston_params* p)
+{
+    Grid g;
+    g.ns = p->ns; g.nv = p->nv;
+    g.smax = p->smax; g.vmax = p->vmax;
+    g.hS = p->smax / (p->ns + 1);
+    g.hv = p->vmax / p->nv;
+    g.kappa = p->kappa; g.thv = p->theta_v; g.sigma = p->sigma; g.rho = p->rho; g.r = p->r;
+    g.K = p->strike; g.dt = p->dt; g.th = p->theta_scheme;
+    return g;
+}
+
+struct HestonWs {
+    double *sa, *sb, *sc, *sd, *se, *va, *vb, *vc, *vd, *ve, *R1, *W;
+    int* step;
+    void* solver_ws;
+    size_t solver_bytes;
+};
+
+size_t solver_ws_bytes(const pcrp_heston_params* p, int precision)
+{
+    size_t a = pcrp_workspace_size(p->ns, p->nv, precision);
+    size_t b = pcrp_workspace_size(p->nv, p->ns, precision);
+    return a > b ? a : b;
+}
+
+size_t heston_ws_bytes(const pcrp_heston_params* p, int precision)
+{
+    const size_t N = (size_t)p->ns * p->nv;
+    size_t bytes = 12 * N * sizeof(double) + 256;
+    bytes = (bytes + 255) / 256 * 256;
+    return bytes + solver_ws_bytes(p, precision);
+}
+
+HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
+{
+    const size_t N = (size_t)p->ns * p->nv;
+    double* d = static_cast<double*>(ws);
+    HestonWs h;
+    double** slots[12] = {&h.sa, &h.sb, &h.sc, &h.sd, &h.se, &h.va,
+                          &h.vb, &h.vc, &h.vd, &h.ve, &h.R1, &h.W};
+    for (int k = 0; k < 12; ++k) *slots[k] = d + (size_t)k * N;
+    h.step = reinterpret_cast<int*>(d + 12 * N);
+    size_t off = 12 * N * sizeof(double) + 256;
+    off = (off + 255) / 256 * 256;
+    h.solver_ws = static_cast<char*>(ws) + off;
+    h.solver_bytes = solver_ws_bytes(p, precision);
+    return h;
+}
+
+int solve_lines(int precision, const double* a, const double* b, const double* c, const double* d,
+                const double* e, double* y, int n, int m, int layout, const HestonWs& h,
+                hipStream_t st)
+{
+    if (precision == PCRP_PREC_F64)
+        return pcrp_solve_f64(a, b, c, d, e, y, n, m, layout, h.solver_ws, h.solver_bytes, st);
+    return pcrp_solve_f32r_f64(a, b, c, d, e, y, n, m, layout, 8, 1e-13, nullptr, h.solver_ws,
+                               h.solver_bytes, st);
+}
+
+// one Douglas step: four launches, fixed pointers
+int enqueue_step(const Grid& g, int precision, double* U, const HestonWs& h, hipStream_t st)
+{
+    const long long N = (long long)g.ns * g.nv;
+    const int bs = 256;
+    const unsigned blocks = (unsigned)((N + bs - 1) / bs);
+    hipLaunchKernelGGL(explicit_kernel, dim3(blocks), dim3(bs), 0, st, g, U, h.R1, h.W, h.step);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    int rc = solve_lines(precision, h.sa, h.sb, h.sc, h.sd, h.se, h.R1, g.ns, g.nv,
+                         PCRP_LAYOUT_CONTIGUOUS, h, st);
+    if (rc != PCRP_OK) return rc;
+    hipLaunchKernelGGL(combine_kernel, dim3(blocks), dim3(bs), 0, st, g, h.R1, h.W, U, h.step);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    return solve_lines(precision, h.va, h.vb, h.vc, h.vd, h.ve, U, g.nv, g.ns,
+                       PCRP_LAYOUT_INTERLEAVED, h, st);
+}
+
+}  // namespace
+}  // namespace pcrp
+
+extern "C" {
+
+size_t pcrp_heston_workspace_size(const pcrp_heston_params* p, int precision)
+{
+    if (!p || p->ns < 2 || p->nv < 2) return 0;
+    return pcrp::heston_ws_bytes(p, precision);
+}
+
+int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph, double* u,
+                      void* ws, size_t ws_bytes, void* stream)
+{
+    using namespace pcrp;
+    if (!p || !u || !ws) return PCRP_ERR_NULL;
+    if (p->ns < 2 || p->nv < 2 || p->steps < 0) return PCRP_ERR_INVALID_SIZE;
+    if ((long long)p->ns * p->nv > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
+    if (precision != PCRP_PREC_F64 && precision != PCRP_PREC_F32_REFINE) return PCRP_ERR_ARG;
+    if (!(p->dt > 0.0) || !(p->smax > 0.0) || !(p->vmax > 0.0)) return PCRP_ERR_ARG;
+    if (ws_bytes < heston_ws_bytes(p, precision)) return PCRP_ERR_WORKSPACE;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const Grid g = make_grid(p);
+    const HestonWs h = carve(p, precision, ws);
+    const long long N = (long long)g.ns * g.nv;
+    const int bs = 256;
+    const unsigned blocks = (unsigned)((N + bs - 1) / bs);
+    hipLaunchKernelGGL(setup_lines_kernel, dim3(blocks), dim3(bs), 0, st, g, h.sa, h.sb, h.sc,
+                       h.sd, h.se, h.va, h.vb, h.vc, h.vd, h.ve);
+    hipLaunchKernelGGL(payoff_kernel, dim3(blocks), dim3(bs), 0, st, g, u, h.step);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    if (!use_graph) {
+        for (int n = 0; n < p->steps; ++n) {
+            const int rc = enqueue_step(g, precision, u, h, st);
+            if (rc != PCRP_OK) return rc;
+        }
+        return PCRP_OK;
+    }
+    if (p->steps == 0) return PCRP_OK;
+    if (st == nullptr) return PCRP_ERR_ARG;   // the null stream cannot be captured
+    // capture one step, replay it `steps` times
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PCRP_ERR_HIP;
+    const int rc = enqueue_step(g, precision, u, h, st);
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (rc != PCRP_OK || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != PCRP_OK ? rc : PCRP_ERR_HIP;
+    }
+    int out = PCRP_OK;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        out = PCRP_ERR_HIP;
+    } else {
+        for (int n = 0; n < p->steps && out == PCRP_OK; ++n)
+            if (hipGraphLaunch(exec, st) != hipSuccess) out = PCRP_ERR_HIP;
+        // replays finish before the executable graph is released
+        if (hipStreamSynchronize(st) != hipSuccess) out = PCRP_ERR_HIP;
+        if (hipGraphExecDestroy(exec) != hipSuccess) out = PCRP_ERR_HIP;
+    }
+    (void)hipGraphDestroy(graph);
+    return out;
+}
+
+}  // extern "C"

@@ -82,6 +82,10 @@ def _declare(lib):
     lib.pcrp_generate_uneven.argtypes = [ctypes.c_int, ctypes.c_uint64] + [_vp] * 2 + \
         [ctypes.c_int] + [_vp] * 6 + [_vp]
     lib.pcrp_generate_uneven.restype = ctypes.c_int
+    lib.pcrp_heston_workspace_size.argtypes = [_vp, ctypes.c_int]
+    lib.pcrp_heston_workspace_size.restype = ctypes.c_size_t
+    lib.pcrp_heston_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_size_t, _vp]
+    lib.pcrp_heston_solve.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -322,3 +326,47 @@ def config4_sizes(m: int = 10000, seed: int = 1, lo: int = 64, hi: int = 131072)
     f = (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
     ln_lo, ln_hi = np.log(lo), np.log(hi)
     return np.floor(np.exp(ln_lo + (ln_hi - ln_lo) * f)).astype(np.int32)
+
+
+# ---- Heston ADI driver ---------------------------------------------------------
+
+class HestonParams(ctypes.Structure):
+    """Mirror of pcrp_heston_params (include/pcr_penta.h)."""
+    _fields_ = [("ns", ctypes.c_int), ("nv", ctypes.c_int),
+                ("smax", ctypes.c_double), ("vmax", ctypes.c_double),
+                ("kappa", ctypes.c_double), ("theta_v", ctypes.c_double),
+                ("sigma", ctypes.c_double), ("rho", ctypes.c_double),
+                ("r", ctypes.c_double), ("strike", ctypes.c_double),
+                ("dt", ctypes.c_double), ("steps", ctypes.c_int),
+                ("theta_scheme", ctypes.c_double)]
+
+
+# config 3: Ns = 1024, Nv = 512 unknowns, 200 steps of 1/200
+CONFIG3 = dict(ns=1024, nv=512, smax=800.0, vmax=5.0, kappa=3.0, theta_v=0.04, sigma=0.15,
+               rho=-0.7, r=0.03, strike=100.0, dt=1.0 / 200, steps=200, theta_scheme=0.5)
+
+
+def heston_params(**kw) -> HestonParams:
+    d = dict(CONFIG3)
+    d.update(kw)
+    return HestonParams(**d)
+
+
+def heston(params: HestonParams, precision: int = PREC_F64, use_graph: bool = True, stream=None,
+           ws=None, u=None):
+    """Price grid u[j*Ns + i-1] at tau = steps*dt (device tensor)."""
+    import torch
+    lib = load()
+    nb = int(lib.pcrp_heston_workspace_size(ctypes.byref(params), precision))
+    if nb == 0:
+        raise PCRPError(1, "pcrp_heston_workspace_size")
+    if ws is None:
+        ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    if u is None:
+        u = torch.empty(params.ns * params.nv, dtype=torch.float64, device="cuda")
+    if use_graph and stream is None:
+        raise ValueError("graph capture needs a non-default stream")
+    st = lib.pcrp_heston_solve(ctypes.byref(params), precision, int(use_graph), _ptr(u), _ptr(ws),
+                               ws.numel(), _stream(stream))
+    _check(st, "pcrp_heston_solve")
+    return u

@@ -1,0 +1,135 @@
+"""CPU restatement of the Heston ADI driver (pcr-penta_amd/csrc/pcrp_heston.hip),
+used only as a test reference: same grid, stencils, boundary terms and Douglas
+steps in numpy, with the line solves done by the CPU oracle's PCR-Penta.
+Also a semi-analytic Heston call price (characteristic-function integral,
+Albrecher et al. 2007 formulation) for validating the discretisation."""
+import numpy as np
+from scipy.integrate import quad
+
+
+def coefs(p):
+    ns, nv = p["ns"], p["nv"]
+    hS = p["smax"] / (ns + 1)
+    hv = p["vmax"] / nv
+    i = np.arange(1, ns + 1)
+    j = np.arange(nv)
+    S = i * hS
+    v = j * hv
+    r = p["r"]
+    # A1 weights per (j, i) at offsets -2..2
+    D = 0.5 * v[:, None] * S[None, :] ** 2
+    C = r * S[None, :] * np.ones((nv, 1))
+    w1 = np.zeros((5, nv, ns))
+    d2, d1 = D / (12.0 * hS * hS), C / (12.0 * hS)
+    w1[0], w1[1], w1[2] = -d2 + d1, 16.0 * d2 - 8.0 * d1, -30.0 * d2 - 0.5 * r
+    w1[3], w1[4] = 16.0 * d2 + 8.0 * d1, -d2 - d1
+    for col in (0, ns - 1):   # i = 1 and i = ns: 3-point
+        d2e, d1e = D[:, col] / (hS * hS), C[:, col] / (2.0 * hS)
+        w1[0, :, col] = 0.0
+        w1[1, :, col] = d2e - d1e
+        w1[2, :, col] = -2.0 * d2e - 0.5 * r
+        w1[3, :, col] = d2e + d1e
+        w1[4, :, col] = 0.0
+    # A2 weights per j
+    w2 = np.zeros((5, nv))
+    diff = 0.5 * p["sigma"] ** 2 * v / (hv * hv)
+    mu = p["kappa"] * (p["theta_v"] - v)
+    w2[2] = -0.5 * r
+    jj = j >= 1
+    w2[1, jj] += diff[jj]; w2[2, jj] += -2.0 * diff[jj]; w2[3, jj] += diff[jj]
+    for k in range(nv):
+        if mu[k] >= 0:
+            if k + 2 <= nv:
+                f = mu[k] / (2 * hv); w2[2, k] += -3 * f; w2[3, k] += 4 * f; w2[4, k] += -f
+            else:
+                f = mu[k] / hv; w2[2, k] += -f; w2[3, k] += f
+        else:
+            if k >= 2:
+                f = mu[k] / (2 * hv); w2[2, k] += 3 * f; w2[1, k] += -4 * f; w2[0, k] += f
+            else:
+                f = mu[k] / hv; w2[2, k] += f; w2[1, k] += -f
+    return hS, hv, S, v, w1, w2
+
+
+def padded(p, U, tau, interior=True):
+    """Node values on a padded grid: entry [j+2, i+1] holds node (i, j) for
+    i in -1..ns+2, j in -2..nv+1 (boundary nodes i = 0, ns+1 and j = nv from
+    the boundary conditions at tau; nodes beyond them are never weighted)."""
+    ns, nv = p["ns"], p["nv"]
+    hS = p["smax"] / (ns + 1)
+    G = np.zeros((nv + 4, ns + 4))
+    if interior:
+        G[2:nv + 2, 2:ns + 2] = U
+    G[2:nv + 3, ns + 2] = p["smax"] - p["strike"] * np.exp(-p["r"] * tau)
+    G[nv + 2, 2:ns + 2] = np.arange(1, ns + 1) * hS
+    return G
+
+
+def adi(p, oracle, precision_solver=None):
+    """Run the Douglas scheme; returns U (nv x ns)."""
+    ns, nv, dt, th = p["ns"], p["nv"], p["dt"], p["theta_scheme"]
+    hS, hv, S, v, w1, w2 = coefs(p)
+    f = -th * dt
+    # line systems
+    sa = np.where(np.arange(1, ns + 1)[None, :] - 2 >= 1, f * w1[0], 0.0)
+    sb = np.where(np.arange(1, ns + 1)[None, :] - 1 >= 1, f * w1[1], 0.0)
+    sc = 1.0 + f * w1[2]
+    sd = np.where(np.arange(1, ns + 1)[None, :] + 1 <= ns, f * w1[3], 0.0)
+    se = np.where(np.arange(1, ns + 1)[None, :] + 2 <= ns, f * w1[4], 0.0)
+    jv = np.arange(nv)[:, None] * np.ones((1, ns))
+    va = np.where(jv - 2 >= 0, f * w2[0][:, None], 0.0)
+    vb = np.where(jv - 1 >= 0, f * w2[1][:, None], 0.0)
+    vc = 1.0 + f * w2[2][:, None] * np.ones((1, ns))
+    vd = np.where(jv + 1 <= nv - 1, f * w2[3][:, None], 0.0)
+    ve = np.where(jv + 2 <= nv - 1, f * w2[4][:, None], 0.0)
+    U = np.maximum(S[None, :] - p["strike"], 0.0) * np.ones((nv, 1))
+    for n in range(p["steps"]):
+        t0, t1 = n * dt, (n + 1) * dt
+        G0 = padded(p, U, t0)
+        B1 = padded(p, None, t1, interior=False)   # boundary nodes only, at t1
+
+        def shiftS(G, o):
+            return G[2:nv + 2, 2 + o:ns + 2 + o]
+
+        def shiftV(G, o):
+            return G[2 + o:nv + 2 + o, 2:ns + 2]
+        l1 = sum(w1[k] * shiftS(G0, k - 2) for k in range(5))
+        l2 = sum(w2[k][:, None] * shiftV(G0, k - 2) for k in range(5))
+        # mixed term (zero at v = 0)
+        c0 = p["rho"] * p["sigma"] * v[:, None] * S[None, :] / (4 * hS * hv)
+        l0 = c0 * (G0[3:nv + 3, 3:ns + 3] - G0[1:nv + 1, 3:ns + 3]
+                   - G0[3:nv + 3, 1:ns + 1] + G0[1:nv + 1, 1:ns + 1])
+        l0[0] = 0.0
+        y0 = U + dt * (l0 + l1 + l2)
+        # boundary parts at t1: S = 0 contributes 0, S = Smax and v = vmax nodes
+        b1 = sum(w1[k] * shiftS(B1, k - 2) for k in range(5))
+        b2 = sum(w2[k][:, None] * shiftV(B1, k - 2) for k in range(3, 5))
+        R1 = y0 - th * dt * l1 + th * dt * b1
+        W = -th * dt * l2 + th * dt * b2
+        Y1 = oracle.pcr_batch(sa.ravel(), sb.ravel(), sc.ravel(), sd.ravel(), se.ravel(),
+                              R1.ravel(), ns, nv, oracle.CONTIGUOUS).reshape(nv, ns)
+        X = Y1 + W
+        U = oracle.pcr_batch(va.ravel(), vb.ravel(), vc.ravel(), vd.ravel(), ve.ravel(),
+                             X.ravel(), nv, ns, oracle.INTERLEAVED).reshape(nv, ns)
+    return U
+
+
+def heston_call(S0, v0, K, T, r, kappa, theta, sigma, rho):
+    """Semi-analytic Heston European call (Albrecher et al. 2007 'little trap')."""
+    def phi(u):
+        d = np.sqrt((rho * sigma * 1j * u - kappa) ** 2 + sigma ** 2 * (1j * u + u * u))
+        g = (kappa - rho * sigma * 1j * u - d) / (kappa - rho * sigma * 1j * u + d)
+        C = r * 1j * u * T + kappa * theta / sigma ** 2 * (
+            (kappa - rho * sigma * 1j * u - d) * T - 2 * np.log((1 - g * np.exp(-d * T)) / (1 - g)))
+        D = (kappa - rho * sigma * 1j * u - d) / sigma ** 2 * (1 - np.exp(-d * T)) / (1 - g * np.exp(-d * T))
+        return np.exp(C + D * v0 + 1j * u * np.log(S0))
+
+    def integrand(u, j):
+        if j == 1:
+            val = np.exp(-1j * u * np.log(K)) * phi(u - 1j) / (1j * u * phi(-1j))
+        else:
+            val = np.exp(-1j * u * np.log(K)) * phi(u) / (1j * u)
+        return val.real
+    P1 = 0.5 + quad(lambda u: integrand(u, 1), 1e-10, 200, limit=400)[0] / np.pi
+    P2 = 0.5 + quad(lambda u: integrand(u, 2), 1e-10, 200, limit=400)[0] / np.pi
+    return S0 * P1 - K * np.exp(-r * T) * P2

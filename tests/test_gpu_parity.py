@@ -327,3 +327,58 @@ def test_uneven_config4_sample(gpu, oracle_lib):
     _say("uneven-c4", len(sizes), int(sizes.sum()))
     x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 0, 1)
     assert np.array_equal(x, ref)
+
+
+# ---------------- Heston ADI driver (hipGraph time steps) ----------------
+
+def _heston_gpu(gpu, precision, use_graph, **kw):
+    import torch
+    p = gpu.heston_params(**kw)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        u = gpu.heston(p, precision=precision, use_graph=use_graph, stream=s)
+    s.synchronize()
+    return _np(u).reshape(p.nv, p.ns), p
+
+
+def test_heston_matches_cpu_restatement(gpu, oracle_lib):
+    """Small grid, FP64: the GPU Douglas steps equal a numpy restatement whose
+    line solves use the CPU oracle (normwise relative difference <= 1e-12;
+    the stencil sums round in a different order)."""
+    import heston_ref as H
+    kw = dict(ns=64, nv=32, steps=20)
+    _say("heston-small")
+    U, p = _heston_gpu(gpu, gpu.PREC_F64, False, **kw)
+    d = dict(gpu.CONFIG3)
+    d.update(kw)
+    ref = H.adi(d, oracle_lib)
+    err = np.abs(U - ref).max() / np.abs(ref).max()
+    assert err <= 1e-12, err
+
+
+def test_heston_graph_replay_equals_eager(gpu):
+    _say("heston-graph")
+    kw = dict(ns=128, nv=64, steps=30)
+    for prec in (gpu.PREC_F64, gpu.PREC_F32_REFINE):
+        Ue, _ = _heston_gpu(gpu, prec, False, **kw)
+        Ug, _ = _heston_gpu(gpu, prec, True, **kw)
+        assert np.array_equal(Ue, Ug), prec
+
+
+def test_heston_config3_price_and_refine(gpu):
+    """Config 3 (Ns=1024, Nv=512, 200 steps): the FP64 price at S=100, v=0.04
+    is within 2e-3 relative of the semi-analytic Heston price, and FP32+refine
+    agrees with FP64 to 1e-10 normwise."""
+    import heston_ref as H
+    from scipy.interpolate import RectBivariateSpline
+    _say("heston-c3")
+    U64, p = _heston_gpu(gpu, gpu.PREC_F64, True)
+    U32, _ = _heston_gpu(gpu, gpu.PREC_F32_REFINE, True)
+    hS, hv = p.smax / (p.ns + 1), p.vmax / p.nv
+    f = RectBivariateSpline(np.arange(p.nv) * hv, np.arange(1, p.ns + 1) * hS, U64)
+    price = f(0.04, 100.0)[0, 0]
+    ref = H.heston_call(100.0, 0.04, p.strike, p.steps * p.dt, p.r, p.kappa, p.theta_v, p.sigma,
+                        p.rho)
+    print("heston price", price, "semi-analytic", ref, flush=True)
+    assert abs(price - ref) / ref < 2e-3
+    assert np.abs(U32 - U64).max() / np.abs(U64).max() <= 1e-10
