@@ -91,6 +91,28 @@ __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* 
     }
 }
 
+// Block -> (group, tile). Groups of G neighbouring sub-systems share cache
+// lines (rows are interleaved across sub-systems), so the groups of one tile
+// are made consecutive and kept on one XCD: blocks are dealt round-robin to
+// the 8 XCDs, so block b runs on XCD b % 8 and the b / 8-th block there takes
+// (tile, group) with the group index fastest and tiles striped over XCDs.
+// Blocks past the last tile return at once (whole workgroup).
+__device__ __forceinline__ bool tile_of_block(int b, int groups, int tiles, int& grp, int& tile)
+{
+    constexpr int NX = 8;
+    const int xcd = b % NX, s = b / NX;
+    const int tl = s / groups;
+    grp = s - tl * groups;
+    tile = tl * NX + xcd;
+    return tile < tiles;
+}
+
+__host__ __forceinline__ long long blocks_for(int groups, int tiles)
+{
+    constexpr int NX = 8;
+    return (long long)groups * ((tiles + NX - 1) / NX) * NX;
+}
+
 // Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
 template <int TB, int G>
 __global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
@@ -103,7 +125,8 @@ __global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
     __shared__ double sR[6][TB * G];
     const int H = 2 * ((1 << k) - 1);
     const int T = TB - 2 * H;
-    const int grp = blockIdx.x / tiles, tile = blockIdx.x - grp * tiles;
+    int grp, tile;
+    if (!tile_of_block(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
     const int lr = threadIdx.x / G, g = threadIdx.x - lr * G;
     const int q = grp * G + g;
     const int l = tile * T - H + lr;
@@ -142,7 +165,8 @@ __global__ __launch_bounds__(TB* G) void hbm_pass_uneven_f64_kernel(
     }
     const PassSys P = tab[lo];
     const int local = (int)(b - P.block0);
-    const int grp = local / P.tiles, tile = local - grp * P.tiles;
+    int grp, tile;
+    if (!tile_of_block(local, ((1 << Sin) + G - 1) / G, P.tiles, grp, tile)) return;
     const int k = P.k;
     const int H = 2 * ((1 << k) - 1);
     const int T = TB - 2 * H;
@@ -169,8 +193,8 @@ static int launch_pass(const double* const in[6], double* const out[6], int n0, 
     if (T < 1) return -2;
     const int L = (n0 + (1 << Sin) - 1) >> Sin;   // longest current sub-system
     const int tiles = (L + T - 1) / T;
-    const long long groups = (Qin + G - 1) / G;
-    const long long blocks = groups * tiles;
+    const int groups = (Qin + G - 1) / G;
+    const long long blocks = blocks_for(groups, tiles);
     if (blocks > 0x7fffffffLL) return -2;
     hipLaunchKernelGGL((hbm_pass_f64_kernel<TB, G>), dim3((unsigned)blocks), dim3(TB * G), 0, st,
                        in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2], out[3],

@@ -29,6 +29,9 @@ struct SolveArgsF64 {
     SplitInfo split;
 };
 
+// Work queues of the persistent LDS kernel: one per XCD.
+constexpr int kQueues = 8;
+
 // One job of the persistent LDS queue: rows base + (i << stride_log2),
 // i < n, of array set `src`; x written to the same indices of the output.
 struct LdsJob {
@@ -52,7 +55,7 @@ struct LdsVariantF64 {
     // persistent work-queue form over a job list (jobs read array set
     // sets[6*src .. 6*src+5], write x into x at the job's indices)
     int (*launch_queue)(const double* const* sets, double* x, const LdsJob* jobs, int njobs,
-                        unsigned int* counter, hipStream_t);
+                        const int* qstart, unsigned int* counters, hipStream_t);
 };
 
 int lds_f64_variant_count();

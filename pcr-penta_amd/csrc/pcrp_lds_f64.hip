@@ -255,31 +255,40 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
     lds_solve_one<KIND, NT, RPT>(ga, gb, gc, gd, ge, gy, gx, base, rs, xbase, xrs, n, steps, sR, sE);
 }
 
-// Persistent work-queue form: each workgroup takes the next job from an
-// atomic counter until the list is exhausted (every wave reaches the exit).
-// Jobs are whole systems or residue sub-systems of split systems; src picks
-// the array set a job reads (the caller's arrays or a pass workspace).
+// Persistent work-queue form. Jobs (whole systems or residue sub-systems of
+// split systems) are split into one queue per XCD; a workgroup drains the
+// queue of the XCD it runs on (hardware register XCC_ID), then helps the
+// others, so every wave reaches the exit once all queues are empty. Residue
+// sub-systems of one system read interleaved rows that share cache lines, and
+// the host puts runs of them into the same queue, i.e. behind the same L2.
+// src picks the array set a job reads (the caller's arrays or a pass
+// workspace); x is written at the job's own indices.
 template <int KIND, int NT, int RPT>
 __global__ __launch_bounds__(NT) void lds_f64_queue_kernel(
     const double* const* __restrict__ sets, double* __restrict__ gx, const LdsJob* __restrict__ jobs,
-    int njobs, unsigned int* __restrict__ counter)
+    const int* __restrict__ qstart, unsigned int* __restrict__ counters)
 {
     constexpr int NMAX = NT * RPT;
     __shared__ double sR[6][NMAX];
     __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
     __shared__ int sJob;
-    for (;;) {
-        if (threadIdx.x == 0) sJob = (int)atomicAdd(counter, 1u);
-        __syncthreads();
-        const int jid = sJob;
-        __syncthreads();
-        if (jid >= njobs) break;
-        const LdsJob J = jobs[jid];
-        const double* const* S = sets + 6 * J.src;
-        const long long rs = 1LL << J.stride_log2;
-        lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base, rs, J.base,
-                                     rs, J.n, J.steps, sR, sE);
-        __syncthreads();
+    const int home = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kQueues - 1));
+    for (int t = 0; t < kQueues; ++t) {
+        const int qx = (home + t) & (kQueues - 1);
+        const int begin = qstart[qx], len = qstart[qx + 1] - begin;
+        for (;;) {
+            if (threadIdx.x == 0) sJob = (int)atomicAdd(&counters[qx], 1u);
+            __syncthreads();
+            const int idx = sJob;
+            __syncthreads();
+            if (idx >= len) break;
+            const LdsJob J = jobs[begin + idx];
+            const double* const* S = sets + 6 * J.src;
+            const long long rs = 1LL << J.stride_log2;
+            lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base, rs,
+                                         J.base, rs, J.n, J.steps, sR, sE);
+            __syncthreads();
+        }
     }
 }
 
@@ -296,7 +305,7 @@ static int launch(const SolveArgsF64& A, hipStream_t st)
 // Persistent launch: one workgroup per resident slot (CUs x blocks per CU).
 template <int KIND, int NT, int RPT>
 static int launch_queue(const double* const* sets, double* x, const LdsJob* jobs, int njobs,
-                        unsigned int* counter, hipStream_t st)
+                        const int* qstart, unsigned int* counters, hipStream_t st)
 {
     static int grid = 0;
     if (grid == 0) {
@@ -312,7 +321,7 @@ static int launch_queue(const double* const* sets, double* x, const LdsJob* jobs
     const int g = njobs < grid ? njobs : grid;
     if (g <= 0) return 0;
     hipLaunchKernelGGL((lds_f64_queue_kernel<KIND, NT, RPT>), dim3(g), dim3(NT), 0, st, sets, x,
-                       jobs, njobs, counter);
+                       jobs, qstart, counters);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -351,18 +360,18 @@ int lds_f64_lds_bytes(const LdsVariantF64& v)
                                                 : LdsWords<KIND_REDUNDANT>::value);
 }
 
-// Default choice before tuning: the smallest workgroup of 256 threads or
-// fewer (RPT up to 4) whose capacity covers n, redundant form.
+// Default choice before tuning (measured on MI355X at n = 1024): the
+// redundant form with 256 threads per workgroup and as many rows per thread as
+// n needs (up to 4); larger workgroups for n > 1024 never occur here.
 int lds_f64_default_variant(int n)
 {
-    int best = -1;
+    int best = -1, best_score = 1 << 30;
     for (int id = 0; id < lds_f64_variant_count(); ++id) {
         const LdsVariantF64& v = kVariants[id];
         if (v.kind != KIND_REDUNDANT || v.max_n < n) continue;
-        if (best < 0) { best = id; continue; }
-        const LdsVariantF64& b = kVariants[best];
-        // prefer fewer total rows (tighter fit), then more threads
-        if (v.max_n < b.max_n || (v.max_n == b.max_n && v.nt > b.nt && v.nt <= 256)) best = id;
+        // tightest capacity first, then the workgroup size closest to 256
+        const int score = v.max_n * 4 + (v.nt > 256 ? v.nt / 256 : 256 / v.nt);
+        if (score < best_score) { best = id; best_score = score; }
     }
     return best;
 }

@@ -11,10 +11,11 @@
 //     passes (k_fuse steps per pass); all large systems share each pass round
 //     (one launch per round, a table maps blocks to systems); then every one of
 //     its 2^S residue sub-systems is one LDS job.
-// All LDS jobs go through one persistent kernel: each workgroup pulls the
-// next job from an atomic counter, largest systems first (longest-processing-
-// time order), sub-systems of one system adjacent so their interleaved rows
-// share cache lines.
+// All LDS jobs go through one persistent kernel with one work queue per XCD:
+// a workgroup drains its own XCD's queue, then the others. Runs of 16
+// consecutive residue sub-systems of a system (whose rows share cache lines)
+// stay in one queue, behind one L2; runs go to the least-loaded queue,
+// largest first (longest-processing-time order).
 // Results are bit-identical to solving each system alone (same arithmetic,
 // same guards; the pass rounds are the split path with one system per batch).
 #include "pcr_penta.h"
@@ -27,14 +28,18 @@ namespace pcrp {
 
 namespace {
 
-__global__ void reset_counter_kernel(unsigned int* c) { *c = 0u; }
+__global__ void reset_counters_kernel(unsigned int* c)
+{
+    if (threadIdx.x < kQueues) c[threadIdx.x] = 0u;
+}
 
 constexpr size_t kAlign = 256;
 
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct UnevenPlan {
-    std::vector<LdsJob> jobs;
+    std::vector<LdsJob> jobs;                   // queue 0 jobs, then queue 1, ...
+    int qstart[kQueues + 1] = {0};
     std::vector<std::vector<PassSys>> rounds;   // per pass round
     std::vector<long long> round_blocks;
     std::vector<int> round_shape;
@@ -62,20 +67,44 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
     if (P.total > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
     std::stable_sort(order.begin(), order.end(),
                      [](const Sys& x, const Sys& y) { return x.nsub > y.nsub; });
-    // LDS jobs
+    // LDS jobs, in chunks: a small system alone, or up to 16 consecutive
+    // residue sub-systems of a split system (their rows share cache lines).
+    // Chunks go to the least-loaded XCD queue, largest first.
+    struct Chunk { long long work; std::vector<LdsJob> jobs; };
+    std::vector<Chunk> chunks;
     for (const Sys& s : order) {
         const int n = h_n[s.j];
         const int steps = pcr_steps(n);
         if (s.S == 0) {
-            P.jobs.push_back({h_off[s.j], 0, n, steps, 0});
-        } else {
-            const int rounds = (s.S + tune.k_fuse - 1) / tune.k_fuse;
-            const int src = 1 + ((rounds - 1) & 1);
-            for (int q = 0; q < (1 << s.S); ++q)
-                P.jobs.push_back({h_off[s.j] + q, s.S, (n - q + (1 << s.S) - 1) >> s.S,
-                                  steps - s.S, src});
+            chunks.push_back({(long long)n, {LdsJob{h_off[s.j], 0, n, steps, 0}}});
+            continue;
+        }
+        const int rounds = (s.S + tune.k_fuse - 1) / tune.k_fuse;
+        const int src = 1 + ((rounds - 1) & 1);
+        for (int q0 = 0; q0 < (1 << s.S); q0 += 16) {
+            Chunk c{0, {}};
+            for (int q = q0; q < q0 + 16 && q < (1 << s.S); ++q) {
+                const int nq = (n - q + (1 << s.S) - 1) >> s.S;
+                c.jobs.push_back(LdsJob{h_off[s.j] + q, s.S, nq, steps - s.S, src});
+                c.work += nq;
+            }
+            chunks.push_back(std::move(c));
         }
     }
+    std::vector<std::vector<LdsJob>> queues(kQueues);
+    long long load[kQueues] = {0};
+    for (Chunk& c : chunks) {   // already largest first (systems sorted by sub-system size)
+        int best = 0;
+        for (int x = 1; x < kQueues; ++x)
+            if (load[x] < load[best]) best = x;
+        load[best] += c.work;
+        queues[best].insert(queues[best].end(), c.jobs.begin(), c.jobs.end());
+    }
+    for (int x = 0; x < kQueues; ++x) {
+        P.qstart[x] = (int)P.jobs.size();
+        P.jobs.insert(P.jobs.end(), queues[x].begin(), queues[x].end());
+    }
+    P.qstart[kQueues] = (int)P.jobs.size();
     // HBM pass rounds: round r covers steps [r*k_fuse, min(S, (r+1)*k_fuse))
     for (int r = 0;; ++r) {
         const int Sin = r * tune.k_fuse;
@@ -95,7 +124,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
             const int tiles = (L + T - 1) / T;
             const int groups = ((1 << Sin) + G - 1) / G;
             tab.push_back({h_off[j], blocks, h_n[j], k, tiles, 0});
-            blocks += (long long)groups * tiles;
+            blocks += (long long)groups * ((tiles + 7) / 8) * 8;   // XCD-striped tiles
         }
         if (tab.empty()) break;
         P.rounds.push_back(std::move(tab));
@@ -104,7 +133,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
     }
     // workspace layout
     size_t o = 0;
-    P.off_counter = o; o = align_up(o + sizeof(unsigned int));
+    P.off_counter = o; o = align_up(o + kQueues * sizeof(unsigned int) + sizeof(P.qstart));
     P.off_sets = o;    o = align_up(o + 18 * sizeof(double*));
     P.off_jobs = o;    o = align_up(o + P.jobs.size() * sizeof(LdsJob));
     for (const auto& t : P.rounds) {
@@ -142,6 +171,9 @@ int uneven_f64_solve(const double* a, const double* b, const double* c, const do
                           ? reinterpret_cast<double*>(w + P.off_arrays) + (size_t)v * P.total
                           : nullptr;
     std::vector<char> host(P.off_arrays);
+    std::copy(reinterpret_cast<const char*>(P.qstart),
+              reinterpret_cast<const char*>(P.qstart) + sizeof(P.qstart),
+              host.begin() + P.off_counter + kQueues * sizeof(unsigned int));
     std::copy(reinterpret_cast<const char*>(sets), reinterpret_cast<const char*>(sets) + sizeof sets,
               host.begin() + P.off_sets);
     std::copy(reinterpret_cast<const char*>(P.jobs.data()),
@@ -164,12 +196,13 @@ int uneven_f64_solve(const double* a, const double* b, const double* c, const do
         if (rc != 0) return PCRP_ERR_HIP;
     }
     // one persistent LDS launch over every job
-    unsigned int* counter = reinterpret_cast<unsigned int*>(w + P.off_counter);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, st, counter);
+    unsigned int* counters = reinterpret_cast<unsigned int*>(w + P.off_counter);
+    const int* qstart = reinterpret_cast<const int*>(w + P.off_counter + kQueues * sizeof(unsigned int));
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counters);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, split_f64_get().lds_max));
     rc = v->launch_queue(dsets, y, reinterpret_cast<const LdsJob*>(w + P.off_jobs),
-                         (int)P.jobs.size(), counter, st);
+                         (int)P.jobs.size(), qstart, counters, st);
     return rc == 0 ? PCRP_OK : PCRP_ERR_HIP;
 }
 
