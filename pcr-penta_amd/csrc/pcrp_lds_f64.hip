@@ -20,14 +20,21 @@
 //             rows, exactly as kernel.cu:66-117 writes it: 6 divisions per
 //             row-step but only the rows in LDS (6 doubles per row), so three
 //             times as many workgroups fit per CU.
-// Both give the same bits (the pair equation is the same expression on the
+//   INPLACE : the shared form with each pair equation written over its own
+//             row in one 6-double buffer (four barriers per step), so 4
+//             divisions per row-step at the redundant form's LDS footprint.
+//             A step in which any pair equation has an exactly zero pivot
+//             (detected block-wide) runs as a redundant step instead, whose
+//             rows are still intact, because the reference's zero-pivot rule
+//             needs the neighbours' multipliers.
+// All give the same bits (the pair equation is the same expression on the
 // same operands either way).
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
 namespace pcrp {
 
-enum { KIND_SHARED = 0, KIND_REDUNDANT = 1 };
+enum { KIND_SHARED = 0, KIND_REDUNDANT = 1, KIND_INPLACE = 2 };
 
 template <int KIND>
 struct LdsWords {
@@ -70,6 +77,72 @@ __device__ __forceinline__ void register_step(Row<double> (&r)[RPT], int t, int 
 #pragma unroll
     for (int k = 0; k < RPT; ++k)
         if (t + NT * k < n) r[k] = N[k];
+}
+
+// One INPLACE step through LDS. Builds every own pair equation E(j) from the
+// current rows; if any has an exactly zero pivot (C == 0), returns true
+// without touching LDS (the caller then runs the redundant step, which keeps
+// the reference's zero-pivot rule). Otherwise writes E(j) over row j, forms
+// the new rows from E(i-d), E(i+d) and writes them back; returns false.
+template <int NT, int RPT, typename ReadR>
+__device__ __forceinline__ bool inplace_zero_pivot_step(Row<double> (&r)[RPT], int n, int delta,
+                                                        double (&sR)[6][NT * RPT], ReadR&& readR)
+{
+    const int t = threadIdx.x;
+    Pair<double> E[RPT];
+    int zero = 0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int j = t + NT * k;
+        if (j < n) {
+            const bool hasL = j - delta >= 0, hasR = j + delta <= n - 1;
+            const bool aC = j - 2 * delta >= 0, arC = j + 2 * delta <= n - 1;
+            const Row<double> L = readR(hasL ? j - delta : j);
+            const Row<double> R = readR(hasR ? j + delta : j);
+            E[k] = make_pair(L, r[k], R, hasL, aC, hasR, arC);
+            zero |= (E[k].C == 0.0);
+        }
+    }
+    if (__syncthreads_or(zero)) return true;   // uniform: every thread takes the same branch
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int j = t + NT * k;
+        if (j < n) {
+            sR[0][j] = E[k].A; sR[1][j] = E[k].B; sR[2][j] = E[k].C;
+            sR[3][j] = E[k].D; sR[4][j] = E[k].E; sR[5][j] = E[k].K;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = t + NT * k;
+        if (i < n) {
+            const bool hp1 = i - delta >= 0, hq1 = i + delta <= n - 1;
+            const bool hp2 = i - 2 * delta >= 0, hq2 = i + 2 * delta <= n - 1;
+            const int il = hp1 ? i - delta : i, ir = hq1 ? i + delta : i;
+            Pair<double> EL, ER;
+            EL.A = sR[0][il]; EL.B = sR[1][il]; EL.C = sR[2][il];
+            EL.D = sR[3][il]; EL.E = sR[4][il]; EL.K = sR[5][il];
+            ER.A = sR[0][ir]; ER.B = sR[1][ir]; ER.C = sR[2][ir];
+            ER.D = sR[3][ir]; ER.E = sR[4][ir]; ER.K = sR[5][ir];
+            // no pivot is zero in this step: the multiplier reuse never triggers
+            r[k] = reduce_row(r[k], EL, ER, hp1, hq1, hp2, hq2,
+                              [&](double& alL, double& arL, double& alR, double& arR) {
+                                  alL = arL = alR = arR = 0.0;
+                              });
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = t + NT * k;
+        if (i < n) {
+            sR[0][i] = r[k].a; sR[1][i] = r[k].b; sR[2][i] = r[k].c;
+            sR[3][i] = r[k].d; sR[4][i] = r[k].e; sR[5][i] = r[k].y;
+        }
+    }
+    __syncthreads();
+    return false;
 }
 
 // Solve one system (or sub-system) held at ga..gy[base + i*rs], i < n, with
@@ -156,6 +229,8 @@ __device__ __forceinline__ void lds_solve_one(
                 }
             }
             __syncthreads();
+        } else if (KIND == KIND_INPLACE && !inplace_zero_pivot_step<NT, RPT>(r, n, delta, sR, readR)) {
+            // fast path done inside (pair equations over the rows, 4 barriers)
         } else {
             // kernel.cu:66-140 as written: each row builds eq1 and eq2 itself.
 #pragma unroll
@@ -336,11 +411,17 @@ static const LdsVariantF64 kVariants[] = {
     PCRP_V(KIND_SHARED, 64, 4),   PCRP_V(KIND_SHARED, 128, 4),  PCRP_V(KIND_SHARED, 256, 4),
     // reference-style redundant pairs, 6 doubles of LDS per row
     PCRP_V(KIND_REDUNDANT, 64, 1),   PCRP_V(KIND_REDUNDANT, 128, 1),
+    PCRP_V(KIND_REDUNDANT, 128, 8),
     PCRP_V(KIND_REDUNDANT, 256, 1),  PCRP_V(KIND_REDUNDANT, 512, 1),
     PCRP_V(KIND_REDUNDANT, 1024, 1), PCRP_V(KIND_REDUNDANT, 64, 2),
     PCRP_V(KIND_REDUNDANT, 128, 2),  PCRP_V(KIND_REDUNDANT, 256, 2),
     PCRP_V(KIND_REDUNDANT, 512, 2),  PCRP_V(KIND_REDUNDANT, 64, 4),
     PCRP_V(KIND_REDUNDANT, 128, 4),  PCRP_V(KIND_REDUNDANT, 256, 4),
+    // pair equations written over the rows, 6 doubles of LDS per row
+    PCRP_V(KIND_INPLACE, 64, 1),    PCRP_V(KIND_INPLACE, 128, 1),  PCRP_V(KIND_INPLACE, 256, 1),
+    PCRP_V(KIND_INPLACE, 512, 1),   PCRP_V(KIND_INPLACE, 1024, 1), PCRP_V(KIND_INPLACE, 128, 2),
+    PCRP_V(KIND_INPLACE, 256, 2),   PCRP_V(KIND_INPLACE, 512, 2),  PCRP_V(KIND_INPLACE, 64, 4),
+    PCRP_V(KIND_INPLACE, 128, 4),   PCRP_V(KIND_INPLACE, 256, 4),  PCRP_V(KIND_INPLACE, 128, 8),
 };
 
 #undef PCRP_V
@@ -357,7 +438,7 @@ int lds_f64_max_n() { return 1024; }
 int lds_f64_lds_bytes(const LdsVariantF64& v)
 {
     return 8 * v.max_n * (v.kind == KIND_SHARED ? LdsWords<KIND_SHARED>::value
-                                                : LdsWords<KIND_REDUNDANT>::value);
+                                                : LdsWords<KIND_REDUNDANT>::value);   // INPLACE: 6 too
 }
 
 // Default choice before tuning (measured on MI355X at n = 1024): the

@@ -103,10 +103,15 @@ def test_zero_pivot_rule(gpu, oracle_lib):
     arrs[0][j0:j0 + n] = 0.0
     arrs[4][j0:j0 + n] = 0.0     # tridiagonal system: E(j).C = c_j when a=e=0
     arrs[2][j0 + 5] = 0.0        # zero main diagonal entry -> zero pivot at step 0
-    dev = [torch.from_numpy(v).cuda() for v in arrs]
-    x = _run_f64(gpu, dev, n, m, C)
     ref = oracle_lib.pcr_batch(*arrs, n, m, C)
-    assert np.array_equal(np.nan_to_num(x, nan=7.0), np.nan_to_num(ref, nan=7.0))
+    try:
+        for v in gpu.variants():   # every form, incl. the in-place form's fallback step
+            gpu.tune_set(n, v["id"])
+            dev = [torch.from_numpy(a).cuda() for a in arrs]
+            x = _run_f64(gpu, dev, n, m, C)
+            assert np.array_equal(np.nan_to_num(x, nan=7.0), np.nan_to_num(ref, nan=7.0)), v
+    finally:
+        gpu.tune_set(n, -1)
 
 
 def test_every_lds_variant_bit_exact(gpu, oracle_lib):
