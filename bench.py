@@ -110,9 +110,40 @@ def bench_uneven(args, m, prec, kind):
     nbuf = max(1, min(K + W, int(8e9 // (8 * total))))
     ys = [y0.clone() for _ in range(nbuf)]
     y_check = y0.clone()
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    tuned = None
+    if not args.no_tune:
+        # autotune steps fused per pass and the later rounds' tile shape on this batch
+        lds_max, _, _, n_shapes = pcrpenta.split_tune_get()
+        best = None
+        ys_t = y0.clone()
+        for kf in (2, 3, 4, 5):
+            for sh in range(n_shapes):
+                try:
+                    pcrpenta.split_tune_set(lds_max, kf, sh)
+                except pcrpenta.PCRPError:
+                    continue
+                nb = pcrpenta.uneven_workspace_size(off, nn)
+                ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+                ys_t.copy_(y0)
+                pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ys_t.copy_(y0)
+                e0.record(stream)
+                pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                t = e0.elapsed_time(e1)
+                if best is None or t < best[0]:
+                    best = (t, kf, sh)
+                del ws
+        pcrpenta.split_tune_set(lds_max, best[1], best[2])
+        tuned = {"k_fuse": best[1], "shape": best[2], "ms": best[0]}
+        print(json.dumps({"split_autotune": tuned}), file=sys.stderr, flush=True)
+        del ys_t
     nb = pcrpenta.uneven_workspace_size(off, nn)
     ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
-    stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
 
     def step(k, y=None):
@@ -252,6 +283,13 @@ def main():
     tuned = None
     if args.tune_load:
         pcrpenta.tune_load(args.tune_load)
+    elif not args.no_tune and prec == "f64" and n > 1024:
+        # split path: the autotuner picks steps fused per pass and the pass tile shape
+        kf, sh, ms = pcrpenta.split_autotune(a, b, c, d, e, y0, n, m, layout, reps=2, ws=ws)
+        tuned = {"split": [kf, sh], "ms": ms}
+        print(json.dumps({"split_autotune": tuned}), file=sys.stderr, flush=True)
+        if args.tune_save:
+            pcrpenta.tune_save(args.tune_save)
     elif not args.no_tune and prec == "f64" and n <= 1024:
         # the autotuner times every LDS variant on this very batch (outside the timed region)
         best, best_ms, times = pcrpenta.autotune(a, b, c, d, e, y0, n, m, layout, reps=3)

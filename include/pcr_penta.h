@@ -194,7 +194,15 @@ int pcrp_autotune(int precision, int n, int m, int layout,
 int pcrp_split_tune_set(int precision, int lds_max, int k_fuse, int shape);
 int pcrp_split_tune_get(int precision, int *lds_max, int *k_fuse, int *shape,
                         int *n_shapes);
-/* Persist / restore the table as text (host file). */
+/* Time every (k_fuse, shape) pair of the split path on the caller's batch
+ * (n must be above the LDS threshold; ws as for pcrp_solve_f64), keep the
+ * fastest. Synchronises `stream`. */
+int pcrp_split_autotune(int precision, int n, int m, int layout,
+                        const double *a, const double *b, const double *c,
+                        const double *d, const double *e, const double *y,
+                        double *y_work, void *ws, size_t ws_bytes, int reps, void *stream,
+                        int *best_k, int *best_shape, float *best_ms);
+/* Persist / restore the tables (LDS variants and split tunables) as text. */
 int pcrp_tune_save(const char *path);
 int pcrp_tune_load(const char *path);
 

@@ -287,12 +287,64 @@ int pcrp_split_tune_get(int precision, int* lds_max, int* k_fuse, int* shape, in
     return PCRP_OK;
 }
 
+int pcrp_split_autotune(int precision, int n, int m, int layout, const double* a, const double* b,
+                        const double* c, const double* d, const double* e, const double* y,
+                        double* y_work, void* ws, size_t ws_bytes, int reps, void* stream,
+                        int* best_k, int* best_shape, float* best_ms)
+{
+    int st = check_shape(n, m);
+    if (st != PCRP_OK) return st;
+    if (!a || !b || !c || !d || !e || !y || !y_work) return PCRP_ERR_NULL;
+    if (precision != PCRP_PREC_F64 || reps < 1) return PCRP_ERR_ARG;
+    long long rs, ss;
+    if (!layout_strides(n, m, layout, &rs, &ss)) return PCRP_ERR_LAYOUT;
+    const SplitTune keep = split_f64_get();
+    if (split_f64_levels(n, keep.lds_max) == 0) return PCRP_ERR_ARG;   // not a split size
+    if (!ws || ws_bytes < split_f64_workspace(n, m)) return PCRP_ERR_WORKSPACE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return PCRP_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return PCRP_ERR_HIP; }
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)m;
+    SplitTune best = keep;
+    float bt = -1.f;
+    for (int k = 1; k <= 6 && st == PCRP_OK; ++k) {
+        for (int sh = 0; sh < split_f64_shape_count() && st == PCRP_OK; ++sh) {
+            if (split_f64_set(SplitTune{keep.lds_max, k, sh}) != 0) continue;   // halo too wide
+            float tmin = -1.f;
+            for (int r = 0; r < reps; ++r) {
+                float t = 0.f;
+                if (hipMemcpyAsync(y_work, y, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    hipEventRecord(e0, s) != hipSuccess ||
+                    pcrp_solve_f64(a, b, c, d, e, y_work, n, m, layout, ws, ws_bytes, s) != PCRP_OK ||
+                    hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+                    st = PCRP_ERR_HIP;
+                    break;
+                }
+                if (tmin < 0.f || t < tmin) tmin = t;
+            }
+            if (st == PCRP_OK && (bt < 0.f || tmin < bt)) { bt = tmin; best = SplitTune{keep.lds_max, k, sh}; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    split_f64_set(st == PCRP_OK ? best : keep);
+    if (st != PCRP_OK) return st;
+    if (best_k) *best_k = best.k_fuse;
+    if (best_shape) *best_shape = best.shape;
+    if (best_ms) *best_ms = bt;
+    return PCRP_OK;
+}
+
 int pcrp_tune_save(const char* path)
 {
     if (!path) return PCRP_ERR_NULL;
     FILE* f = fopen(path, "w");
     if (!f) return PCRP_ERR_ARG;
     fprintf(f, "# pcrp tuning table: precision bucket(ceil log2 n) variant kind nt rpt\n");
+    const SplitTune sp = split_f64_get();
+    fprintf(f, "split %d %d %d\n", sp.lds_max, sp.k_fuse, sp.shape);
     for (int p = 0; p < 2; ++p)
         for (int b = 0; b < kBuckets; ++b) {
             const LdsVariantF64* v = g_tuned[p][b] >= 0 ? lds_f64_variant(g_tuned[p][b]) : nullptr;
@@ -311,6 +363,11 @@ int pcrp_tune_load(const char* path)
     int st = PCRP_OK;
     while (fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
+        int lm, kf, sh;
+        if (sscanf(line, "split %d %d %d", &lm, &kf, &sh) == 3) {
+            if (split_f64_set(SplitTune{lm, kf, sh}) != 0) st = PCRP_ERR_ARG;
+            continue;
+        }
         int p, b, id, kind, nt, rpt;
         if (sscanf(line, "%d %d %d %d %d %d", &p, &b, &id, &kind, &nt, &rpt) != 6) continue;
         const LdsVariantF64* v = (p == PCRP_PREC_F64) ? lds_f64_variant(id) : nullptr;

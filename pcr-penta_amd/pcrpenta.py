@@ -86,6 +86,9 @@ def _declare(lib):
     lib.pcrp_heston_workspace_size.restype = ctypes.c_size_t
     lib.pcrp_heston_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_size_t, _vp]
     lib.pcrp_heston_solve.restype = ctypes.c_int
+    lib.pcrp_split_autotune.argtypes = [ctypes.c_int] * 4 + [_vp] * 8 + \
+        [ctypes.c_size_t, ctypes.c_int, _vp, ip, ip, fp]
+    lib.pcrp_split_autotune.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -233,6 +236,26 @@ def autotune(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, reps: 
                            ctypes.byref(best_ms), times)
     _check(st, "pcrp_autotune")
     return best.value, best_ms.value, [times[i] for i in range(nv)]
+
+
+def split_autotune(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, reps: int = 2,
+                   stream=None, ws=None):
+    """Time every (steps per pass, tile shape) of the split path; keeps the fastest.
+    Returns (k_fuse, shape, ms)."""
+    import torch
+    _validate([a, b, c, d, e, y], n, m)
+    if ws is None:
+        ws, nb = _workspace(n, m, PREC_F64, y.device)
+    else:
+        nb = ws.numel()
+    work = torch.empty_like(y)
+    k, sh, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+    st = load().pcrp_split_autotune(PREC_F64, n, m, layout,
+                                    *[_ptr(t) for t in (a, b, c, d, e, y)], _ptr(work),
+                                    _ptr(ws), nb, reps, _stream(stream), ctypes.byref(k),
+                                    ctypes.byref(sh), ctypes.byref(ms))
+    _check(st, "pcrp_split_autotune")
+    return k.value, sh.value, ms.value
 
 
 def tune_save(path: str):

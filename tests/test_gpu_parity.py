@@ -387,3 +387,22 @@ def test_heston_config3_price_and_refine(gpu):
     print("heston price", price, "semi-analytic", ref, flush=True)
     assert abs(price - ref) / ref < 2e-3
     assert np.abs(U32 - U64).max() / np.abs(U64).max() <= 1e-10
+
+
+def test_split_autotune_keeps_results_exact(gpu, oracle_lib):
+    import torch
+    n, m = 5000, 4
+    _say("split-autotune", n, m)
+    saved = gpu.split_tune_get()
+    try:
+        dev = gpu.generate(0, 12, n, m, I)
+        ins = [_np(t).copy() for t in dev]
+        k, sh, ms = gpu.split_autotune(*dev, n, m, I, reps=1)
+        assert 1 <= k <= 6 and ms > 0
+        assert gpu.split_tune_get()[1:3] == (k, sh)
+        assert np.array_equal(_np(dev[5]), ins[5])
+        x = _run_f64(gpu, dev, n, m, I)
+        assert np.array_equal(x, oracle_lib.pcr_batch(*ins, n, m, I))
+    finally:
+        gpu.split_tune_set(*saved[:3])
+        torch.cuda.synchronize()
