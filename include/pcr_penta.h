@@ -168,7 +168,9 @@ int pcrp_init_t2(int n, int m, int layout,
  * row, the n <= 1024 threshold: main.cu:16-22, 46-49). Here each size bucket
  * ceil(log2 n) maps to one compiled LDS-kernel variant (kind, threads NT, rows
  * per thread RPT); the table starts at a built-in default and is set by
- * timing every variant on the device. */
+ * timing every variant on the device. kind: 0 shared pair equations, 1
+ * redundant (reference form), 2 in-place pair equations; +4 = persistent
+ * batch form that prefetches the next system. */
 int pcrp_lds_variant_count(int precision);
 int pcrp_lds_variant_info(int precision, int id, int *kind, int *nt, int *rpt,
                           int *max_n, int *lds_bytes);

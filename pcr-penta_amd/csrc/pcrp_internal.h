@@ -47,7 +47,8 @@ struct PassSys {
     int n, k, tiles, pad;
 };
 
-// A compiled LDS-kernel variant: KIND (0 shared pair equations, 1 redundant),
+// A compiled LDS-kernel variant: kind (0 shared pair equations, 1 redundant,
+// 2 in-place pair equations; +4 for the persistent prefetching batch form),
 // NT threads x RPT rows per thread, capacity max_n = NT*RPT rows.
 struct LdsVariantF64 {
     int kind, nt, rpt, max_n;

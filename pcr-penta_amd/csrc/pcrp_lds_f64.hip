@@ -41,6 +41,13 @@ struct LdsWords {
     static constexpr int value = KIND == KIND_SHARED ? 14 : 6;
 };
 
+// Field-wise row copy: keeps loops over row arrays from being turned into a
+// memcpy of a private array (which would live in scratch memory).
+__device__ __forceinline__ void set_row(Row<double>& d, const Row<double>& s)
+{
+    d.a = s.a; d.b = s.b; d.c = s.c; d.d = s.d; d.e = s.e; d.y = s.y;
+}
+
 // Register-resident step at stride delta = DK*NT: rows k-DK, k, k+DK of the
 // same thread are the neighbours. new[k] from E(k-DK), row k, E(k+DK).
 template <int NT, int RPT, int DK>
@@ -76,7 +83,7 @@ __device__ __forceinline__ void register_step(Row<double> (&r)[RPT], int t, int 
     }
 #pragma unroll
     for (int k = 0; k < RPT; ++k)
-        if (t + NT * k < n) r[k] = N[k];
+        if (t + NT * k < n) set_row(r[k], N[k]);
 }
 
 // One INPLACE step through LDS. Builds every own pair equation E(j) from the
@@ -148,16 +155,17 @@ __device__ __forceinline__ bool inplace_zero_pivot_step(Row<double> (&r)[RPT], i
 // Solve one system (or sub-system) held at ga..gy[base + i*rs], i < n, with
 // `steps` reduction steps; x goes to gx[xbase + i*xrs]. All threads of the
 // workgroup call this together (it contains barriers).
-template <int KIND, int NT, int RPT>
-__device__ __forceinline__ void lds_solve_one(
-    const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
-    const double* __restrict__ gd, const double* __restrict__ ge, const double* gy,
-    double* gx, long long base, long long rs, long long xbase, long long xrs, int n,
-    int steps, double (&sR)[6][NT * RPT],
-    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1])
+// This thread's rows i = t + NT*k of one system into registers.
+template <int NT, int RPT>
+__device__ __forceinline__ void lds_load_rows(const double* __restrict__ ga,
+                                              const double* __restrict__ gb,
+                                              const double* __restrict__ gc,
+                                              const double* __restrict__ gd,
+                                              const double* __restrict__ ge, const double* gy,
+                                              long long base, long long rs, int n,
+                                              Row<double> (&r)[RPT])
 {
     const int t = threadIdx.x;
-    Row<double> r[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; ++k) {
         const int i = t + NT * k;
@@ -169,6 +177,35 @@ __device__ __forceinline__ void lds_solve_one(
             r[k] = Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
         }
     }
+}
+
+template <int KIND, int NT, int RPT>
+__device__ __forceinline__ void lds_solve_regs(
+    Row<double> (&r)[RPT], double* gx, long long xbase, long long xrs, int n, int steps,
+    double (&sR)[6][NT * RPT],
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1]);
+
+template <int KIND, int NT, int RPT>
+__device__ __forceinline__ void lds_solve_one(
+    const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
+    const double* __restrict__ gd, const double* __restrict__ ge, const double* gy,
+    double* gx, long long base, long long rs, long long xbase, long long xrs, int n,
+    int steps, double (&sR)[6][NT * RPT],
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1])
+{
+    Row<double> r[RPT];
+    lds_load_rows<NT, RPT>(ga, gb, gc, gd, ge, gy, base, rs, n, r);
+    lds_solve_regs<KIND, NT, RPT>(r, gx, xbase, xrs, n, steps, sR, sE);
+}
+
+// The solve proper, from rows already in registers.
+template <int KIND, int NT, int RPT>
+__device__ __forceinline__ void lds_solve_regs(
+    Row<double> (&r)[RPT], double* gx, long long xbase, long long xrs, int n, int steps,
+    double (&sR)[6][NT * RPT],
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1])
+{
+    const int t = threadIdx.x;
 
     auto writeR = [&](int i, const Row<double>& q) {
         sR[0][i] = q.a; sR[1][i] = q.b; sR[2][i] = q.c;
@@ -264,7 +301,7 @@ __device__ __forceinline__ void lds_solve_one(
 #pragma unroll
     for (int dk = 1; dk < RPT; dk <<= 1) {
         if (s < steps) {
-            if (dk == 1) register_step<NT, RPT, (RPT > 1 ? 1 : 1)>(r, t, n);
+            if (dk == 1) register_step<NT, RPT, 1>(r, t, n);
             if constexpr (RPT > 2) if (dk == 2) register_step<NT, RPT, 2>(r, t, n);
             if constexpr (RPT > 4) if (dk == 4) register_step<NT, RPT, 4>(r, t, n);
             if constexpr (RPT > 8) if (dk == 8) register_step<NT, RPT, 8>(r, t, n);
@@ -282,13 +319,16 @@ __device__ __forceinline__ void lds_solve_one(
             if (i < n - delta) {
                 double cii, bii, yii;
                 if (delta >= NT) {
-                    // partner is row k + delta/NT of this thread
+                    // partner is row k + delta/NT of this thread (selects, no indexing)
                     const int kk = k + delta / NT;
-                    Row<double> P = r[0];
+                    cii = 0.0; bii = 0.0; yii = 0.0;
 #pragma unroll
-                    for (int q = 1; q < RPT; ++q)
-                        if (q == kk) P = r[q];
-                    cii = P.c; bii = P.b; yii = P.y;
+                    for (int q = 0; q < RPT; ++q) {
+                        const bool hit = q == kk;
+                        cii = hit ? r[q].c : cii;
+                        bii = hit ? r[q].b : bii;
+                        yii = hit ? r[q].y : yii;
+                    }
                 } else {
                     cii = sR[2][i + delta]; bii = sR[1][i + delta]; yii = sR[5][i + delta];
                 }
@@ -328,6 +368,43 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
         }
     }
     lds_solve_one<KIND, NT, RPT>(ga, gb, gc, gd, ge, gy, gx, base, rs, xbase, xrs, n, steps, sR, sE);
+}
+
+// Persistent batch form: a resident grid walks the systems, and while it
+// solves one system the rows of its next system are already being loaded
+// into registers, so the strided HBM loads overlap the LDS steps. Virtual
+// block ids b, b + G, b + 2G, ... keep b's XCD (G is a multiple of 8), and
+// xcd_remap gives each XCD runs of consecutive systems as in the batch form.
+template <int KIND, int NT, int RPT>
+__global__ __launch_bounds__(NT) void lds_f64_persist_kernel(
+    const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
+    const double* __restrict__ gd, const double* __restrict__ ge, double* gy,
+    int n, int m, long long rs, long long ss, int steps)
+{
+    constexpr int NMAX = NT * RPT;
+    __shared__ double sR[6][NMAX];
+    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
+    const int G = gridDim.x;
+    int vb = blockIdx.x;
+    if (vb >= m) return;
+    Row<double> cur[RPT];
+    int sys = xcd_remap(vb, m);
+    lds_load_rows<NT, RPT>(ga, gb, gc, gd, ge, gy, (long long)sys * ss, rs, n, cur);
+    for (;;) {
+        const int nvb = vb + G;
+        Row<double> nxt[RPT];
+        int nsys = 0;
+        if (nvb < m) {
+            nsys = xcd_remap(nvb, m);
+            lds_load_rows<NT, RPT>(ga, gb, gc, gd, ge, gy, (long long)nsys * ss, rs, n, nxt);
+        }
+        lds_solve_regs<KIND, NT, RPT>(cur, gy, (long long)sys * ss, rs, n, steps, sR, sE);
+        if (nvb >= m) break;
+        __syncthreads();   // LDS of this system fully consumed before the next one starts
+        for (int k = 0; k < RPT; ++k) set_row(cur[k], nxt[k]);
+        vb = nvb;
+        sys = nsys;
+    }
 }
 
 // Persistent work-queue form. Jobs (whole systems or residue sub-systems of
@@ -400,6 +477,32 @@ static int launch_queue(const double* const* sets, double* x, const LdsJob* jobs
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int KIND, int NT, int RPT>
+static int launch_persist(const SolveArgsF64& A, hipStream_t st)
+{
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void*>(&lds_f64_persist_kernel<KIND, NT, RPT>), NT, 0) !=
+                hipSuccess)
+            return -1;
+        grid = ((cus * (per > 0 ? per : 1)) / 8) * 8;
+        if (grid < 8) grid = 8;
+    }
+    // split views (SplitInfo) use the batch form; this form covers plain batches
+    if (A.split.S > 0) return launch<KIND, NT, RPT>(A, st);
+    const int g = A.m < grid ? A.m : grid;
+    hipLaunchKernelGGL((lds_f64_persist_kernel<KIND, NT, RPT>), dim3(g), dim3(NT), 0, st, A.a, A.b,
+                       A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.steps);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+#define PCRP_P(KIND, NT, RPT) \
+    {KIND + 4, NT, RPT, NT * RPT, &launch_persist<KIND, NT, RPT>, &launch_queue<KIND, NT, RPT>}
+
 #define PCRP_V(KIND, NT, RPT) \
     {KIND, NT, RPT, NT * RPT, &launch<KIND, NT, RPT>, &launch_queue<KIND, NT, RPT>}
 
@@ -422,6 +525,9 @@ static const LdsVariantF64 kVariants[] = {
     PCRP_V(KIND_INPLACE, 512, 1),   PCRP_V(KIND_INPLACE, 1024, 1), PCRP_V(KIND_INPLACE, 128, 2),
     PCRP_V(KIND_INPLACE, 256, 2),   PCRP_V(KIND_INPLACE, 512, 2),  PCRP_V(KIND_INPLACE, 64, 4),
     PCRP_V(KIND_INPLACE, 128, 4),   PCRP_V(KIND_INPLACE, 256, 4),  PCRP_V(KIND_INPLACE, 128, 8),
+    // persistent batch form with next-system prefetch (kind code + 4)
+    PCRP_P(KIND_REDUNDANT, 256, 4), PCRP_P(KIND_REDUNDANT, 1024, 1), PCRP_P(KIND_REDUNDANT, 256, 2),
+    PCRP_P(KIND_REDUNDANT, 256, 1), PCRP_P(KIND_REDUNDANT, 128, 4), PCRP_P(KIND_SHARED, 512, 1),
 };
 
 #undef PCRP_V
@@ -437,7 +543,7 @@ int lds_f64_max_n() { return 1024; }
 
 int lds_f64_lds_bytes(const LdsVariantF64& v)
 {
-    return 8 * v.max_n * (v.kind == KIND_SHARED ? LdsWords<KIND_SHARED>::value
+    return 8 * v.max_n * ((v.kind & 3) == KIND_SHARED ? LdsWords<KIND_SHARED>::value
                                                 : LdsWords<KIND_REDUNDANT>::value);   // INPLACE: 6 too
 }
 
