@@ -477,19 +477,35 @@ static int launch_queue(const double* const* sets, double* x, const LdsJob* jobs
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Workgroups of one kernel that fit on a gfx950 CU at once: 4 SIMDs with 512
+// VGPRs per lane each (allocated in granules of 8), at most 8 waves per SIMD,
+// 160 KB of LDS.
+static int resident_blocks_per_cu(int nt, int vgprs, int lds_bytes)
+{
+    const int waves = (nt + 63) / 64;
+    const int v = vgprs < 8 ? 8 : ((vgprs + 7) / 8) * 8;
+    int per_simd = 512 / v;
+    if (per_simd > 8) per_simd = 8;
+    int by_regs = (4 * per_simd) / waves;
+    int by_lds = lds_bytes > 0 ? (160 * 1024) / lds_bytes : 32;
+    int b = by_regs < by_lds ? by_regs : by_lds;
+    return b < 1 ? 1 : b;
+}
+
 template <int KIND, int NT, int RPT>
 static int launch_persist(const SolveArgsF64& A, hipStream_t st)
 {
     static int grid = 0;
     if (grid == 0) {
-        int dev = 0, cus = 0, per = 0;
+        int dev = 0, cus = 0;
+        hipFuncAttributes fa;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per, reinterpret_cast<const void*>(&lds_f64_persist_kernel<KIND, NT, RPT>), NT, 0) !=
-                hipSuccess)
+            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(
+                                          &lds_f64_persist_kernel<KIND, NT, RPT>)) != hipSuccess)
             return -1;
-        grid = ((cus * (per > 0 ? per : 1)) / 8) * 8;
+        grid = cus * resident_blocks_per_cu(NT, fa.numRegs, (int)fa.sharedSizeBytes);
+        grid = (grid / 8) * 8;
         if (grid < 8) grid = 8;
     }
     // split views (SplitInfo) use the batch form; this form covers plain batches
