@@ -33,20 +33,31 @@ namespace pcrp {
 
 // One tile of one HBM pass: rows l = tile*T - H + lr of current sub-system q
 // (valid when 0 <= l < L), k fused steps in LDS, interior rows stored.
+// Each thread owns RPT (row, sub-system) slots: slot tid + NTH*kk is row
+// lr0 + (TB/RPT)*kk of sub-system g (the same g for all its slots), so its
+// RPT rows are independent work for the same division latency.
 // in_at(l) / out_at(l) give the element index of row l of this sub-system in
 // the input arrays and of its destination in the output arrays.
-template <int TB, int G, typename InAt, typename OutAt>
+template <int TB, int G, int RPT, typename InAt, typename OutAt>
 __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* const (&out)[6],
-                                          bool qvalid, int l, int L, int k, int H, int T,
+                                          bool qvalid, int l0, int L, int k, int H, int T,
                                           InAt in_at, OutAt out_at, double (&sR)[6][TB * G])
 {
-    const int tid = threadIdx.x, lr = tid / G, g = tid - lr * G;
-    const bool valid = qvalid && l >= 0 && l < L;
-    Row<double> r{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
-    if (valid) {
-        const long long gi = in_at(l);
-        r.a = in[0][gi]; r.b = in[1][gi]; r.c = in[2][gi];
-        r.d = in[3][gi]; r.e = in[4][gi]; r.y = in[5][gi];
+    constexpr int NTH = TB * G / RPT;
+    constexpr int RSTEP = TB / RPT;
+    const int tid = threadIdx.x, lr0 = tid / G, g = tid - lr0 * G;
+    Row<double> r[RPT];
+    bool valid[RPT];
+#pragma unroll
+    for (int kk = 0; kk < RPT; ++kk) {
+        const int l = l0 + RSTEP * kk;
+        valid[kk] = qvalid && l >= 0 && l < L;
+        r[kk] = Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+        if (valid[kk]) {
+            const long long gi = in_at(l);
+            r[kk].a = in[0][gi]; r[kk].b = in[1][gi]; r[kk].c = in[2][gi];
+            r[kk].d = in[3][gi]; r[kk].e = in[4][gi]; r[kk].y = in[5][gi];
+        }
     }
     auto readR = [&](int i) {
         Row<double> w;
@@ -54,40 +65,55 @@ __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* 
         w.d = sR[3][i]; w.e = sR[4][i]; w.y = sR[5][i];
         return w;
     };
-    auto writeR = [&](const Row<double>& w) {
-        sR[0][tid] = w.a; sR[1][tid] = w.b; sR[2][tid] = w.c;
-        sR[3][tid] = w.d; sR[4][tid] = w.e; sR[5][tid] = w.y;
+    auto writeR = [&](int i, const Row<double>& w) {
+        sR[0][i] = w.a; sR[1][i] = w.b; sR[2][i] = w.c;
+        sR[3][i] = w.d; sR[4][i] = w.e; sR[5][i] = w.y;
     };
-    writeR(r);
+#pragma unroll
+    for (int kk = 0; kk < RPT; ++kk) writeR(tid + NTH * kk, r[kk]);
     __syncthreads();
     for (int s = 0; s < k; ++s) {
         const int delta = 1 << s;
-        if (valid) {
-            const bool hp1 = l - delta >= 0, hq1 = l + delta <= L - 1;
-            const bool hp2 = l - 2 * delta >= 0, hq2 = l + 2 * delta <= L - 1;
-            const bool hp3 = l - 3 * delta >= 0, hq3 = l + 3 * delta <= L - 1;
-            // neighbours outside the tile only feed rows that are not kept
-            auto nb = [&](bool exists, int off) {
-                const int lj = lr + off;
-                return (exists && lj >= 0 && lj < TB) ? lj * G + g : tid;
-            };
-            const Row<double> P1 = readR(nb(hp1, -delta)), P2 = readR(nb(hp2, -2 * delta));
-            const Row<double> Q1 = readR(nb(hq1, delta)), Q2 = readR(nb(hq2, 2 * delta));
-            const Pair<double> EL = make_pair(P2, P1, r, hp2, hp3, true, hq1);
-            const Pair<double> ER = make_pair(r, Q1, Q2, true, hp1, hq2, hq3);
-            r = reduce_row(r, EL, ER, hp1, hq1, hp2, hq2,
-                           [&](double& alL, double& arL, double& alR, double& arR) {
-                               alL = EL.al; arL = EL.ar; alR = ER.al; arR = ER.ar;
-                           });
+        Row<double> N[RPT];
+#pragma unroll
+        for (int kk = 0; kk < RPT; ++kk) {
+            const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk, self = tid + NTH * kk;
+            N[kk] = r[kk];
+            if (valid[kk]) {
+                const bool hp1 = l - delta >= 0, hq1 = l + delta <= L - 1;
+                const bool hp2 = l - 2 * delta >= 0, hq2 = l + 2 * delta <= L - 1;
+                const bool hp3 = l - 3 * delta >= 0, hq3 = l + 3 * delta <= L - 1;
+                // neighbours outside the tile only feed rows that are not kept
+                auto nb = [&](bool exists, int off) {
+                    const int lj = lr + off;
+                    return (exists && lj >= 0 && lj < TB) ? lj * G + g : self;
+                };
+                const Row<double> P1 = readR(nb(hp1, -delta)), P2 = readR(nb(hp2, -2 * delta));
+                const Row<double> Q1 = readR(nb(hq1, delta)), Q2 = readR(nb(hq2, 2 * delta));
+                const Pair<double> EL = make_pair(P2, P1, r[kk], hp2, hp3, true, hq1);
+                const Pair<double> ER = make_pair(r[kk], Q1, Q2, true, hp1, hq2, hq3);
+                N[kk] = reduce_row(r[kk], EL, ER, hp1, hq1, hp2, hq2,
+                                   [&](double& alL, double& arL, double& alR, double& arR) {
+                                       alL = EL.al; arL = EL.ar; alR = ER.al; arR = ER.ar;
+                                   });
+            }
         }
         __syncthreads();
-        writeR(r);
+#pragma unroll
+        for (int kk = 0; kk < RPT; ++kk) {
+            r[kk] = N[kk];
+            writeR(tid + NTH * kk, r[kk]);
+        }
         __syncthreads();
     }
-    if (valid && lr >= H && lr < H + T) {
-        const long long go = out_at(l);
-        out[0][go] = r.a; out[1][go] = r.b; out[2][go] = r.c;
-        out[3][go] = r.d; out[4][go] = r.e; out[5][go] = r.y;
+#pragma unroll
+    for (int kk = 0; kk < RPT; ++kk) {
+        const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
+        if (valid[kk] && lr >= H && lr < H + T) {
+            const long long go = out_at(l);
+            out[0][go] = r[kk].a; out[1][go] = r[kk].b; out[2][go] = r[kk].c;
+            out[3][go] = r[kk].d; out[4][go] = r[kk].e; out[5][go] = r[kk].y;
+        }
     }
 }
 
@@ -114,8 +140,8 @@ __host__ __forceinline__ long long blocks_for(int groups, int tiles)
 }
 
 // Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
-template <int TB, int G>
-__global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
+template <int TB, int G, int RPT>
+__global__ __launch_bounds__(TB* G / RPT) void hbm_pass_f64_kernel(
     const double* __restrict__ ia, const double* __restrict__ ib, const double* __restrict__ ic,
     const double* __restrict__ id, const double* __restrict__ ie, const double* __restrict__ iy,
     double* __restrict__ oa, double* __restrict__ ob, double* __restrict__ oc,
@@ -127,17 +153,17 @@ __global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
     const int T = TB - 2 * H;
     int grp, tile;
     if (!tile_of_block(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
-    const int lr = threadIdx.x / G, g = threadIdx.x - lr * G;
+    const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
     const int q = grp * G + g;
-    const int l = tile * T - H + lr;
+    const int l0 = tile * T - H + lr0;
     // current sub-system q = R*m0 + j holds rows R + 2^Sin * l of system j
     const int R = q / m0;
     const int L = (n0 - R + (1 << Sin) - 1) >> Sin;
     const long long Qout = (long long)Qin << k;
     const double* const in[6] = {ia, ib, ic, id, ie, iy};
     double* const out[6] = {oa, ob, oc, od, oe, oy};
-    pass_tile<TB, G>(
-        in, out, q < Qin, l, L, k, H, T,
+    pass_tile<TB, G, RPT>(
+        in, out, q < Qin, l0, L, k, H, T,
         [&](int ll) { return (long long)q * in_ss + (long long)ll * in_rs; },
         [&](int ll) {
             return (long long)(ll >> k) * Qout + (long long)(ll & ((1 << k) - 1)) * Qin + q;
@@ -150,8 +176,8 @@ __global__ __launch_bounds__(TB* G) void hbm_pass_f64_kernel(
 // is q + l*2^Sin, i.e. the row's own index, so each pass reads and writes
 // the same element positions (in the caller's arrays or a same-shaped
 // workspace). tab lists the systems that still need passes in this round.
-template <int TB, int G>
-__global__ __launch_bounds__(TB* G) void hbm_pass_uneven_f64_kernel(
+template <int TB, int G, int RPT>
+__global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     const double* const* __restrict__ sets, int src, int dst, const PassSys* __restrict__ tab,
     int ntab, int Sin)
 {
@@ -170,21 +196,21 @@ __global__ __launch_bounds__(TB* G) void hbm_pass_uneven_f64_kernel(
     const int k = P.k;
     const int H = 2 * ((1 << k) - 1);
     const int T = TB - 2 * H;
-    const int lr = threadIdx.x / G, g = threadIdx.x - lr * G;
+    const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
     const int Qin = 1 << Sin;
     const int q = grp * G + g;
     const int L = q < Qin ? (P.n - q + Qin - 1) >> Sin : 0;
-    const int l = tile * T - H + lr;
+    const int l0 = tile * T - H + lr0;
     const double* const in[6] = {sets[6 * src + 0], sets[6 * src + 1], sets[6 * src + 2],
                                  sets[6 * src + 3], sets[6 * src + 4], sets[6 * src + 5]};
     double* const out[6] = {const_cast<double*>(sets[6 * dst + 0]), const_cast<double*>(sets[6 * dst + 1]),
                             const_cast<double*>(sets[6 * dst + 2]), const_cast<double*>(sets[6 * dst + 3]),
                             const_cast<double*>(sets[6 * dst + 4]), const_cast<double*>(sets[6 * dst + 5])};
     auto at = [&](int ll) { return P.off + q + ((long long)ll << Sin); };
-    pass_tile<TB, G>(in, out, q < Qin, l, L, k, H, T, at, at, sR);
+    pass_tile<TB, G, RPT>(in, out, q < Qin, l0, L, k, H, T, at, at, sR);
 }
 
-template <int TB, int G>
+template <int TB, int G, int RPT>
 static int launch_pass(const double* const in[6], double* const out[6], int n0, int m0, int Sin,
                        int k, long long in_rs, long long in_ss, int Qin, hipStream_t st)
 {
@@ -196,19 +222,19 @@ static int launch_pass(const double* const in[6], double* const out[6], int n0, 
     const int groups = (Qin + G - 1) / G;
     const long long blocks = blocks_for(groups, tiles);
     if (blocks > 0x7fffffffLL) return -2;
-    hipLaunchKernelGGL((hbm_pass_f64_kernel<TB, G>), dim3((unsigned)blocks), dim3(TB * G), 0, st,
+    hipLaunchKernelGGL((hbm_pass_f64_kernel<TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT), 0, st,
                        in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2], out[3],
                        out[4], out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int TB, int G>
+template <int TB, int G, int RPT>
 static int launch_pass_uneven_t(const double* const* sets, int src, int dst, const PassSys* tab,
                                 int ntab, long long blocks, int Sin, hipStream_t st)
 {
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffffLL) return -2;
-    hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G>), dim3((unsigned)blocks), dim3(TB * G),
+    hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT),
                        0, st, sets, src, dst, tab, ntab, Sin);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -220,20 +246,37 @@ using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, int
                              hipStream_t);
 
 struct PassShape {
-    int tb, g;
+    int tb, g, rpt;
     PassFn fn;
     PassUnevenFn uneven;
 };
 
 static const PassShape kPassShapes[] = {
-    {256, 4, &launch_pass<256, 4>, &launch_pass_uneven_t<256, 4>},
-    {512, 2, &launch_pass<512, 2>, &launch_pass_uneven_t<512, 2>},
-    {128, 8, &launch_pass<128, 8>, &launch_pass_uneven_t<128, 8>},
-    {1024, 1, &launch_pass<1024, 1>, &launch_pass_uneven_t<1024, 1>},
+    // one (row, sub-system) slot per thread
+    {256, 4, 1, &launch_pass<256, 4, 1>, &launch_pass_uneven_t<256, 4, 1>},
+    {512, 2, 1, &launch_pass<512, 2, 1>, &launch_pass_uneven_t<512, 2, 1>},
+    {128, 8, 1, &launch_pass<128, 8, 1>, &launch_pass_uneven_t<128, 8, 1>},
+    {1024, 1, 1, &launch_pass<1024, 1, 1>, &launch_pass_uneven_t<1024, 1, 1>},
+    // four slots per thread (256 threads)
+    {256, 4, 4, &launch_pass<256, 4, 4>, &launch_pass_uneven_t<256, 4, 4>},
+    {512, 2, 4, &launch_pass<512, 2, 4>, &launch_pass_uneven_t<512, 2, 4>},
+    {128, 8, 4, &launch_pass<128, 8, 4>, &launch_pass_uneven_t<128, 8, 4>},
+    {1024, 1, 4, &launch_pass<1024, 1, 4>, &launch_pass_uneven_t<1024, 1, 4>},
 };
 
 int pass_shape_tb(int shape) { return kPassShapes[shape].tb; }
 int pass_shape_g(int shape) { return kPassShapes[shape].g; }
+int pass_shape_rpt(int shape) { return kPassShapes[shape].rpt; }
+
+// The shape of the same rows-per-thread with whole-row tiles (G = 1), used by
+// the first pass round of uneven batches (one sub-system per system there).
+int pass_shape_single(int shape)
+{
+    const int nshapes = (int)(sizeof(kPassShapes) / sizeof(kPassShapes[0]));
+    for (int i = 0; i < nshapes; ++i)
+        if (kPassShapes[i].g == 1 && kPassShapes[i].rpt == kPassShapes[shape].rpt) return i;
+    return shape;
+}
 
 int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
                        int ntab, long long blocks, int Sin, hipStream_t st)

@@ -95,6 +95,8 @@ int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, c
                        int ntab, long long blocks, int Sin, hipStream_t st);
 int pass_shape_tb(int shape);
 int pass_shape_g(int shape);
+int pass_shape_rpt(int shape);
+int pass_shape_single(int shape);
 
 int lds_f32r_max_n();
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);

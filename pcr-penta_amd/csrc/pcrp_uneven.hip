@@ -110,7 +110,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
         const int Sin = r * tune.k_fuse;
         // first round: one sub-system per system, whole-row tiles; later rounds:
         // several residue sub-systems side by side for coalesced row loads
-        const int shape = (r == 0) ? 3 : tune.shape;
+        const int shape = (r == 0) ? pass_shape_single(tune.shape) : tune.shape;
         const int TB = pass_shape_tb(shape), G = pass_shape_g(shape);
         std::vector<PassSys> tab;
         long long blocks = 0;

@@ -315,7 +315,10 @@ def test_uneven_deep_splits_bit_exact(gpu, oracle_lib):
     """Small LDS threshold: several pass rounds with different k per system."""
     saved = gpu.split_tune_get()
     try:
-        for lds_max, k, shape in ((16, 2, 0), (8, 3, 2), (64, 1, 1), (32, 4, 0)):
+        n_shapes = saved[3]
+        combos = [(16, 2, 0), (8, 3, 2), (64, 1, 1), (32, 4, 0)]
+        combos += [(16, 2 + (sh % 3), sh) for sh in range(n_shapes)]
+        for lds_max, k, shape in combos:
             gpu.split_tune_set(lds_max, k, shape)
             rng = np.random.default_rng(lds_max + k)
             sizes = list(rng.integers(1, 3000, 40)) + [4096, 777]
