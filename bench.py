@@ -33,6 +33,7 @@ WORKLOADS = {
     "lds_f32r_m16384_n2048": (2048, 16384, "f32r", 0),
     "lds_f64_m256_n512": (512, 256, "f64", 0),
     "hbm_f64_m32_n1048576": (1 << 20, 32, "f64", 1),
+    "hbm_f32r_m32_n1048576": (1 << 20, 32, "f32r", 1),
     # uneven: n = 0 marks the config-4 size draw (10000 systems, n log-uniform in [64, 131072])
     "uneven_f64_m10000": (0, 10000, "f64", 0),
     # Heston ADI (config 3): n = -1 marks the ADI driver; one step = one whole 200-step run
@@ -56,7 +57,22 @@ def parse():
     ap.add_argument("--tune-save", default=None, help="write the tuning table to this file")
     ap.add_argument("--tune-load", default=None,
                     help="load a tuning table (implies --no-tune)")
-    return ap.parse_args()
+    ap.add_argument("--split", default=None, metavar="K_FUSE,SHAPE",
+                    help="fix the split path's steps per pass and pass tile shape "
+                         "(implies --no-tune for the split path)")
+    args = ap.parse_args()
+    if args.split:
+        kf, sh = (int(v) for v in args.split.split(","))
+        args.split = (kf, sh)
+        args.no_tune = True
+    return args
+
+
+def apply_split(args):
+    if args.split:
+        import pcrpenta
+        lds_max = pcrpenta.split_tune_get()[0]
+        pcrpenta.split_tune_set(lds_max, *args.split)
 
 
 def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys, prec):
@@ -262,6 +278,7 @@ def main():
     import pcrpenta
 
     n, m, prec, kind = WORKLOADS[args.workload]
+    apply_split(args)
     if n == 0:
         return bench_uneven(args, m, prec, kind)
     if n == -1:

@@ -85,7 +85,15 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
  * the contraction is <= 0.1). Norms are max-norms. Inputs and output are FP64.
  * sweeps_out: optional DEVICE array of m ints receiving each system's number
  * of refinement sweeps (max_sweeps + 1 flags a system that did not stop).
- * Sizes n <= 2048 run in one LDS-resident kernel per system.
+ * Sizes n <= 2048 run in one LDS-resident kernel per system (the FP32
+ * factorisation is computed once and its multipliers kept in registers).
+ * Larger n need a workspace (pcrp_workspace_size with PCRP_PREC_F32_REFINE):
+ * every sweep re-runs the FP32 split solve (FP32 HBM passes + FP32 LDS
+ * stage, the same split scheme as pcrp_solve_f64) on the FP32 residual, with
+ * per-system stop flags so converged systems drop out of later sweeps.
+ * A zero or below-2^-100 FP32 pivot gives a zero multiplier (the refinement
+ * corrects the difference). A NaN correction stops the system with
+ * sweeps_out = max_sweeps + 1.
  */
 int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
                         const double *d, const double *e, double *y,

@@ -146,6 +146,7 @@ size_t pcrp_workspace_size(int n, int m, int precision)
 {
     if (check_shape(n, m) != PCRP_OK) return 0;
     if (precision == PCRP_PREC_F64) return split_f64_workspace(n, m);
+    if (precision == PCRP_PREC_F32_REFINE && n > lds_f32r_max_n()) return split_f32r_workspace(n, m);
     return 0;
 }
 
@@ -172,7 +173,6 @@ int pcrp_solve_f32r_f64(const double* a, const double* b, const double* c, const
                         const double* e, double* y, int n, int m, int layout, int max_sweeps,
                         double tol, int* sweeps_out, void* ws, size_t ws_bytes, void* stream)
 {
-    (void)ws; (void)ws_bytes;
     int st = check_shape(n, m);
     if (st != PCRP_OK) return st;
     if (!a || !b || !c || !d || !e || !y) return PCRP_ERR_NULL;
@@ -181,7 +181,9 @@ int pcrp_solve_f32r_f64(const double* a, const double* b, const double* c, const
     if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (n <= lds_f32r_max_n()) return lds_f32r_solve(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
-    return PCRP_ERR_UNSUPPORTED;
+    if (!ws || ws_bytes < split_f32r_workspace(n, m)) return PCRP_ERR_WORKSPACE;
+    const int rc = split_f32r_solve(A, layout, ws, ws_bytes, s);
+    return rc == 0 ? PCRP_OK : (rc == -3 ? PCRP_ERR_WORKSPACE : PCRP_ERR_HIP);
 }
 
 int pcrp_lds_variant_count(int precision)

@@ -90,6 +90,53 @@ __device__ __forceinline__ Row<double> reduce_row(const Row<double>& M, const Pa
     return N;
 }
 
+// ---- FP32 form (inner solver of the mixed-precision refinement) ----------
+// Same recurrences with approximate reciprocals (v_rcp_f32, 1 ulp): the FP64
+// refinement absorbs the difference. Off-diagonal couplings of a dominant
+// system shrink doubly exponentially with the level, so deep-level pivots
+// reach the FP32 denormal range, where v_rcp_f32 returns infinity; pivots
+// below 2^-100 are treated as zero (the coupling they would eliminate is
+// below FP32 resolution), and a zero pivot gives a zero multiplier.
+constexpr float kTinyF32 = 0x1p-100f;
+
+__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ Pair<float> make_pair(const Row<float>& L, const Row<float>& M,
+                                                 const Row<float>& R, bool hasL, bool aC,
+                                                 bool hasR, bool arC)
+{
+    Pair<float> E;
+    E.al = (aC && fabsf(L.b) >= kTinyF32) ? -M.a * rcp_f32(L.b) : 0.f;
+    E.ar = (arC && fabsf(R.d) >= kTinyF32) ? -M.e * rcp_f32(R.d) : 0.f;
+    E.A = E.al * L.a;
+    E.B = hasL ? fmaf(E.ar, R.a, fmaf(E.al, L.c, M.b)) : 0.f;
+    E.C = fmaf(E.ar, R.b, fmaf(E.al, L.d, M.c));
+    E.D = hasR ? fmaf(E.ar, R.c, fmaf(E.al, L.e, M.d)) : 0.f;
+    E.E = E.ar * R.e;
+    E.K = fmaf(E.ar, R.y, fmaf(E.al, L.y, M.y));
+    return E;
+}
+
+template <typename LoadAl>
+__device__ __forceinline__ Row<float> reduce_row(const Row<float>& M, const Pair<float>& EL,
+                                                 const Pair<float>& ER, bool hp1, bool hq1,
+                                                 bool, bool, LoadAl&&)
+{
+    const float g1 = (hp1 && fabsf(EL.C) >= kTinyF32) ? M.b * rcp_f32(EL.C) : 0.f;
+    const float g2 = (hq1 && fabsf(ER.C) >= kTinyF32) ? M.d * rcp_f32(ER.C) : 0.f;
+    Row<float> N;
+    N.a = g1 * EL.A;
+    N.b = fmaf(g2, ER.A, fmaf(g1, EL.B, -M.a));
+    N.c = fmaf(g2, ER.B, fmaf(g1, EL.D, -M.c));
+    N.d = fmaf(g2, ER.D, fmaf(g1, EL.E, -M.e));
+    N.e = g2 * ER.E;
+    N.y = fmaf(g2, ER.K, fmaf(g1, EL.K, -M.y));
+    return N;
+}
+
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
 // Remap so that each XCD works on runs of C consecutive systems at a time:
 // in the interleaved layout those share cache lines, so the line is fetched

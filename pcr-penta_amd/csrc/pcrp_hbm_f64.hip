@@ -38,21 +38,23 @@ namespace pcrp {
 // RPT rows are independent work for the same division latency.
 // in_at(l) / out_at(l) give the element index of row l of this sub-system in
 // the input arrays and of its destination in the output arrays.
-template <int TB, int G, int RPT, typename InAt, typename OutAt>
-__device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* const (&out)[6],
-                                          bool qvalid, int l0, int L, int k, int H, int T,
-                                          InAt in_at, OutAt out_at, double (&sR)[6][TB * G])
+// T is the element type: double for the FP64 solve, float for the inner solve
+// of the mixed-precision refinement (pcrp_device.h's FP32 step form).
+template <typename T, int TB, int G, int RPT, typename InAt, typename OutAt>
+__device__ __forceinline__ void pass_tile(const T* const (&in)[6], T* const (&out)[6],
+                                          bool qvalid, int l0, int L, int k, int H, int TI,
+                                          InAt in_at, OutAt out_at, T (&sR)[6][TB * G])
 {
     constexpr int NTH = TB * G / RPT;
     constexpr int RSTEP = TB / RPT;
     const int tid = threadIdx.x, lr0 = tid / G, g = tid - lr0 * G;
-    Row<double> r[RPT];
+    Row<T> r[RPT];
     bool valid[RPT];
 #pragma unroll
     for (int kk = 0; kk < RPT; ++kk) {
         const int l = l0 + RSTEP * kk;
         valid[kk] = qvalid && l >= 0 && l < L;
-        r[kk] = Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+        r[kk] = Row<T>{T(0), T(0), T(1), T(0), T(0), T(0)};
         if (valid[kk]) {
             const long long gi = in_at(l);
             r[kk].a = in[0][gi]; r[kk].b = in[1][gi]; r[kk].c = in[2][gi];
@@ -60,12 +62,12 @@ __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* 
         }
     }
     auto readR = [&](int i) {
-        Row<double> w;
+        Row<T> w;
         w.a = sR[0][i]; w.b = sR[1][i]; w.c = sR[2][i];
         w.d = sR[3][i]; w.e = sR[4][i]; w.y = sR[5][i];
         return w;
     };
-    auto writeR = [&](int i, const Row<double>& w) {
+    auto writeR = [&](int i, const Row<T>& w) {
         sR[0][i] = w.a; sR[1][i] = w.b; sR[2][i] = w.c;
         sR[3][i] = w.d; sR[4][i] = w.e; sR[5][i] = w.y;
     };
@@ -74,7 +76,7 @@ __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* 
     __syncthreads();
     for (int s = 0; s < k; ++s) {
         const int delta = 1 << s;
-        Row<double> N[RPT];
+        Row<T> N[RPT];
 #pragma unroll
         for (int kk = 0; kk < RPT; ++kk) {
             const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk, self = tid + NTH * kk;
@@ -88,12 +90,12 @@ __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* 
                     const int lj = lr + off;
                     return (exists && lj >= 0 && lj < TB) ? lj * G + g : self;
                 };
-                const Row<double> P1 = readR(nb(hp1, -delta)), P2 = readR(nb(hp2, -2 * delta));
-                const Row<double> Q1 = readR(nb(hq1, delta)), Q2 = readR(nb(hq2, 2 * delta));
-                const Pair<double> EL = make_pair(P2, P1, r[kk], hp2, hp3, true, hq1);
-                const Pair<double> ER = make_pair(r[kk], Q1, Q2, true, hp1, hq2, hq3);
+                const Row<T> P1 = readR(nb(hp1, -delta)), P2 = readR(nb(hp2, -2 * delta));
+                const Row<T> Q1 = readR(nb(hq1, delta)), Q2 = readR(nb(hq2, 2 * delta));
+                const Pair<T> EL = make_pair(P2, P1, r[kk], hp2, hp3, true, hq1);
+                const Pair<T> ER = make_pair(r[kk], Q1, Q2, true, hp1, hq2, hq3);
                 N[kk] = reduce_row(r[kk], EL, ER, hp1, hq1, hp2, hq2,
-                                   [&](double& alL, double& arL, double& alR, double& arR) {
+                                   [&](T& alL, T& arL, T& alR, T& arR) {
                                        alL = EL.al; arL = EL.ar; alR = ER.al; arR = ER.ar;
                                    });
             }
@@ -109,7 +111,7 @@ __device__ __forceinline__ void pass_tile(const double* const (&in)[6], double* 
 #pragma unroll
     for (int kk = 0; kk < RPT; ++kk) {
         const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
-        if (valid[kk] && lr >= H && lr < H + T) {
+        if (valid[kk] && lr >= H && lr < H + TI) {
             const long long go = out_at(l);
             out[0][go] = r[kk].a; out[1][go] = r[kk].b; out[2][go] = r[kk].c;
             out[3][go] = r[kk].d; out[4][go] = r[kk].e; out[5][go] = r[kk].y;
@@ -140,30 +142,34 @@ __host__ __forceinline__ long long blocks_for(int groups, int tiles)
 }
 
 // Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
-template <int TB, int G, int RPT>
-__global__ __launch_bounds__(TB* G / RPT) void hbm_pass_f64_kernel(
-    const double* __restrict__ ia, const double* __restrict__ ib, const double* __restrict__ ic,
-    const double* __restrict__ id, const double* __restrict__ ie, const double* __restrict__ iy,
-    double* __restrict__ oa, double* __restrict__ ob, double* __restrict__ oc,
-    double* __restrict__ od, double* __restrict__ oe, double* __restrict__ oy,
-    int n0, int m0, int Sin, int k, long long in_rs, long long in_ss, int Qin, int tiles)
+// stop (optional, m0 flags): systems whose flag is set are skipped (the
+// refinement's converged systems).
+template <typename T, int TB, int G, int RPT>
+__global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
+    const T* __restrict__ ia, const T* __restrict__ ib, const T* __restrict__ ic,
+    const T* __restrict__ id, const T* __restrict__ ie, const T* __restrict__ iy,
+    T* __restrict__ oa, T* __restrict__ ob, T* __restrict__ oc,
+    T* __restrict__ od, T* __restrict__ oe, T* __restrict__ oy,
+    int n0, int m0, int Sin, int k, long long in_rs, long long in_ss, int Qin, int tiles,
+    const int* __restrict__ stop)
 {
-    __shared__ double sR[6][TB * G];
+    __shared__ T sR[6][TB * G];
     const int H = 2 * ((1 << k) - 1);
-    const int T = TB - 2 * H;
+    const int TI = TB - 2 * H;   // interior rows kept per tile
     int grp, tile;
     if (!tile_of_block(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
     const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
     const int q = grp * G + g;
-    const int l0 = tile * T - H + lr0;
+    const int l0 = tile * TI - H + lr0;
     // current sub-system q = R*m0 + j holds rows R + 2^Sin * l of system j
     const int R = q / m0;
     const int L = (n0 - R + (1 << Sin) - 1) >> Sin;
     const long long Qout = (long long)Qin << k;
-    const double* const in[6] = {ia, ib, ic, id, ie, iy};
-    double* const out[6] = {oa, ob, oc, od, oe, oy};
-    pass_tile<TB, G, RPT>(
-        in, out, q < Qin, l0, L, k, H, T,
+    const T* const in[6] = {ia, ib, ic, id, ie, iy};
+    T* const out[6] = {oa, ob, oc, od, oe, oy};
+    const bool run = q < Qin && !(stop && stop[q - R * m0]);
+    pass_tile<T, TB, G, RPT>(
+        in, out, run, l0, L, k, H, TI,
         [&](int ll) { return (long long)q * in_ss + (long long)ll * in_rs; },
         [&](int ll) {
             return (long long)(ll >> k) * Qout + (long long)(ll & ((1 << k) - 1)) * Qin + q;
@@ -207,24 +213,25 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
                             const_cast<double*>(sets[6 * dst + 2]), const_cast<double*>(sets[6 * dst + 3]),
                             const_cast<double*>(sets[6 * dst + 4]), const_cast<double*>(sets[6 * dst + 5])};
     auto at = [&](int ll) { return P.off + q + ((long long)ll << Sin); };
-    pass_tile<TB, G, RPT>(in, out, q < Qin, l0, L, k, H, T, at, at, sR);
+    pass_tile<double, TB, G, RPT>(in, out, q < Qin, l0, L, k, H, T, at, at, sR);
 }
 
-template <int TB, int G, int RPT>
-static int launch_pass(const double* const in[6], double* const out[6], int n0, int m0, int Sin,
-                       int k, long long in_rs, long long in_ss, int Qin, hipStream_t st)
+template <typename T, int TB, int G, int RPT>
+static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, int Sin,
+                       int k, long long in_rs, long long in_ss, int Qin, const int* stop,
+                       hipStream_t st)
 {
     const int H = 2 * ((1 << k) - 1);
-    const int T = TB - 2 * H;
-    if (T < 1) return -2;
+    const int TI = TB - 2 * H;
+    if (TI < 1) return -2;
     const int L = (n0 + (1 << Sin) - 1) >> Sin;   // longest current sub-system
-    const int tiles = (L + T - 1) / T;
+    const int tiles = (L + TI - 1) / TI;
     const int groups = (Qin + G - 1) / G;
     const long long blocks = blocks_for(groups, tiles);
     if (blocks > 0x7fffffffLL) return -2;
-    hipLaunchKernelGGL((hbm_pass_f64_kernel<TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT), 0, st,
-                       in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2], out[3],
-                       out[4], out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles);
+    hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT),
+                       0, st, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2],
+                       out[3], out[4], out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles, stop);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -240,7 +247,9 @@ static int launch_pass_uneven_t(const double* const* sets, int src, int dst, con
 }
 
 using PassFn = int (*)(const double* const*, double* const*, int, int, int, int, long long,
-                       long long, int, hipStream_t);
+                       long long, int, const int*, hipStream_t);
+using PassF32Fn = int (*)(const float* const*, float* const*, int, int, int, int, long long,
+                          long long, int, const int*, hipStream_t);
 
 using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, int, long long, int,
                              hipStream_t);
@@ -249,20 +258,26 @@ struct PassShape {
     int tb, g, rpt;
     PassFn fn;
     PassUnevenFn uneven;
+    PassF32Fn f32;
 };
+
+#define PCRP_PASS_SHAPE(TB, G, RPT)                                                     \
+    {TB, G, RPT, &launch_pass<double, TB, G, RPT>, &launch_pass_uneven_t<TB, G, RPT>,  \
+     &launch_pass<float, TB, G, RPT>}
 
 static const PassShape kPassShapes[] = {
     // one (row, sub-system) slot per thread
-    {256, 4, 1, &launch_pass<256, 4, 1>, &launch_pass_uneven_t<256, 4, 1>},
-    {512, 2, 1, &launch_pass<512, 2, 1>, &launch_pass_uneven_t<512, 2, 1>},
-    {128, 8, 1, &launch_pass<128, 8, 1>, &launch_pass_uneven_t<128, 8, 1>},
-    {1024, 1, 1, &launch_pass<1024, 1, 1>, &launch_pass_uneven_t<1024, 1, 1>},
+    PCRP_PASS_SHAPE(256, 4, 1),
+    PCRP_PASS_SHAPE(512, 2, 1),
+    PCRP_PASS_SHAPE(128, 8, 1),
+    PCRP_PASS_SHAPE(1024, 1, 1),
     // four slots per thread (256 threads)
-    {256, 4, 4, &launch_pass<256, 4, 4>, &launch_pass_uneven_t<256, 4, 4>},
-    {512, 2, 4, &launch_pass<512, 2, 4>, &launch_pass_uneven_t<512, 2, 4>},
-    {128, 8, 4, &launch_pass<128, 8, 4>, &launch_pass_uneven_t<128, 8, 4>},
-    {1024, 1, 4, &launch_pass<1024, 1, 4>, &launch_pass_uneven_t<1024, 1, 4>},
+    PCRP_PASS_SHAPE(256, 4, 4),
+    PCRP_PASS_SHAPE(512, 2, 4),
+    PCRP_PASS_SHAPE(128, 8, 4),
+    PCRP_PASS_SHAPE(1024, 1, 4),
 };
+#undef PCRP_PASS_SHAPE
 
 int pass_shape_tb(int shape) { return kPassShapes[shape].tb; }
 int pass_shape_g(int shape) { return kPassShapes[shape].g; }
@@ -276,6 +291,13 @@ int pass_shape_single(int shape)
     for (int i = 0; i < nshapes; ++i)
         if (kPassShapes[i].g == 1 && kPassShapes[i].rpt == kPassShapes[shape].rpt) return i;
     return shape;
+}
+
+int launch_pass_f32(int shape, const float* const* in, float* const* out, int n0, int m0, int Sin,
+                    int k, long long in_rs, long long in_ss, int Qin, const int* stop,
+                    hipStream_t st)
+{
+    return kPassShapes[shape].f32(in, out, n0, m0, Sin, k, in_rs, in_ss, Qin, stop, st);
 }
 
 int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
@@ -342,7 +364,7 @@ int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes
     double* const* dst = set0;
     while (Sin < S) {
         const int k = (S - Sin < tune.k_fuse) ? S - Sin : tune.k_fuse;
-        const int rc = shape.fn(in, dst, n0, m0, Sin, k, in_rs, in_ss, Qin, st);
+        const int rc = shape.fn(in, dst, n0, m0, Sin, k, in_rs, in_ss, Qin, nullptr, st);
         if (rc != 0) return rc;
         Sin += k;
         Qin <<= k;

@@ -89,6 +89,17 @@ def test_workspace_too_small_is_rejected(lib):
     assert lib.pcrp_solve_f64(*_fake(6), n, m, 1, _fake(1)[0], need - 1, None) == 4
 
 
+def test_refine_split_workspace_is_required(lib):
+    """n above the one-workgroup FP32 limit: the refinement needs a workspace."""
+    n, m = 4096, 4
+    need = lib.pcrp_workspace_size(n, m, 1)
+    assert need > 0
+    assert lib.pcrp_workspace_size(2048, m, 1) == 0
+    assert lib.pcrp_solve_f32r_f64(*_fake(6), n, m, 1, 4, 1e-12, None, None, 0, None) == 4
+    assert lib.pcrp_solve_f32r_f64(*_fake(6), n, m, 1, 4, 1e-12, None, _fake(1)[0], need - 1,
+                                   None) == 4
+
+
 def test_uneven_argument_checks(lib):
     import numpy as np
     off = np.array([0, 10], dtype=np.int64)

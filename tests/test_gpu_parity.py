@@ -173,7 +173,8 @@ def _refined(gpu, dev, n, m, layout, max_sweeps=8, tol=1e-12):
     return _np(y), _np(sweeps)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 33, 100, 255, 256, 512, 700, 1000, 1024, 1500, 2048])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 33, 100, 255, 256, 512, 700, 1000, 1024, 1500, 2048,
+                               2049, 3000, 4096, 5000, 8191, 20000])
 def test_refined_meets_1e12_vs_dgbsv(gpu, oracle_lib, n):
     """FP32 PCR + FP64 refinement: normwise relative error <= 1e-12 against the
     FP64 banded-LU (dgbsv) oracle, both generators, both layouts."""
@@ -214,6 +215,63 @@ def test_refined_config1_shape(gpu, oracle_lib):
     ref = oracle_lib.gbsv_batch(*ins, n, m, I)
     assert oracle_lib.normwise_relerr(x, ref, n, m, I) <= 1e-12
     print("sweeps histogram", np.bincount(sw), flush=True)
+
+
+def test_refined_split_reports_nonconvergence(gpu, oracle_lib):
+    """Split refinement path: max_sweeps = 0 gives the bare FP32 split solve,
+    flagged with sweeps = 1, both layouts."""
+    n, m = 4096, 5
+    for layout in (I, C):
+        _say("refined-split-flag", n, m, layout)
+        dev = gpu.generate(0, 9, n, m, layout)
+        ins = [_np(t).copy() for t in dev]
+        x, sw = _refined(gpu, dev, n, m, layout, max_sweeps=0, tol=1e-15)
+        assert (sw == 1).all(), sw
+        ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+        err = oracle_lib.normwise_relerr(x, ref, n, m, layout)
+        assert 1e-9 < err < 1e-2, err
+
+
+def test_refined_split_tunables(gpu, oracle_lib):
+    """The FP32 split solve under every LDS threshold / steps-per-pass / tile
+    shape still refines to 1e-12 (its passes are the FP64 passes' float form)."""
+    saved = gpu.split_tune_get()
+    n_shapes = saved[3]
+    n, m = 3000, 3
+    dev0 = gpu.generate(1, 77, n, m, I)
+    ins = [_np(t).copy() for t in dev0]
+    ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+    import torch
+    try:
+        for lds_max in (64, 256, 1024):
+            for k in (1, 3, 5):
+                for shape in range(n_shapes):
+                    try:
+                        gpu.split_tune_set(lds_max, k, shape)
+                    except gpu.PCRPError:
+                        continue
+                    _say("refined-split-tune", n, m, lds_max, k, shape)
+                    dev = list(dev0[:5]) + [torch.from_numpy(ins[5]).cuda()]
+                    x, sw = _refined(gpu, dev, n, m, I)
+                    err = oracle_lib.normwise_relerr(x, ref, n, m, I)
+                    assert err <= 1e-12, (lds_max, k, shape, err, sw)
+                    assert sw.max() <= 8, sw
+    finally:
+        gpu.split_tune_set(*saved[:3])
+
+
+def test_refined_split_config2_sample(gpu, oracle_lib):
+    """Config-2 size (n = 2^20, weakly dominant), 2 systems, FP32 + refinement:
+    within 1e-12 of dgbsv."""
+    n, m = 1 << 20, 2
+    _say("refined-split-c2", n, m)
+    dev = gpu.generate(1, 0, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    x, sw = _refined(gpu, dev, n, m, I)
+    xg = oracle_lib.gbsv_batch(*ins, n, m, I)
+    err = oracle_lib.normwise_relerr(x, xg, n, m, I)
+    print("c2 refined vs dgbsv", err, "sweeps", sw, flush=True)
+    assert err <= 1e-12 and sw.max() <= 8
 
 
 # ---------------- HBM split path (n above the LDS threshold) ----------------
