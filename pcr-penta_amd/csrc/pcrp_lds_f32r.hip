@@ -1,19 +1,33 @@
-// LDS-resident FP32 PCR-Penta with FP64 iterative refinement, one workgroup
-// per system (n <= 2048). No counterpart in the reference, which is FP64-only
-// (kernel.cu:32-154); the reduction steps follow the same recurrences.
+// LDS-resident FP32 inner solve with FP64 iterative refinement, one
+// workgroup per system (n <= 2048). No counterpart in the reference, which is
+// FP64-only (kernel.cu:32-154).
 //
-//   1. factor: PCR-Penta on the FP32-rounded matrix. Every multiplier of
-//      every level (alfa, alfa' of the pair equation centred on the row,
-//      g1, g2 of the row) stays in VGPRs, so later solves touch only the
+// Inner solver: parallel cyclic reduction on the 2x2-block tridiagonal form
+// of the pentadiagonal matrix. Rows (2k, 2k+1) form block row k:
+//     L_k X_{k-1} + D_k X_k + U_k X_{k+1} = Y_k,   X_k = (x_2k, x_2k+1),
+//     L_k = [[a_2k, b_2k], [0, a_2k+1]],  D_k = [[c_2k, d_2k], [b_2k+1, c_2k+1]],
+//     U_k = [[e_2k, 0], [d_2k+1, e_2k+1]].
+// One level at block stride s: alpha = -L_k D_{k-s}^-1, beta = -U_k D_{k+s}^-1,
+//     D_k += alpha U_{k-s} + beta L_{k+s},  L_k = alpha L_{k-s},
+//     U_k = beta U_{k+s},                   Y_k += alpha Y_{k-s} + beta Y_{k+s},
+// for s = 1, 2, 4, ... until every coupling has left the system, then
+// X_k = D_k^-1 Y_k. The pivots are the diagonal blocks, which stay dominant
+// on a dominant matrix. PCR-Penta's own recurrences divide by off-diagonals
+// (-a/b, -e/d, kernel.cu:73,75,99,101); on random dominant matrices some of
+// those are 1e-6..1e-8, and in FP32 that leaves 1e-5..1e-2 relative error
+// (measured on the oracle's generators up to n = 2^20), so refinement needs
+// many sweeps or diverges. The block form stays at ~3e-7 at every size, so
+// the refinement converges in two corrections.
+//
+//   1. factor: every level's alpha, beta (8 floats per block row and level)
+//      and the final D^-1 stay in VGPRs, so the sweeps touch only the
 //      right-hand side;
-//   2. x = 0, r = y; repeat: dx = PCR32(r) with the cached multipliers,
-//      x += dx (FP64), r = y - A x (FP64, A and y exact), until the
-//      relative error left is estimated below tol (see the stopping rule
-//      below) or max_sweeps refinement sweeps have run.
-// A zero (or below 2^-100) FP32 pivot gives a zero multiplier (the FP64 path
-// keeps the reference's rule, kernel.cu:122,129): the refinement corrects it.
-//
+//   2. x = 0, r = y; repeat: dx = solve32(r), x += dx (FP64),
+//      r = y - A x (FP64, A and y exact), until the error left is estimated
+//      below tol (stopping rule below) or max_sweeps corrections have run.
 // Reciprocals are approximate (v_rcp_f32): the refinement absorbs the error.
+// A block determinant below 2^-100 gives a zero inverse (the refinement then
+// flags the system if it cannot converge).
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
@@ -21,17 +35,42 @@ namespace pcrp {
 
 namespace {
 
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// 2x2 matrix [[p, q], [r, s]]
+struct M2 {
+    float p, q, r, s;
+};
 
-// Off-diagonal couplings of a dominant system shrink doubly exponentially
-// with the level (squaring per step), so deep-level FP32 pivots reach the
-// denormal range, where v_rcp_f32 returns infinity. Pivots below 2^-100 are
-// treated as zero: the coupling they would eliminate is below FP32
-// resolution, and the FP64 refinement corrects what is left.
-constexpr float kTiny = 0x1p-100f;
+__device__ __forceinline__ M2 mmul(const M2& x, const M2& y)
+{
+    return M2{fmaf(x.p, y.p, x.q * y.r), fmaf(x.p, y.q, x.q * y.s),
+              fmaf(x.r, y.p, x.s * y.r), fmaf(x.r, y.q, x.s * y.s)};
+}
 
-struct RowF { float a, b, c, d, e; };
-struct PairF { float A, B, C, D, E, rC; };   // rC = 1/C (approximate)
+// -(x y)
+__device__ __forceinline__ M2 mmul_neg(const M2& x, const M2& y)
+{
+    return M2{-fmaf(x.p, y.p, x.q * y.r), -fmaf(x.p, y.q, x.q * y.s),
+              -fmaf(x.r, y.p, x.s * y.r), -fmaf(x.r, y.q, x.s * y.s)};
+}
+
+// d + x y + z w
+__device__ __forceinline__ M2 madd2(const M2& d, const M2& x, const M2& y, const M2& z,
+                                    const M2& w)
+{
+    return M2{fmaf(z.p, w.p, fmaf(z.q, w.r, fmaf(x.p, y.p, fmaf(x.q, y.r, d.p)))),
+              fmaf(z.p, w.q, fmaf(z.q, w.s, fmaf(x.p, y.q, fmaf(x.q, y.s, d.q)))),
+              fmaf(z.r, w.p, fmaf(z.s, w.r, fmaf(x.r, y.p, fmaf(x.s, y.r, d.r)))),
+              fmaf(z.r, w.q, fmaf(z.s, w.s, fmaf(x.r, y.q, fmaf(x.s, y.s, d.s))))};
+}
+
+__device__ __forceinline__ M2 minv(const M2& m)
+{
+    const float det = fmaf(m.p, m.s, -(m.q * m.r));
+    const float rd = fabsf(det) >= kTinyF32 ? rcp_f32(det) : 0.f;
+    return M2{m.s * rd, -m.q * rd, -m.r * rd, m.p * rd};
+}
+
+constexpr M2 kZero2 = {0.f, 0.f, 0.f, 0.f};
 
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* red)
@@ -51,213 +90,150 @@ __device__ __forceinline__ double block_max(double v, double* red)
 
 }  // namespace
 
-template <int NT, int RPT, int LMAX>
+// NT threads, BR block rows per thread (block row k = t + NT*q), LMAX levels
+// (>= ceil(log2(NT*BR))).
+template <int NT, int BR, int LMAX>
 __global__ __launch_bounds__(NT) void lds_f32r_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
-    int n, int m, long long rs, long long ss, int steps, int max_sweeps, double tol,
+    int n, int m, long long rs, long long ss, int max_sweeps, double tol,
     int* __restrict__ sweeps_out)
 {
-    constexpr int NMAX = NT * RPT;
+    constexpr int NB = NT * BR;        // block rows
+    constexpr int NMAX = 2 * NB;       // rows
     __shared__ double sA[5][NMAX];     // exact FP64 matrix (residual)
-    __shared__ double sX[NMAX];        // FP64 iterate x (residual neighbours)
-    __shared__ float sF[6][NMAX];      // factor: rows, then pair equations; sweeps: y, K, dx
+    // factor: D^-1, L, U of every block row; sweeps: FP64 x (aliases it)
+    __shared__ __align__(16) float sF[12][NB];
+    __shared__ float sY[2][2][NB];     // right-hand side exchange, ping-pong
     __shared__ double sRed[2][NT / 64];
+    double* const sX = reinterpret_cast<double*>(&sF[0][0]);   // NMAX doubles <= 12*NB floats
+    static_assert(2 * NMAX <= 12 * NB, "x fits in the factor buffer");
 
     const int t = threadIdx.x;
     const int sys = xcd_remap(blockIdx.x, m);
     const long long base = (long long)sys * ss;
+    const int nb = (n + 1) >> 1;
 
-    double Y[RPT], X[RPT];
-    RowF r[RPT];
+    double X[BR][2];
+    float v[BR][2];   // FP32 right-hand side of the next inner solve (first: y)
+    M2 L[BR], D[BR], U[BR];
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int i = t + NT * k;
-        double a = 0.0, b = 0.0, c = 1.0, d = 0.0, e = 0.0, y = 0.0;
-        if (i < n) {
-            const long long g = base + (long long)i * rs;
-            a = ga[g]; b = gb[g]; c = gc[g]; d = gd[g]; e = ge[g]; y = gy[g];
-            // out-of-range couplings are not part of the matrix
-            if (i < 2) a = 0.0;
-            if (i < 1) b = 0.0;
-            if (i > n - 2) d = 0.0;
-            if (i > n - 3) e = 0.0;
-            sA[0][i] = a; sA[1][i] = b; sA[2][i] = c; sA[3][i] = d; sA[4][i] = e;
-            sF[0][i] = (float)a; sF[1][i] = (float)b; sF[2][i] = (float)c;
-            sF[3][i] = (float)d; sF[4][i] = (float)e;
+    for (int q = 0; q < BR; ++q) {
+        const int k = t + NT * q;
+        double rw[2][5];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = 2 * k + h;
+            double a = 0.0, b = 0.0, c = 1.0, d = 0.0, e = 0.0, y = 0.0;
+            if (i < n) {
+                const long long g = base + (long long)i * rs;
+                a = ga[g]; b = gb[g]; c = gc[g]; d = gd[g]; e = ge[g]; y = gy[g];
+                // out-of-range couplings are not part of the matrix
+                if (i < 2) a = 0.0;
+                if (i < 1) b = 0.0;
+                if (i > n - 2) d = 0.0;
+                if (i > n - 3) e = 0.0;
+                sA[0][i] = a; sA[1][i] = b; sA[2][i] = c; sA[3][i] = d; sA[4][i] = e;
+            }
+            rw[h][0] = a; rw[h][1] = b; rw[h][2] = c; rw[h][3] = d; rw[h][4] = e;
+            X[q][h] = 0.0;
+            v[q][h] = (float)y;
         }
-        Y[k] = y;
-        X[k] = 0.0;
-        r[k] = RowF{(float)a, (float)b, (float)c, (float)d, (float)e};
+        L[q] = M2{(float)rw[0][0], (float)rw[0][1], 0.f, (float)rw[1][0]};
+        D[q] = M2{(float)rw[0][2], (float)rw[0][3], (float)rw[1][1], (float)rw[1][2]};
+        U[q] = M2{(float)rw[0][4], 0.f, (float)rw[1][3], (float)rw[1][4]};
     }
 
-    // cached multipliers per level: al, ar (pair equation on this row), g1, g2
-    float Mal[LMAX][RPT], Mar[LMAX][RPT], Mg1[LMAX][RPT], Mg2[LMAX][RPT];
-    // final solve: x_i = F1*y_i + F2*y_{i+D}; partner x_{i+D} = F3*y_{i+D} + F4*y_i
-    float F1[RPT], F2[RPT], F3[RPT], F4[RPT];
-    __syncthreads();
-
-    // ---------------- factor (one LDS buffer, four barriers per level) ----------------
+    // ---------------- factor: two barriers per level ----------------
+    M2 cA[LMAX][BR], cB[LMAX][BR];     // alpha, beta per level
+    int levels = 0;
+    while ((1 << levels) < nb) ++levels;
 #pragma unroll
     for (int s = 0; s < LMAX; ++s) {
-        if (s < steps) {
+        if (s < levels) {
             const int delta = 1 << s;
-            PairF E[RPT];
 #pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int j = t + NT * k;
-                float al = 0.f, ar = 0.f;
-                E[k] = PairF{0.f, 0.f, 1.f, 0.f, 0.f, 1.f};
-                if (j < n) {
-                    const bool hasL = j - delta >= 0, hasR = j + delta <= n - 1;
-                    const bool aC = j - 2 * delta >= 0, arC = j + 2 * delta <= n - 1;
-                    const int jl = hasL ? j - delta : j, jr = hasR ? j + delta : j;
-                    const float Lb = sF[1][jl], Rd = sF[3][jr];
-                    al = (aC && fabsf(Lb) >= kTiny) ? -r[k].a * frcp(Lb) : 0.f;
-                    ar = (arC && fabsf(Rd) >= kTiny) ? -r[k].e * frcp(Rd) : 0.f;
-                    E[k].A = al * sF[0][jl];
-                    E[k].B = hasL ? fmaf(ar, sF[0][jr], fmaf(al, sF[2][jl], r[k].b)) : 0.f;
-                    E[k].C = fmaf(ar, sF[1][jr], fmaf(al, sF[3][jl], r[k].c));
-                    E[k].D = hasR ? fmaf(ar, sF[2][jr], fmaf(al, sF[4][jl], r[k].d)) : 0.f;
-                    E[k].E = ar * sF[4][jr];
-                    E[k].rC = fabsf(E[k].C) >= kTiny ? frcp(E[k].C) : 0.f;
-                }
-                Mal[s][k] = al;
-                Mar[s][k] = ar;
+            for (int q = 0; q < BR; ++q) {
+                const int k = t + NT * q;
+                const M2 Di = minv(D[q]);
+                sF[0][k] = Di.p; sF[1][k] = Di.q; sF[2][k] = Di.r; sF[3][k] = Di.s;
+                sF[4][k] = L[q].p; sF[5][k] = L[q].q; sF[6][k] = L[q].r; sF[7][k] = L[q].s;
+                sF[8][k] = U[q].p; sF[9][k] = U[q].q; sF[10][k] = U[q].r; sF[11][k] = U[q].s;
             }
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int j = t + NT * k;
-                if (j < n) {
-                    sF[0][j] = E[k].A; sF[1][j] = E[k].B; sF[2][j] = E[k].C;
-                    sF[3][j] = E[k].D; sF[4][j] = E[k].E; sF[5][j] = E[k].rC;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int i = t + NT * k;
-                float g1 = 0.f, g2 = 0.f;
-                if (i < n) {
-                    const bool hp1 = i - delta >= 0, hq1 = i + delta <= n - 1;
-                    const int il = hp1 ? i - delta : i, ir = hq1 ? i + delta : i;
-                    g1 = hp1 ? r[k].b * sF[5][il] : 0.f;
-                    g2 = hq1 ? r[k].d * sF[5][ir] : 0.f;
-                    RowF N;
-                    N.a = g1 * sF[0][il];
-                    N.b = fmaf(g2, sF[0][ir], fmaf(g1, sF[1][il], -r[k].a));
-                    N.c = fmaf(g2, sF[1][ir], fmaf(g1, sF[3][il], -r[k].c));
-                    N.d = fmaf(g2, sF[3][ir], fmaf(g1, sF[4][il], -r[k].e));
-                    N.e = g2 * sF[4][ir];
-                    r[k] = N;
-                }
-                Mg1[s][k] = g1;
-                Mg2[s][k] = g2;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int i = t + NT * k;
-                if (i < n) {
-                    sF[0][i] = r[k].a; sF[1][i] = r[k].b; sF[2][i] = r[k].c;
-                    sF[3][i] = r[k].d; sF[4][i] = r[k].e;
-                }
+            for (int q = 0; q < BR; ++q) {
+                const int k = t + NT * q;
+                const bool hl = k - delta >= 0, hr = k + delta < nb;
+                const int kl = hl ? k - delta : k, kr = hr ? k + delta : k;
+                const M2 DiL{sF[0][kl], sF[1][kl], sF[2][kl], sF[3][kl]};
+                const M2 LL{sF[4][kl], sF[5][kl], sF[6][kl], sF[7][kl]};
+                const M2 UL{sF[8][kl], sF[9][kl], sF[10][kl], sF[11][kl]};
+                const M2 DiR{sF[0][kr], sF[1][kr], sF[2][kr], sF[3][kr]};
+                const M2 LR{sF[4][kr], sF[5][kr], sF[6][kr], sF[7][kr]};
+                const M2 UR{sF[8][kr], sF[9][kr], sF[10][kr], sF[11][kr]};
+                const M2 al = hl ? mmul_neg(L[q], DiL) : kZero2;
+                const M2 be = hr ? mmul_neg(U[q], DiR) : kZero2;
+                D[q] = madd2(D[q], al, UL, be, LR);
+                L[q] = mmul(al, LL);
+                U[q] = mmul(be, UR);
+                cA[s][q] = al;
+                cB[s][q] = be;
             }
             __syncthreads();
         }
     }
-    const int D = 1 << steps;   // final delta
+    M2 Fin[BR];
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int i = t + NT * k;
-        F1[k] = F2[k] = F3[k] = F4[k] = 0.f;
-        if (i < n) {
-            if (i < n - D) {
-                const float cii = sF[2][i + D], bii = sF[1][i + D];
-                const float det = fmaf(r[k].c, cii, -(bii * r[k].d));
-                const float rd = fabsf(det) >= kTiny ? frcp(det) : 0.f;
-                F1[k] = cii * rd; F2[k] = -r[k].d * rd; F3[k] = r[k].c * rd; F4[k] = -bii * rd;
-            } else if (i < D) {
-                F1[k] = fabsf(r[k].c) >= kTiny ? frcp(r[k].c) : 0.f;
-            }
-        }
-    }
-    __syncthreads();
+    for (int q = 0; q < BR; ++q) Fin[q] = minv(D[q]);
 
     // ---------------- solve + refinement sweeps ----------------
-    float* sy = sF[0];
-    float* sk = sF[1];
-    float* sdx = sF[2];
-    double res[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) res[k] = Y[k];
     int sweep = 0;
     double dprev = 0.0;
     for (;;) {
-        float v[RPT];
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            v[k] = (float)res[k];
-            const int i = t + NT * k;
-            if (i < n) sy[i] = v[k];
-        }
-        __syncthreads();
 #pragma unroll
         for (int s = 0; s < LMAX; ++s) {
-            if (s < steps) {
-                const int delta = 1 << s;
+            if (s < levels) {
+                const int delta = 1 << s, p = s & 1;
 #pragma unroll
-                for (int k = 0; k < RPT; ++k) {
-                    const int j = t + NT * k;
-                    if (j < n) {
-                        const int jl = j - delta >= 0 ? j - delta : j;
-                        const int jr = j + delta <= n - 1 ? j + delta : j;
-                        sk[j] = fmaf(Mar[s][k], sy[jr], fmaf(Mal[s][k], sy[jl], v[k]));
-                    }
+                for (int q = 0; q < BR; ++q) {
+                    const int k = t + NT * q;
+                    sY[p][0][k] = v[q][0];
+                    sY[p][1][k] = v[q][1];
                 }
                 __syncthreads();
 #pragma unroll
-                for (int k = 0; k < RPT; ++k) {
-                    const int i = t + NT * k;
-                    if (i < n) {
-                        const int il = i - delta >= 0 ? i - delta : i;
-                        const int ir = i + delta <= n - 1 ? i + delta : i;
-                        v[k] = fmaf(Mg2[s][k], sk[ir], fmaf(Mg1[s][k], sk[il], -v[k]));
-                        sy[i] = v[k];
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        // final 2x2 / 1x1 on the reduced right-hand side
-#pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int i = t + NT * k;
-            if (i < n) {
-                if (i < n - D) {
-                    const float yii = sy[i + D];
-                    sdx[i] = fmaf(F1[k], v[k], F2[k] * yii);
-                    sdx[i + D] = fmaf(F3[k], yii, F4[k] * v[k]);
-                } else if (i < D) {
-                    sdx[i] = F1[k] * v[k];
+                for (int q = 0; q < BR; ++q) {
+                    const int k = t + NT * q;
+                    const int kl = k - delta >= 0 ? k - delta : k;
+                    const int kr = k + delta < nb ? k + delta : k;
+                    const float l0 = sY[p][0][kl], l1 = sY[p][1][kl];
+                    const float r0 = sY[p][0][kr], r1 = sY[p][1][kr];
+                    const M2& al = cA[s][q];
+                    const M2& be = cB[s][q];
+                    v[q][0] = fmaf(be.p, r0, fmaf(be.q, r1, fmaf(al.p, l0, fmaf(al.q, l1, v[q][0]))));
+                    v[q][1] = fmaf(be.r, r0, fmaf(be.s, r1, fmaf(al.r, l0, fmaf(al.s, l1, v[q][1]))));
                 }
             }
         }
-        __syncthreads();
         double dmax = 0.0, xmax = 0.0;
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int i = t + NT * k;
-            if (i < n) {
-                const double dx = (double)sdx[i];
-                X[k] += dx;
-                sX[i] = X[k];
-                dmax = fmax(dmax, fabs(dx));
-                xmax = fmax(xmax, fabs(X[k]));
+        for (int q = 0; q < BR; ++q) {
+            const int k = t + NT * q;
+            const float dx0 = fmaf(Fin[q].p, v[q][0], Fin[q].q * v[q][1]);
+            const float dx1 = fmaf(Fin[q].r, v[q][0], Fin[q].s * v[q][1]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = 2 * k + h;
+                if (i < n) {
+                    const double dx = (double)(h == 0 ? dx0 : dx1);
+                    X[q][h] += dx;
+                    dmax = fmax(dmax, fabs(dx));
+                    xmax = fmax(xmax, fabs(X[q][h]));
+                }
             }
         }
-        dmax = block_max<NT>(dmax, sRed[0]);   // contains a barrier: sX complete
+        dmax = block_max<NT>(dmax, sRed[0]);
         xmax = block_max<NT>(xmax, sRed[1]);
         // Stop when the correction is below tol, or when the error left after
         // applying it, estimated from the observed contraction
@@ -274,39 +250,56 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
             break;
         }
         ++sweep;
-        // r = y - A x, exact FP64
+        // r = y - A x, exact FP64 (y re-read: it is overwritten only at the end)
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int i = t + NT * k;
-            if (i < n) {
-                const double xm2 = i >= 2 ? sX[i - 2] : 0.0, xm1 = i >= 1 ? sX[i - 1] : 0.0;
-                const double xp1 = i + 1 < n ? sX[i + 1] : 0.0, xp2 = i + 2 < n ? sX[i + 2] : 0.0;
-                double ax = sA[2][i] * X[k];
-                ax = __builtin_fma(sA[0][i], xm2, ax);
-                ax = __builtin_fma(sA[1][i], xm1, ax);
-                ax = __builtin_fma(sA[3][i], xp1, ax);
-                ax = __builtin_fma(sA[4][i], xp2, ax);
-                res[k] = Y[k] - ax;
+        for (int q = 0; q < BR; ++q) {
+            const int k = t + NT * q;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (2 * k + h < n) sX[2 * k + h] = X[q][h];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < BR; ++q) {
+            const int k = t + NT * q;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = 2 * k + h;
+                if (i < n) {
+                    const double xm2 = i >= 2 ? sX[i - 2] : 0.0, xm1 = i >= 1 ? sX[i - 1] : 0.0;
+                    const double xp1 = i + 1 < n ? sX[i + 1] : 0.0;
+                    const double xp2 = i + 2 < n ? sX[i + 2] : 0.0;
+                    double ax = sA[2][i] * X[q][h];
+                    ax = __builtin_fma(sA[0][i], xm2, ax);
+                    ax = __builtin_fma(sA[1][i], xm1, ax);
+                    ax = __builtin_fma(sA[3][i], xp1, ax);
+                    ax = __builtin_fma(sA[4][i], xp2, ax);
+                    v[q][h] = (float)(gy[base + (long long)i * rs] - ax);
+                }
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int i = t + NT * k;
-        if (i < n) gy[base + (long long)i * rs] = X[k];
+    for (int q = 0; q < BR; ++q) {
+        const int k = t + NT * q;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = 2 * k + h;
+            if (i < n) gy[base + (long long)i * rs] = X[q][h];
+        }
     }
     if (sweeps_out && t == 0) sweeps_out[sys] = sweep;
 }
 
-template <int NT, int RPT>
+template <int NT, int BR>
 static int launch_f32r(const SolveArgsF32R& A, hipStream_t st)
 {
-    constexpr int LMAX = (NT * RPT >= 2048) ? 11 : (NT * RPT >= 1024) ? 10 : (NT * RPT >= 512) ? 9
-                       : (NT * RPT >= 256) ? 8 : (NT * RPT >= 128) ? 7 : 6;
-    static_assert((1 << (LMAX + 1)) > NT * RPT, "LMAX covers floor(log2 NMAX)");
-    hipLaunchKernelGGL((lds_f32r_kernel<NT, RPT, LMAX>), dim3(A.m), dim3(NT), 0, st, A.a, A.b,
-                       A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.steps, A.max_sweeps, A.tol,
+    constexpr int LMAX = (NT * BR >= 1024) ? 10 : (NT * BR >= 512) ? 9 : (NT * BR >= 256) ? 8
+                       : (NT * BR >= 128) ? 7 : 6;
+    static_assert((1 << LMAX) >= NT * BR, "LMAX covers ceil(log2 block rows)");
+    hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX>), dim3(A.m), dim3(NT), 0, st, A.a, A.b,
+                       A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.max_sweeps, A.tol,
                        A.sweeps_out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -316,12 +309,11 @@ int lds_f32r_max_n() { return 2048; }
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
 {
     const int n = A.n;
-    if (n <= 64) return launch_f32r<64, 1>(A, st);
-    if (n <= 128) return launch_f32r<128, 1>(A, st);
-    if (n <= 256) return launch_f32r<128, 2>(A, st);
-    if (n <= 512) return launch_f32r<256, 2>(A, st);
-    if (n <= 1024) return launch_f32r<256, 4>(A, st);
-    if (n <= 2048) return launch_f32r<512, 4>(A, st);
+    if (n <= 128) return launch_f32r<64, 1>(A, st);
+    if (n <= 256) return launch_f32r<128, 1>(A, st);
+    if (n <= 512) return launch_f32r<256, 1>(A, st);
+    if (n <= 1024) return launch_f32r<512, 1>(A, st);
+    if (n <= 2048) return launch_f32r<512, 2>(A, st);
     return -2;
 }
 
