@@ -90,52 +90,54 @@ __device__ __forceinline__ Row<double> reduce_row(const Row<double>& M, const Pa
     return N;
 }
 
-// ---- FP32 form (inner solver of the mixed-precision refinement) ----------
-// Same recurrences with approximate reciprocals (v_rcp_f32, 1 ulp): the FP64
-// refinement absorbs the difference. Off-diagonal couplings of a dominant
-// system shrink doubly exponentially with the level, so deep-level pivots
-// reach the FP32 denormal range, where v_rcp_f32 returns infinity; pivots
-// below 2^-100 are treated as zero (the coupling they would eliminate is
-// below FP32 resolution), and a zero pivot gives a zero multiplier.
+// ---- FP32 helpers (inner solve of the mixed-precision refinement) --------
+// Approximate reciprocal (v_rcp_f32, 1 ulp): the FP64 refinement absorbs the
+// difference. Pivots below 2^-100 are treated as zero (a zero inverse): deep
+// reduction levels of a dominant system push couplings into the FP32
+// denormal range, where v_rcp_f32 returns infinity.
 constexpr float kTinyF32 = 0x1p-100f;
 
 __device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
 
-__device__ __forceinline__ Pair<float> make_pair(const Row<float>& L, const Row<float>& M,
-                                                 const Row<float>& R, bool hasL, bool aC,
-                                                 bool hasR, bool arC)
+// 2x2 FP32 matrix [[p, q], [r, s]]: the blocks of the block-tridiagonal form
+// the FP32 inner solve works on.
+struct M2 {
+    float p, q, r, s;
+};
+
+constexpr M2 kZero2 = {0.f, 0.f, 0.f, 0.f};
+constexpr M2 kEye2 = {1.f, 0.f, 0.f, 1.f};
+
+__device__ __forceinline__ M2 mmul(const M2& x, const M2& y)
 {
-    Pair<float> E;
-    E.al = (aC && fabsf(L.b) >= kTinyF32) ? -M.a * rcp_f32(L.b) : 0.f;
-    E.ar = (arC && fabsf(R.d) >= kTinyF32) ? -M.e * rcp_f32(R.d) : 0.f;
-    E.A = E.al * L.a;
-    E.B = hasL ? fmaf(E.ar, R.a, fmaf(E.al, L.c, M.b)) : 0.f;
-    E.C = fmaf(E.ar, R.b, fmaf(E.al, L.d, M.c));
-    E.D = hasR ? fmaf(E.ar, R.c, fmaf(E.al, L.e, M.d)) : 0.f;
-    E.E = E.ar * R.e;
-    E.K = fmaf(E.ar, R.y, fmaf(E.al, L.y, M.y));
-    return E;
+    return M2{fmaf(x.p, y.p, x.q * y.r), fmaf(x.p, y.q, x.q * y.s),
+              fmaf(x.r, y.p, x.s * y.r), fmaf(x.r, y.q, x.s * y.s)};
 }
 
-template <typename LoadAl>
-__device__ __forceinline__ Row<float> reduce_row(const Row<float>& M, const Pair<float>& EL,
-                                                 const Pair<float>& ER, bool hp1, bool hq1,
-                                                 bool, bool, LoadAl&&)
+// -(x y)
+__device__ __forceinline__ M2 mmul_neg(const M2& x, const M2& y)
 {
-    const float g1 = (hp1 && fabsf(EL.C) >= kTinyF32) ? M.b * rcp_f32(EL.C) : 0.f;
-    const float g2 = (hq1 && fabsf(ER.C) >= kTinyF32) ? M.d * rcp_f32(ER.C) : 0.f;
-    Row<float> N;
-    N.a = g1 * EL.A;
-    N.b = fmaf(g2, ER.A, fmaf(g1, EL.B, -M.a));
-    N.c = fmaf(g2, ER.B, fmaf(g1, EL.D, -M.c));
-    N.d = fmaf(g2, ER.D, fmaf(g1, EL.E, -M.e));
-    N.e = g2 * ER.E;
-    N.y = fmaf(g2, ER.K, fmaf(g1, EL.K, -M.y));
-    return N;
+    return M2{-fmaf(x.p, y.p, x.q * y.r), -fmaf(x.p, y.q, x.q * y.s),
+              -fmaf(x.r, y.p, x.s * y.r), -fmaf(x.r, y.q, x.s * y.s)};
 }
 
-__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+// d + x y + z w
+__device__ __forceinline__ M2 madd2(const M2& d, const M2& x, const M2& y, const M2& z,
+                                    const M2& w)
+{
+    return M2{fmaf(z.p, w.p, fmaf(z.q, w.r, fmaf(x.p, y.p, fmaf(x.q, y.r, d.p)))),
+              fmaf(z.p, w.q, fmaf(z.q, w.s, fmaf(x.p, y.q, fmaf(x.q, y.s, d.q)))),
+              fmaf(z.r, w.p, fmaf(z.s, w.r, fmaf(x.r, y.p, fmaf(x.s, y.r, d.r)))),
+              fmaf(z.r, w.q, fmaf(z.s, w.s, fmaf(x.r, y.q, fmaf(x.s, y.s, d.s))))};
+}
+
+// Inverse by the adjugate; a determinant below 2^-100 gives the zero matrix.
+__device__ __forceinline__ M2 minv(const M2& m)
+{
+    const float det = fmaf(m.p, m.s, -(m.q * m.r));
+    const float rd = fabsf(det) >= kTinyF32 ? rcp_f32(det) : 0.f;
+    return M2{m.s * rd, -m.q * rd, -m.r * rd, m.p * rd};
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
 // Remap so that each XCD works on runs of C consecutive systems at a time:

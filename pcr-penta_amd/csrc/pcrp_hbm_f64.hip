@@ -38,8 +38,6 @@ namespace pcrp {
 // RPT rows are independent work for the same division latency.
 // in_at(l) / out_at(l) give the element index of row l of this sub-system in
 // the input arrays and of its destination in the output arrays.
-// T is the element type: double for the FP64 solve, float for the inner solve
-// of the mixed-precision refinement (pcrp_device.h's FP32 step form).
 template <typename T, int TB, int G, int RPT, typename InAt, typename OutAt>
 __device__ __forceinline__ void pass_tile(const T* const (&in)[6], T* const (&out)[6],
                                           bool qvalid, int l0, int L, int k, int H, int TI,
@@ -142,16 +140,13 @@ __host__ __forceinline__ long long blocks_for(int groups, int tiles)
 }
 
 // Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
-// stop (optional, m0 flags): systems whose flag is set are skipped (the
-// refinement's converged systems).
 template <typename T, int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const T* __restrict__ ia, const T* __restrict__ ib, const T* __restrict__ ic,
     const T* __restrict__ id, const T* __restrict__ ie, const T* __restrict__ iy,
     T* __restrict__ oa, T* __restrict__ ob, T* __restrict__ oc,
     T* __restrict__ od, T* __restrict__ oe, T* __restrict__ oy,
-    int n0, int m0, int Sin, int k, long long in_rs, long long in_ss, int Qin, int tiles,
-    const int* __restrict__ stop)
+    int n0, int m0, int Sin, int k, long long in_rs, long long in_ss, int Qin, int tiles)
 {
     __shared__ T sR[6][TB * G];
     const int H = 2 * ((1 << k) - 1);
@@ -167,9 +162,8 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const long long Qout = (long long)Qin << k;
     const T* const in[6] = {ia, ib, ic, id, ie, iy};
     T* const out[6] = {oa, ob, oc, od, oe, oy};
-    const bool run = q < Qin && !(stop && stop[q - R * m0]);
     pass_tile<T, TB, G, RPT>(
-        in, out, run, l0, L, k, H, TI,
+        in, out, q < Qin, l0, L, k, H, TI,
         [&](int ll) { return (long long)q * in_ss + (long long)ll * in_rs; },
         [&](int ll) {
             return (long long)(ll >> k) * Qout + (long long)(ll & ((1 << k) - 1)) * Qin + q;
@@ -218,8 +212,7 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
 
 template <typename T, int TB, int G, int RPT>
 static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, int Sin,
-                       int k, long long in_rs, long long in_ss, int Qin, const int* stop,
-                       hipStream_t st)
+                       int k, long long in_rs, long long in_ss, int Qin, hipStream_t st)
 {
     const int H = 2 * ((1 << k) - 1);
     const int TI = TB - 2 * H;
@@ -231,7 +224,7 @@ static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, in
     if (blocks > 0x7fffffffLL) return -2;
     hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT),
                        0, st, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2],
-                       out[3], out[4], out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles, stop);
+                       out[3], out[4], out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -247,9 +240,7 @@ static int launch_pass_uneven_t(const double* const* sets, int src, int dst, con
 }
 
 using PassFn = int (*)(const double* const*, double* const*, int, int, int, int, long long,
-                       long long, int, const int*, hipStream_t);
-using PassF32Fn = int (*)(const float* const*, float* const*, int, int, int, int, long long,
-                          long long, int, const int*, hipStream_t);
+                       long long, int, hipStream_t);
 
 using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, int, long long, int,
                              hipStream_t);
@@ -258,12 +249,10 @@ struct PassShape {
     int tb, g, rpt;
     PassFn fn;
     PassUnevenFn uneven;
-    PassF32Fn f32;
 };
 
-#define PCRP_PASS_SHAPE(TB, G, RPT)                                                     \
-    {TB, G, RPT, &launch_pass<double, TB, G, RPT>, &launch_pass_uneven_t<TB, G, RPT>,  \
-     &launch_pass<float, TB, G, RPT>}
+#define PCRP_PASS_SHAPE(TB, G, RPT) \
+    {TB, G, RPT, &launch_pass<double, TB, G, RPT>, &launch_pass_uneven_t<TB, G, RPT>}
 
 static const PassShape kPassShapes[] = {
     // one (row, sub-system) slot per thread
@@ -291,13 +280,6 @@ int pass_shape_single(int shape)
     for (int i = 0; i < nshapes; ++i)
         if (kPassShapes[i].g == 1 && kPassShapes[i].rpt == kPassShapes[shape].rpt) return i;
     return shape;
-}
-
-int launch_pass_f32(int shape, const float* const* in, float* const* out, int n0, int m0, int Sin,
-                    int k, long long in_rs, long long in_ss, int Qin, const int* stop,
-                    hipStream_t st)
-{
-    return kPassShapes[shape].f32(in, out, n0, m0, Sin, k, in_rs, in_ss, Qin, stop, st);
 }
 
 int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
@@ -364,7 +346,7 @@ int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes
     double* const* dst = set0;
     while (Sin < S) {
         const int k = (S - Sin < tune.k_fuse) ? S - Sin : tune.k_fuse;
-        const int rc = shape.fn(in, dst, n0, m0, Sin, k, in_rs, in_ss, Qin, nullptr, st);
+        const int rc = shape.fn(in, dst, n0, m0, Sin, k, in_rs, in_ss, Qin, st);
         if (rc != 0) return rc;
         Sin += k;
         Qin <<= k;

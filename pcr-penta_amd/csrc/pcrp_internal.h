@@ -101,10 +101,6 @@ int pass_shape_single(int shape);
 int lds_f32r_max_n();
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);
 
-// FP32 instantiation of the uniform HBM pass (stop: per-system skip flags).
-int launch_pass_f32(int shape, const float* const* in, float* const* out, int n0, int m0, int Sin,
-                    int k, long long in_rs, long long in_ss, int Qin, const int* stop,
-                    hipStream_t st);
 // FP32 split solve + FP64 refinement for n above lds_f32r_max_n().
 size_t split_f32r_workspace(int n, int m);
 int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_bytes,

@@ -35,43 +35,6 @@ namespace pcrp {
 
 namespace {
 
-// 2x2 matrix [[p, q], [r, s]]
-struct M2 {
-    float p, q, r, s;
-};
-
-__device__ __forceinline__ M2 mmul(const M2& x, const M2& y)
-{
-    return M2{fmaf(x.p, y.p, x.q * y.r), fmaf(x.p, y.q, x.q * y.s),
-              fmaf(x.r, y.p, x.s * y.r), fmaf(x.r, y.q, x.s * y.s)};
-}
-
-// -(x y)
-__device__ __forceinline__ M2 mmul_neg(const M2& x, const M2& y)
-{
-    return M2{-fmaf(x.p, y.p, x.q * y.r), -fmaf(x.p, y.q, x.q * y.s),
-              -fmaf(x.r, y.p, x.s * y.r), -fmaf(x.r, y.q, x.s * y.s)};
-}
-
-// d + x y + z w
-__device__ __forceinline__ M2 madd2(const M2& d, const M2& x, const M2& y, const M2& z,
-                                    const M2& w)
-{
-    return M2{fmaf(z.p, w.p, fmaf(z.q, w.r, fmaf(x.p, y.p, fmaf(x.q, y.r, d.p)))),
-              fmaf(z.p, w.q, fmaf(z.q, w.s, fmaf(x.p, y.q, fmaf(x.q, y.s, d.q)))),
-              fmaf(z.r, w.p, fmaf(z.s, w.r, fmaf(x.r, y.p, fmaf(x.s, y.r, d.r)))),
-              fmaf(z.r, w.q, fmaf(z.s, w.s, fmaf(x.r, y.q, fmaf(x.s, y.s, d.s))))};
-}
-
-__device__ __forceinline__ M2 minv(const M2& m)
-{
-    const float det = fmaf(m.p, m.s, -(m.q * m.r));
-    const float rd = fabsf(det) >= kTinyF32 ? rcp_f32(det) : 0.f;
-    return M2{m.s * rd, -m.q * rd, -m.r * rd, m.p * rd};
-}
-
-constexpr M2 kZero2 = {0.f, 0.f, 0.f, 0.f};
-
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* red)
 {

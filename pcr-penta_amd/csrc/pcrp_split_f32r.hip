@@ -1,29 +1,32 @@
-// FP32 PCR-Penta + FP64 iterative refinement for systems too large for one
+// FP32 inner solve + FP64 iterative refinement for systems too large for one
 // workgroup (n above the LDS FP32 kernel's limit).
 //
-// No counterpart in the reference, which is FP64-only; the inner solve is the
-// reference's split scheme (shared_cached_shared_tile_splitting,
-// kernel.cu:158-320, then pcr_penta_splitting_2nd_stage, kernel.cu:665-791)
-// in FP32: the HBM passes of the FP64 split path instantiated for float, then
-// an FP32 LDS stage that solves every residue sub-system and adds its
-// correction to x in FP64.
-//
+// No counterpart in the reference, which is FP64-only. The inner solve is
+// PCR on the 2x2-block tridiagonal form of the matrix (block row k = rows 2k,
+// 2k+1; L_k X_{k-1} + D_k X_k + U_k X_{k+1} = Y_k, diagonal-block pivots: the
+// form the one-workgroup FP32 kernel uses, for the same accuracy reason),
+// split the way the reference splits PCR-Penta (stage 1 in HBM,
+// kernel.cu:158-320; strided sub-systems in shared memory, kernel.cu:665-791):
+//   passes : S levels in HBM, k_fuse levels per pass over tiles of TB block
+//            rows of G neighbouring sub-systems, halo 2^k - 1 block rows per
+//            side; output regrouped by residue as in the FP64 passes, so
+//            sub-system (R, j) of block rows R + 2^S l lands at q = R*m + j;
+//   stage  : one workgroup per sub-system, the remaining levels in LDS, then
+//            X = D^-1 Y, and x += dx (FP64) at the caller's elements.
 // One refinement sweep (every kernel skips systems already stopped):
-//   residual : r = y - A x in FP64 from the FP64 inputs, stored as FP32
-//              (first sweep: r = y, x = 0);
-//   passes   : S = ceil(log2(n / lds_max)) FP32 reduction steps in HBM;
-//   stage    : FP32 LDS solve of the 2^S sub-systems, x += dx (FP64), and
-//              per-system max |dx|, max |x| (atomic max on the bit patterns,
-//              which order like the values for non-negative doubles);
-//   decide   : the stopping rule of the one-workgroup FP32 kernel: stop when
+//   residual : r = y - A x in FP64 from the FP64 inputs, stored as the FP32
+//              block right-hand side (first sweep: r = y, x = 0);
+//   passes + stage (factor and solve together: recomputing the FP32 factor
+//              moves fewer bytes than storing its multipliers);
+//   decide   : the stopping rule of the one-workgroup kernel: stop when
 //              max|dx| <= tol max|x|, or when the observed contraction
 //              rho = |dx_k| / |dx_{k-1}| (from the second correction on)
 //              puts the error left, 10 rho |dx_k|, below tol with rho <= 0.1.
 //              A system that is not done after max_sweeps corrections, or
 //              whose correction is NaN, reports max_sweeps + 1.
-// The sweep loop is unrolled on the host (max_sweeps + 1 solves); the
-// stop flags make the launches of converged systems return at once, so the
-// call stays asynchronous (no host read-back) and graph-capturable.
+// The sweep loop is unrolled on the host (max_sweeps + 1 solves); the stop
+// flags make the launches of converged systems return at once, so the call
+// stays asynchronous (no host read-back) and graph-capturable.
 #include "pcr_penta.h"
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
@@ -39,28 +42,100 @@ struct SysState {
     int sweep, pad;
 };
 
-constexpr size_t kAlign = 256;
-size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+// Block row of the FP32 inner solve.
+struct BRow {
+    M2 D, L, U;
+    float y0, y1;
+};
+
+constexpr int kWords = 14;            // floats per block row
+constexpr int kStageNT = 256, kStageBR = 4;
+constexpr int kStageMax = kStageNT * kStageBR;   // block rows per LDS sub-system
+constexpr int kFuse = 4;              // levels per HBM pass
+constexpr int kPassTB = 256, kPassG = 4, kPassRPT = 4;
+
+// LDS record of one block row during a level: D^-1, L, U, Y.
+template <int N>
+__device__ __forceinline__ void put_rec(float (&s)[kWords][N], int i, const M2& Di, const BRow& r)
+{
+    s[0][i] = Di.p; s[1][i] = Di.q; s[2][i] = Di.r; s[3][i] = Di.s;
+    s[4][i] = r.L.p; s[5][i] = r.L.q; s[6][i] = r.L.r; s[7][i] = r.L.s;
+    s[8][i] = r.U.p; s[9][i] = r.U.q; s[10][i] = r.U.r; s[11][i] = r.U.s;
+    s[12][i] = r.y0; s[13][i] = r.y1;
+}
+
+// One level for block row r from the records of its neighbours il / ir
+// (hl / hr: the neighbour exists in the sub-system).
+template <int N>
+__device__ __forceinline__ BRow level(const BRow& r, const float (&s)[kWords][N], bool hl, int il,
+                                      bool hr, int ir)
+{
+    const M2 DiL{s[0][il], s[1][il], s[2][il], s[3][il]};
+    const M2 LL{s[4][il], s[5][il], s[6][il], s[7][il]};
+    const M2 UL{s[8][il], s[9][il], s[10][il], s[11][il]};
+    const M2 DiR{s[0][ir], s[1][ir], s[2][ir], s[3][ir]};
+    const M2 LR{s[4][ir], s[5][ir], s[6][ir], s[7][ir]};
+    const M2 UR{s[8][ir], s[9][ir], s[10][ir], s[11][ir]};
+    const float yl0 = s[12][il], yl1 = s[13][il], yr0 = s[12][ir], yr1 = s[13][ir];
+    const M2 al = hl ? mmul_neg(r.L, DiL) : kZero2;
+    const M2 be = hr ? mmul_neg(r.U, DiR) : kZero2;
+    BRow o;
+    o.D = madd2(r.D, al, UL, be, LR);
+    o.L = mmul(al, LL);
+    o.U = mmul(be, UR);
+    o.y0 = fmaf(be.p, yr0, fmaf(be.q, yr1, fmaf(al.p, yl0, fmaf(al.q, yl1, r.y0))));
+    o.y1 = fmaf(be.r, yr0, fmaf(be.s, yr1, fmaf(al.r, yl0, fmaf(al.s, yl1, r.y1))));
+    return o;
+}
+
+__device__ __forceinline__ BRow load_brow(const float* const* w, long long g)
+{
+    BRow r;
+    r.D = M2{w[0][g], w[1][g], w[2][g], w[3][g]};
+    r.L = M2{w[4][g], w[5][g], w[6][g], w[7][g]};
+    r.U = M2{w[8][g], w[9][g], w[10][g], w[11][g]};
+    r.y0 = w[12][g];
+    r.y1 = w[13][g];
+    return r;
+}
+
+__device__ __forceinline__ void store_brow(float* const* w, long long g, const BRow& r)
+{
+    w[0][g] = r.D.p; w[1][g] = r.D.q; w[2][g] = r.D.r; w[3][g] = r.D.s;
+    w[4][g] = r.L.p; w[5][g] = r.L.q; w[6][g] = r.L.r; w[7][g] = r.L.s;
+    w[8][g] = r.U.p; w[9][g] = r.U.q; w[10][g] = r.U.r; w[11][g] = r.U.s;
+    w[12][g] = r.y0; w[13][g] = r.y1;
+}
+
+struct Arrays14 {
+    float* p[kWords];
+};
 
 struct Layout {
-    size_t state, stop, y0, a32, sets, bytes;
-    size_t per;   // elements per pass array
+    size_t state, stop, y0, lvl0, sets, bytes;
+    size_t per;        // elements per pass array
+    long long nb0;     // block rows per system
     int S;
 };
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 Layout plan(int n, int m)
 {
     Layout L{};
-    const SplitTune tune = split_f64_get();
-    L.S = split_f64_levels(n, tune.lds_max);
+    L.nb0 = (n + 1) / 2;
+    L.S = 0;
+    while (((L.nb0 + (1LL << L.S) - 1) >> L.S) > kStageMax) ++L.S;
     const size_t N = (size_t)n * m;
-    L.per = (size_t)m * ((size_t)((n + (1 << L.S) - 1) >> L.S) << L.S);
+    const size_t NB = (size_t)L.nb0 * m;
+    L.per = (size_t)m * ((size_t)((L.nb0 + (1LL << L.S) - 1) >> L.S) << L.S);
     size_t o = 0;
     L.state = o; o = align_up(o + (size_t)m * sizeof(SysState));
     L.stop = o;  o = align_up(o + (size_t)m * sizeof(int));
     L.y0 = o;    o = align_up(o + N * sizeof(double));
-    L.a32 = o;   o = align_up(o + 6 * N * sizeof(float));    // a..e and r, caller layout
-    L.sets = o;  o = align_up(o + 2 * 6 * L.per * sizeof(float));
+    L.lvl0 = o;  o = align_up(o + kWords * NB * sizeof(float));
+    L.sets = o;  o = align_up(o + 2 * kWords * L.per * sizeof(float));
     L.bytes = o;
     return L;
 }
@@ -79,156 +154,221 @@ __global__ void init_state_kernel(SysState* __restrict__ st, int* __restrict__ s
     }
 }
 
-// Row i of system j at element e (both layouts are dense): i, j from e.
-__device__ __forceinline__ void row_of(long long e, int n, int m, int inter, int& i, int& j)
-{
-    if (inter) { i = (int)(e / m); j = (int)(e - (long long)i * m); }
-    else       { j = (int)(e / n); i = (int)(e - (long long)j * n); }
-}
-
-// FP32 copies of a..e (entries outside the band zeroed, so no out-of-range
-// value can overflow FP32) and of y as the first residual; y0 keeps y.
+// Level-0 block rows (block row K of system j at K*m + j) from the FP64
+// rows 2K, 2K+1 (entries outside the band zeroed; a missing row 2K+1 of an
+// odd n is the identity row); y0 keeps y.
 __global__ void prep_kernel(const double* __restrict__ a, const double* __restrict__ b,
                             const double* __restrict__ c, const double* __restrict__ d,
                             const double* __restrict__ e, const double* __restrict__ y,
-                            float* __restrict__ A32, double* __restrict__ y0, int n, int m,
-                            int inter)
+                            Arrays14 w, double* __restrict__ y0, int n, int m, long long rs,
+                            long long ss, long long nb0)
 {
-    const long long N = (long long)n * m;
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (el >= N) return;
-    int i, j;
-    row_of(el, n, m, inter, i, j);
-    A32[0 * N + el] = i >= 2 ? (float)a[el] : 0.f;
-    A32[1 * N + el] = i >= 1 ? (float)b[el] : 0.f;
-    A32[2 * N + el] = (float)c[el];
-    A32[3 * N + el] = i + 1 < n ? (float)d[el] : 0.f;
-    A32[4 * N + el] = i + 2 < n ? (float)e[el] : 0.f;
-    const double yv = y[el];
-    A32[5 * N + el] = (float)yv;
-    y0[el] = yv;
+    if (el >= nb0 * m) return;
+    const long long K = el / m;
+    const int j = (int)(el - K * m);
+    double rw[2][6];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const long long i = 2 * K + h;
+        rw[h][0] = 0.0; rw[h][1] = 0.0; rw[h][2] = 1.0; rw[h][3] = 0.0; rw[h][4] = 0.0;
+        rw[h][5] = 0.0;
+        if (i < n) {
+            const long long g = (long long)j * ss + i * rs;
+            rw[h][0] = i >= 2 ? a[g] : 0.0;
+            rw[h][1] = i >= 1 ? b[g] : 0.0;
+            rw[h][2] = c[g];
+            rw[h][3] = i + 1 < n ? d[g] : 0.0;
+            rw[h][4] = i + 2 < n ? e[g] : 0.0;
+            rw[h][5] = y[g];
+            y0[g] = rw[h][5];
+        }
+    }
+    BRow r;
+    r.L = M2{(float)rw[0][0], (float)rw[0][1], 0.f, (float)rw[1][0]};
+    r.D = M2{(float)rw[0][2], (float)rw[0][3], (float)rw[1][1], (float)rw[1][2]};
+    r.U = M2{(float)rw[0][4], 0.f, (float)rw[1][3], (float)rw[1][4]};
+    r.y0 = (float)rw[0][5];
+    r.y1 = (float)rw[1][5];
+    store_brow(w.p, el, r);
 }
 
-// r = y0 - A x in FP64 (same operation order as the one-workgroup kernel).
+// r = y0 - A x in FP64 (operation order of the one-workgroup kernel), as the
+// FP32 right-hand side of level-0 block row K.
 __global__ void residual_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                 const double* __restrict__ c, const double* __restrict__ d,
                                 const double* __restrict__ e, const double* __restrict__ y0,
-                                const double* __restrict__ x, float* __restrict__ r32,
-                                const int* __restrict__ stop, int n, int m, int inter)
+                                const double* __restrict__ x, float* __restrict__ r0,
+                                float* __restrict__ r1, const int* __restrict__ stop, int n, int m,
+                                long long rs, long long ss, long long nb0)
 {
-    const long long N = (long long)n * m;
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (el >= N) return;
-    int i, j;
-    row_of(el, n, m, inter, i, j);
+    if (el >= nb0 * m) return;
+    const long long K = el / m;
+    const int j = (int)(el - K * m);
     if (stop[j]) return;
-    const long long rs = inter ? m : 1;
-    const double xm2 = i >= 2 ? x[el - 2 * rs] : 0.0, xm1 = i >= 1 ? x[el - rs] : 0.0;
-    const double xp1 = i + 1 < n ? x[el + rs] : 0.0, xp2 = i + 2 < n ? x[el + 2 * rs] : 0.0;
-    double ax = c[el] * x[el];
-    ax = __builtin_fma(a[el], xm2, ax);
-    ax = __builtin_fma(b[el], xm1, ax);
-    ax = __builtin_fma(d[el], xp1, ax);
-    ax = __builtin_fma(e[el], xp2, ax);
-    r32[el] = (float)(y0[el] - ax);
+    float rv[2] = {0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const long long i = 2 * K + h;
+        if (i < n) {
+            const long long g = (long long)j * ss + i * rs;
+            const double xm2 = i >= 2 ? x[g - 2 * rs] : 0.0, xm1 = i >= 1 ? x[g - rs] : 0.0;
+            const double xp1 = i + 1 < n ? x[g + rs] : 0.0, xp2 = i + 2 < n ? x[g + 2 * rs] : 0.0;
+            double ax = c[g] * x[g];
+            ax = __builtin_fma(a[g], xm2, ax);
+            ax = __builtin_fma(b[g], xm1, ax);
+            ax = __builtin_fma(d[g], xp1, ax);
+            ax = __builtin_fma(e[g], xp2, ax);
+            rv[h] = (float)(y0[g] - ax);
+        }
+    }
+    r0[el] = rv[0];
+    r1[el] = rv[1];
 }
 
-// FP32 LDS stage: one workgroup per residue sub-system q = R*m0 + j (element
-// l at q + l*Q of the pass output), `steps` more reduction steps, the final
-// 2x2 / 1x1 solve, then x += dx at the caller's element of row R + 2^S l.
-template <int NT, int RPT>
-__global__ __launch_bounds__(NT) void stage_f32_kernel(
-    const float* __restrict__ ga, const float* __restrict__ gb, const float* __restrict__ gc,
-    const float* __restrict__ gd, const float* __restrict__ ge, const float* __restrict__ gy,
-    int Q, int steps, int S, int n0, int m0, long long xrs, long long xss,
+// Block -> (group, tile), tiles striped over the 8 XCDs with the group index
+// fastest (the sub-systems of a group share cache lines).
+__device__ __forceinline__ bool tile_of(int b, int groups, int tiles, int& grp, int& tile)
+{
+    constexpr int NX = 8;
+    const int xcd = b % NX, s = b / NX;
+    const int tl = s / groups;
+    grp = s - tl * groups;
+    tile = tl * NX + xcd;
+    return tile < tiles;
+}
+
+// One HBM pass: k levels over a tile of TB block rows of G sub-systems.
+// Current sub-system q = R*m0 + j holds block rows R + 2^Sin l of system j,
+// element l at q + l*Qin; output regrouped by residue (q' = (l mod 2^k)*Qin
+// + q at l >> k).
+template <int TB, int G, int RPT>
+__global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
+    Arrays14 in, Arrays14 out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
+    const int* __restrict__ stop)
+{
+    constexpr int NTH = TB * G / RPT, RSTEP = TB / RPT;
+    __shared__ float sR[kWords][TB * G];
+    const int H = (1 << k) - 1;
+    const int TI = TB - 2 * H;
+    int grp, tile;
+    if (!tile_of(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
+    const int tid = threadIdx.x, lr0 = tid / G, g = tid - lr0 * G;
+    const int q = grp * G + g;
+    const int R = q / m0;
+    const int Lq = (int)((nb0 - R + (1LL << Sin) - 1) >> Sin);
+    const bool run = q < Qin && !stop[q - R * m0];
+    const long long Qout = (long long)Qin << k;
+    const int l0 = tile * TI - H + lr0;
+    BRow r[RPT];
+    bool valid[RPT];
+#pragma unroll
+    for (int kk = 0; kk < RPT; ++kk) {
+        const int l = l0 + RSTEP * kk;
+        valid[kk] = run && l >= 0 && l < Lq;
+        if (valid[kk]) {
+            r[kk] = load_brow(in.p, (long long)q + (long long)l * Qin);
+        } else {
+            r[kk].D = kEye2;
+            r[kk].L = kZero2;
+            r[kk].U = kZero2;
+            r[kk].y0 = r[kk].y1 = 0.f;
+        }
+    }
+    for (int s = 0; s < k; ++s) {
+        const int delta = 1 << s;
+#pragma unroll
+        for (int kk = 0; kk < RPT; ++kk) put_rec(sR, tid + NTH * kk, minv(r[kk].D), r[kk]);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < RPT; ++kk) {
+            if (valid[kk]) {
+                const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk, self = tid + NTH * kk;
+                const bool hl = l - delta >= 0, hr = l + delta < Lq;
+                // neighbours outside the tile only feed block rows that are not kept
+                const int il = (hl && lr - delta >= 0) ? (lr - delta) * G + g : self;
+                const int ir = (hr && lr + delta < TB) ? (lr + delta) * G + g : self;
+                r[kk] = level(r[kk], sR, hl, il, hr, ir);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int kk = 0; kk < RPT; ++kk) {
+        const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
+        if (valid[kk] && lr >= H && lr < H + TI) {
+            const long long go =
+                (long long)(l >> k) * Qout + (long long)(l & ((1 << k) - 1)) * Qin + q;
+            store_brow(out.p, go, r[kk]);
+        }
+    }
+}
+
+// LDS stage: one workgroup per sub-system q = R*m0 + j (element l at q + l*Q),
+// the remaining levels, X = D^-1 Y, x += dx at rows 2K, 2K+1, K = R + 2^S l.
+template <int NT, int BR>
+__global__ __launch_bounds__(NT) void bstage_kernel(
+    Arrays14 in, int Q, int S, long long nb0, int n0, int m0, long long xrs, long long xss,
     double* __restrict__ x, SysState* __restrict__ st, const int* __restrict__ stop, int first)
 {
-    constexpr int NMAX = NT * RPT;
-    __shared__ float sR[6][NMAX];
+    constexpr int NB = NT * BR;
+    __shared__ float sR[kWords][NB];
     __shared__ unsigned long long red[2][NT / 64];
     const int q = xcd_remap(blockIdx.x, Q);
     const int R = q / m0, j = q - R * m0;
     if (stop[j]) return;   // uniform over the workgroup
-    const int n = (n0 - R + (1 << S) - 1) >> S;
+    const int nb = (int)((nb0 - R + (1LL << S) - 1) >> S);
     const int t = threadIdx.x;
-    Row<float> r[RPT];
+    BRow r[BR];
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int i = t + NT * k;
-        if (i < n) {
-            const long long g = (long long)q + (long long)i * Q;
-            r[k].a = ga[g]; r[k].b = gb[g]; r[k].c = gc[g];
-            r[k].d = gd[g]; r[k].e = ge[g]; r[k].y = gy[g];
+    for (int b = 0; b < BR; ++b) {
+        const int l = t + NT * b;
+        if (l < nb) {
+            r[b] = load_brow(in.p, (long long)q + (long long)l * Q);
         } else {
-            r[k] = Row<float>{0.f, 0.f, 1.f, 0.f, 0.f, 0.f};
+            r[b].D = kEye2;
+            r[b].L = kZero2;
+            r[b].U = kZero2;
+            r[b].y0 = r[b].y1 = 0.f;
         }
-        sR[0][t + NT * k] = r[k].a; sR[1][t + NT * k] = r[k].b; sR[2][t + NT * k] = r[k].c;
-        sR[3][t + NT * k] = r[k].d; sR[4][t + NT * k] = r[k].e; sR[5][t + NT * k] = r[k].y;
     }
-    __syncthreads();
-    auto readR = [&](int i) {
-        Row<float> w;
-        w.a = sR[0][i]; w.b = sR[1][i]; w.c = sR[2][i];
-        w.d = sR[3][i]; w.e = sR[4][i]; w.y = sR[5][i];
-        return w;
-    };
-    int delta = 1;
-    for (int s = 0; s < steps; ++s, delta <<= 1) {
+    for (int delta = 1; delta < nb; delta <<= 1) {
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int i = t + NT * k;
-            if (i < n) {
-                const bool hp1 = i - delta >= 0, hq1 = i + delta <= n - 1;
-                const bool hp2 = i - 2 * delta >= 0, hq2 = i + 2 * delta <= n - 1;
-                const bool hp3 = i - 3 * delta >= 0, hq3 = i + 3 * delta <= n - 1;
-                const Row<float> P1 = readR(hp1 ? i - delta : i);
-                const Row<float> P2 = readR(hp2 ? i - 2 * delta : i);
-                const Row<float> Q1 = readR(hq1 ? i + delta : i);
-                const Row<float> Q2 = readR(hq2 ? i + 2 * delta : i);
-                const Pair<float> EL = make_pair(P2, P1, r[k], hp2, hp3, true, hq1);
-                const Pair<float> ER = make_pair(r[k], Q1, Q2, true, hp1, hq2, hq3);
-                r[k] = reduce_row(r[k], EL, ER, hp1, hq1, hp2, hq2,
-                                  [](float&, float&, float&, float&) {});
-            }
-        }
+        for (int b = 0; b < BR; ++b) put_rec(sR, t + NT * b, minv(r[b].D), r[b]);
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < RPT; ++k) {
-            const int i = t + NT * k;
-            if (i < n) {
-                sR[0][i] = r[k].a; sR[1][i] = r[k].b; sR[2][i] = r[k].c;
-                sR[3][i] = r[k].d; sR[4][i] = r[k].e; sR[5][i] = r[k].y;
-            }
+        for (int b = 0; b < BR; ++b) {
+            const int l = t + NT * b;
+            const bool hl = l - delta >= 0, hr = l + delta < nb;
+            r[b] = level(r[b], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l);
         }
         __syncthreads();
     }
-    // final 2x2 / 1x1 (kernel.cu:145-153's branches), x += dx in FP64
     unsigned long long dmax = 0ull, xmax = 0ull;
-    auto update = [&](int i, float dxf) {
-        const long long g = (long long)j * xss + (long long)(R + ((long long)i << S)) * xrs;
-        const double dx = (double)dxf;
-        const double xn = first ? dx : x[g] + dx;
-        x[g] = xn;
-        const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
-        dmax = bd > dmax ? bd : dmax;
-        xmax = bx > xmax ? bx : xmax;
-    };
 #pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        const int i = t + NT * k;
-        if (i < n) {
-            if (i < n - delta) {
-                const float cii = sR[2][i + delta], bii = sR[1][i + delta], yii = sR[5][i + delta];
-                const float deno = fmaf(r[k].c, cii, -(bii * r[k].d));
-                const float rd = fabsf(deno) >= kTinyF32 ? rcp_f32(deno) : 0.f;
-                update(i, fmaf(cii, r[k].y, -(r[k].d * yii)) * rd);
-                update(i + delta, fmaf(r[k].c, yii, -(bii * r[k].y)) * rd);
-            } else if (i < delta) {
-                update(i, fabsf(r[k].c) >= kTinyF32 ? r[k].y * rcp_f32(r[k].c) : 0.f);
+    for (int b = 0; b < BR; ++b) {
+        const int l = t + NT * b;
+        if (l < nb) {
+            const M2 Di = minv(r[b].D);
+            const float d0 = fmaf(Di.p, r[b].y0, Di.q * r[b].y1);
+            const float d1 = fmaf(Di.r, r[b].y0, Di.s * r[b].y1);
+            const long long K = R + ((long long)l << S);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const long long i = 2 * K + h;
+                if (i < n0) {
+                    const long long g = (long long)j * xss + i * xrs;
+                    const double dx = (double)(h == 0 ? d0 : d1);
+                    const double xn = first ? dx : x[g] + dx;
+                    x[g] = xn;
+                    const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
+                    dmax = bd > dmax ? bd : dmax;
+                    xmax = bx > xmax ? bx : xmax;
+                }
             }
         }
     }
-    // workgroup max, then one atomic per quantity
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long od = __shfl_xor(dmax, off, 64), ox = __shfl_xor(xmax, off, 64);
@@ -276,71 +416,68 @@ __global__ void decide_kernel(SysState* __restrict__ st, int* __restrict__ stop,
     st[j] = s;
 }
 
-constexpr int kStageNT = 256, kStageRPT = 4;
-
 }  // namespace
 
 size_t split_f32r_workspace(int n, int m)
 {
-    if (split_f64_levels(n, split_f64_get().lds_max) == 0) return 0;
     return plan(n, m).bytes;
 }
 
 int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_bytes,
                      hipStream_t st)
 {
+    (void)layout;
     const int n0 = A.n, m0 = A.m;
-    const SplitTune tune = split_f64_get();
-    const Layout L = plan(n0, m0);
-    const int S = L.S;
-    if (S == 0 || ((n0 + (1 << S) - 1) >> S) > kStageNT * kStageRPT) return -2;
-    if (!ws || ws_bytes < L.bytes) return -3;
+    const Layout P = plan(n0, m0);
+    const int S = P.S;
+    if (!ws || ws_bytes < P.bytes) return -3;
     char* w = static_cast<char*>(ws);
-    SysState* state = reinterpret_cast<SysState*>(w + L.state);
-    int* stop = reinterpret_cast<int*>(w + L.stop);
-    double* y0 = reinterpret_cast<double*>(w + L.y0);
-    float* A32 = reinterpret_cast<float*>(w + L.a32);
-    float* sets = reinterpret_cast<float*>(w + L.sets);
-    const long long N = (long long)n0 * m0;
-    const int inter = layout == PCRP_LAYOUT_INTERLEAVED;
-    const unsigned eblocks = (unsigned)((N + 255) / 256);
+    SysState* state = reinterpret_cast<SysState*>(w + P.state);
+    int* stop = reinterpret_cast<int*>(w + P.stop);
+    double* y0 = reinterpret_cast<double*>(w + P.y0);
+    const long long NB = P.nb0 * m0;
+    Arrays14 lvl0, set0, set1;
+    for (int v = 0; v < kWords; ++v) {
+        lvl0.p[v] = reinterpret_cast<float*>(w + P.lvl0) + (size_t)v * NB;
+        set0.p[v] = reinterpret_cast<float*>(w + P.sets) + (size_t)v * P.per;
+        set1.p[v] = reinterpret_cast<float*>(w + P.sets) + (size_t)(kWords + v) * P.per;
+    }
+    const unsigned bblocks = (unsigned)((NB + 255) / 256);
     const unsigned sblocks = (unsigned)((m0 + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0);
-    hipLaunchKernelGGL(prep_kernel, dim3(eblocks), dim3(256), 0, st, A.a, A.b, A.c, A.d, A.e,
-                       A.y, A32, y0, n0, m0, inter);
+    hipLaunchKernelGGL(prep_kernel, dim3(bblocks), dim3(256), 0, st, A.a, A.b, A.c, A.d, A.e,
+                       A.y, lvl0, y0, n0, m0, A.rs, A.ss, P.nb0);
     if (hipGetLastError() != hipSuccess) return -1;
-    float* set0[6];
-    float* set1[6];
-    for (int v = 0; v < 6; ++v) {
-        set0[v] = sets + (size_t)v * L.per;
-        set1[v] = sets + (size_t)(6 + v) * L.per;
-    }
     for (int it = 0; it <= A.max_sweeps; ++it) {
-        const float* in[6];
-        for (int v = 0; v < 6; ++v) in[v] = A32 + (size_t)v * N;
-        long long in_rs = A.rs, in_ss = A.ss;
+        Arrays14 in = lvl0;
         int Sin = 0, Qin = m0;
-        float* const* dst = set0;
+        bool to0 = true;
         while (Sin < S) {
-            const int k = (S - Sin < tune.k_fuse) ? S - Sin : tune.k_fuse;
-            const int rc = launch_pass_f32(tune.shape, in, dst, n0, m0, Sin, k, in_rs, in_ss, Qin,
-                                           stop, st);
-            if (rc != 0) return rc;
+            const int k = (S - Sin < kFuse) ? S - Sin : kFuse;
+            const int H = (1 << k) - 1, TI = kPassTB - 2 * H;
+            const long long L = (P.nb0 + (1LL << Sin) - 1) >> Sin;
+            const int tiles = (int)((L + TI - 1) / TI);
+            const int groups = (Qin + kPassG - 1) / kPassG;
+            const long long blocks = (long long)groups * ((tiles + 7) / 8) * 8;
+            if (blocks > 0x7fffffffLL) return -2;
+            Arrays14 out = to0 ? set0 : set1;
+            hipLaunchKernelGGL((bpass_kernel<kPassTB, kPassG, kPassRPT>), dim3((unsigned)blocks),
+                               dim3(kPassTB * kPassG / kPassRPT), 0, st, in, out, P.nb0, m0, Sin,
+                               k, Qin, tiles, stop);
             Sin += k;
             Qin <<= k;
-            for (int v = 0; v < 6; ++v) in[v] = dst[v];
-            in_rs = Qin;
-            in_ss = 1;
-            dst = (dst == set0) ? set1 : set0;
+            in = out;
+            to0 = !to0;
         }
-        hipLaunchKernelGGL((stage_f32_kernel<kStageNT, kStageRPT>), dim3((unsigned)Qin),
-                           dim3(kStageNT), 0, st, in[0], in[1], in[2], in[3], in[4], in[5], Qin,
-                           A.steps - S, S, n0, m0, A.rs, A.ss, A.y, state, stop, it == 0 ? 1 : 0);
+        hipLaunchKernelGGL((bstage_kernel<kStageNT, kStageBR>), dim3((unsigned)Qin),
+                           dim3(kStageNT), 0, st, in, Qin, S, P.nb0, n0, m0, A.rs, A.ss, A.y,
+                           state, stop, it == 0 ? 1 : 0);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0,
                            A.max_sweeps, A.tol, A.sweeps_out);
         if (it < A.max_sweeps)
-            hipLaunchKernelGGL(residual_kernel, dim3(eblocks), dim3(256), 0, st, A.a, A.b, A.c,
-                               A.d, A.e, y0, A.y, A32 + 5 * N, stop, n0, m0, inter);
+            hipLaunchKernelGGL(residual_kernel, dim3(bblocks), dim3(256), 0, st, A.a, A.b, A.c,
+                               A.d, A.e, y0, A.y, lvl0.p[12], lvl0.p[13], stop, n0, m0, A.rs,
+                               A.ss, P.nb0);
         if (hipGetLastError() != hipSuccess) return -1;
     }
     return 0;
