@@ -37,6 +37,7 @@ WORKLOADS = {
     "hbm_f32r_m32_n1048576": (1 << 20, 32, "f32r", 1),
     # uneven: n = 0 marks the config-4 size draw (10000 systems, n log-uniform in [64, 131072])
     "uneven_f64_m10000": (0, 10000, "f64", 0),
+    "uneven_f32r_m10000": (0, 10000, "f32r", 0),
     # Heston ADI (config 3): n = -1 marks the ADI driver; one step = one whole 200-step run
     "heston_adi_f64": (-1, 0, "f64", 0),
     "heston_adi_f32r": (-1, 0, "f32r", 0),
@@ -130,7 +131,9 @@ def bench_uneven(args, m, prec, kind):
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
     tuned = None
-    if not args.no_tune:
+    refined = prec != "f64"
+    sweeps = torch.zeros(m, dtype=torch.int32, device=dev) if refined else None
+    if not args.no_tune and not refined:
         # autotune steps fused per pass and the later rounds' tile shape on this batch
         lds_max, _, _, n_shapes = pcrpenta.split_tune_get()
         best = None
@@ -159,12 +162,16 @@ def bench_uneven(args, m, prec, kind):
         tuned = {"k_fuse": best[1], "shape": best[2], "ms": best[0]}
         print(json.dumps({"split_autotune": tuned}), file=sys.stderr, flush=True)
         del ys_t
-    nb = pcrpenta.uneven_workspace_size(off, nn)
+    nb = pcrpenta.uneven_workspace_size(
+        off, nn, pcrpenta.PREC_F32_REFINE if refined else pcrpenta.PREC_F64)
     ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
 
     def step(k, y=None):
         y = ys[k % nbuf] if y is None else y
+        if refined:
+            return pcrpenta.solve_uneven_refined(a, b, c, d, e, y, off, nn, max_sweeps=MAX_SWEEPS,
+                                                 tol=TOL, sweeps=sweeps, stream=stream, ws=ws)
         return pcrpenta.solve_uneven(a, b, c, d, e, y, off, nn, stream=stream, ws=ws)
 
     for w in range(W):
@@ -192,7 +199,10 @@ def bench_uneven(args, m, prec, kind):
         "config": {"workload": args.workload, "m": m, "unknowns": total, "n_min": int(nn.min()),
                    "n_max": int(nn.max()), "large_systems": int((nn > 1024).sum()),
                    "layout": "contiguous per system", "precision": prec, "seed": seed,
-                   "split": list(pcrpenta.split_tune_get()[:3])},
+                   "split": None if refined else list(pcrpenta.split_tune_get()[:3]),
+                   "refine": ({"max_sweeps": MAX_SWEEPS, "tol": TOL,
+                               "sweeps_hist": torch.bincount(sweeps.long()).tolist()}
+                              if refined else None)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "solve_ms": solve_s * 1e3,
@@ -214,12 +224,20 @@ def bench_uneven(args, m, prec, kind):
             times.append(time.perf_counter() - t1)
         tc = sorted(times[1:])[1]
         xg = y_check[:sub_total].cpu().numpy()
+        check = {"bit_identical_to_gpu": bool(np.array_equal(x, xg))}
+        if refined:
+            errs = []
+            for o, nj in zip(off[:cnt], nn[:cnt]):
+                sl = slice(int(o), int(o) + int(nj))
+                ref = oracle.gbsv(*[t[sl] for t in ins])
+                errs.append(float(np.abs(xg[sl] - ref).max() / np.abs(ref).max()))
+            check = {"gpu_max_relerr_vs_dgbsv": max(errs)}
         out["cpu_baseline"] = {
             "value": sub_total / tc, "unit": "unknowns/s", "cores": oracle.max_threads(),
             "kind": "port",
             "sample": f"first {cnt} systems of the draw ({sub_total} unknowns), FP64 PCR-Penta "
                       f"restated from kernel.cu:32-154, OpenMP dynamic schedule over systems",
-            "seconds": tc, "bit_identical_to_gpu": bool(np.array_equal(x, xg)),
+            "seconds": tc, **check,
         }
     print(json.dumps(out), flush=True)
 

@@ -119,6 +119,21 @@ int pcrp_solve_uneven_f64(const double *a, const double *b, const double *c,
                           const int64_t *h_offsets, const int *h_n, int m,
                           void *ws, size_t ws_bytes, void *stream);
 
+/*
+ * Uneven batch, FP32 inner solve + FP64 refinement (no counterpart in the
+ * reference). Same layout, planning and workspace rules as
+ * pcrp_solve_uneven_f64 (pcrp_uneven_workspace_size with
+ * PCRP_PREC_F32_REFINE); refinement arguments and stopping rule as
+ * pcrp_solve_f32r_f64, sweeps_out a DEVICE array of m ints. The inner solve
+ * is PCR on the 2x2-block form: systems up to 2048 rows in one workgroup,
+ * larger ones after HBM pass rounds shared by all large systems.
+ */
+int pcrp_solve_uneven_f32r_f64(const double *a, const double *b, const double *c,
+                               const double *d, const double *e, double *y,
+                               const int64_t *h_offsets, const int *h_n, int m,
+                               int max_sweeps, double tol, int *sweeps_out,
+                               void *ws, size_t ws_bytes, void *stream);
+
 /* Seeded generator for an uneven batch laid out as above (DEVICE arrays
  * d_offsets / d_n): the same values as pcrp_generate gives system j of a
  * batch, row i at d_offsets[j] + i. */

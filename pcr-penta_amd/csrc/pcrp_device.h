@@ -131,6 +131,9 @@ __device__ __forceinline__ M2 madd2(const M2& d, const M2& x, const M2& y, const
               fmaf(z.r, w.q, fmaf(z.s, w.s, fmaf(x.r, y.q, fmaf(x.s, y.s, d.s))))};
 }
 
+__device__ __forceinline__ float4 f4(const M2& m) { return make_float4(m.p, m.q, m.r, m.s); }
+__device__ __forceinline__ M2 m2(const float4& v) { return M2{v.x, v.y, v.z, v.w}; }
+
 // Inverse by the adjugate; a determinant below 2^-100 gives the zero matrix.
 __device__ __forceinline__ M2 minv(const M2& m)
 {

@@ -105,6 +105,8 @@ int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);
 size_t split_f32r_workspace(int n, int m);
 int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_bytes,
                      hipStream_t st);
+// FP32 inner solve + FP64 refinement over an uneven batch (pcrp_uneven_f32r.hip).
+size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m);
 
 inline int pcr_steps(long long n)
 {

@@ -65,12 +65,13 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
     constexpr int NB = NT * BR;        // block rows
     constexpr int NMAX = 2 * NB;       // rows
     __shared__ double sA[5][NMAX];     // exact FP64 matrix (residual)
-    // factor: D^-1, L, U of every block row; sweeps: FP64 x (aliases it)
-    __shared__ __align__(16) float sF[12][NB];
-    __shared__ float sY[2][2][NB];     // right-hand side exchange, ping-pong
+    // factor: D^-1, L, U of every block row (one 16-byte vector each);
+    // sweeps: FP64 x (aliases it)
+    __shared__ float4 sF[3][NB];
+    __shared__ float2 sY[2][NB];       // right-hand side exchange, ping-pong
     __shared__ double sRed[2][NT / 64];
-    double* const sX = reinterpret_cast<double*>(&sF[0][0]);   // NMAX doubles <= 12*NB floats
-    static_assert(2 * NMAX <= 12 * NB, "x fits in the factor buffer");
+    double* const sX = reinterpret_cast<double*>(&sF[0][0]);   // NMAX doubles <= 3*NB float4
+    static_assert(NMAX * sizeof(double) <= 3 * NB * sizeof(float4), "x fits in the factor buffer");
 
     const int t = threadIdx.x;
     const int sys = xcd_remap(blockIdx.x, m);
@@ -118,10 +119,9 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
 #pragma unroll
             for (int q = 0; q < BR; ++q) {
                 const int k = t + NT * q;
-                const M2 Di = minv(D[q]);
-                sF[0][k] = Di.p; sF[1][k] = Di.q; sF[2][k] = Di.r; sF[3][k] = Di.s;
-                sF[4][k] = L[q].p; sF[5][k] = L[q].q; sF[6][k] = L[q].r; sF[7][k] = L[q].s;
-                sF[8][k] = U[q].p; sF[9][k] = U[q].q; sF[10][k] = U[q].r; sF[11][k] = U[q].s;
+                sF[0][k] = f4(minv(D[q]));
+                sF[1][k] = f4(L[q]);
+                sF[2][k] = f4(U[q]);
             }
             __syncthreads();
 #pragma unroll
@@ -129,12 +129,8 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
                 const int k = t + NT * q;
                 const bool hl = k - delta >= 0, hr = k + delta < nb;
                 const int kl = hl ? k - delta : k, kr = hr ? k + delta : k;
-                const M2 DiL{sF[0][kl], sF[1][kl], sF[2][kl], sF[3][kl]};
-                const M2 LL{sF[4][kl], sF[5][kl], sF[6][kl], sF[7][kl]};
-                const M2 UL{sF[8][kl], sF[9][kl], sF[10][kl], sF[11][kl]};
-                const M2 DiR{sF[0][kr], sF[1][kr], sF[2][kr], sF[3][kr]};
-                const M2 LR{sF[4][kr], sF[5][kr], sF[6][kr], sF[7][kr]};
-                const M2 UR{sF[8][kr], sF[9][kr], sF[10][kr], sF[11][kr]};
+                const M2 DiL = m2(sF[0][kl]), LL = m2(sF[1][kl]), UL = m2(sF[2][kl]);
+                const M2 DiR = m2(sF[0][kr]), LR = m2(sF[1][kr]), UR = m2(sF[2][kr]);
                 const M2 al = hl ? mmul_neg(L[q], DiL) : kZero2;
                 const M2 be = hr ? mmul_neg(U[q], DiR) : kZero2;
                 D[q] = madd2(D[q], al, UL, be, LR);
@@ -161,8 +157,7 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
 #pragma unroll
                 for (int q = 0; q < BR; ++q) {
                     const int k = t + NT * q;
-                    sY[p][0][k] = v[q][0];
-                    sY[p][1][k] = v[q][1];
+                    sY[p][k] = make_float2(v[q][0], v[q][1]);
                 }
                 __syncthreads();
 #pragma unroll
@@ -170,8 +165,8 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
                     const int k = t + NT * q;
                     const int kl = k - delta >= 0 ? k - delta : k;
                     const int kr = k + delta < nb ? k + delta : k;
-                    const float l0 = sY[p][0][kl], l1 = sY[p][1][kl];
-                    const float r0 = sY[p][0][kr], r1 = sY[p][1][kr];
+                    const float2 yl = sY[p][kl], yr = sY[p][kr];
+                    const float l0 = yl.x, l1 = yl.y, r0 = yr.x, r1 = yr.y;
                     const M2& al = cA[s][q];
                     const M2& be = cB[s][q];
                     v[q][0] = fmaf(be.p, r0, fmaf(be.q, r1, fmaf(al.p, l0, fmaf(al.q, l1, v[q][0]))));

@@ -28,88 +28,17 @@
 // flags make the launches of converged systems return at once, so the call
 // stays asynchronous (no host read-back) and graph-capturable.
 #include "pcr_penta.h"
-#include "pcrp_device.h"
+#include "pcrp_block_f32.h"
 #include "pcrp_internal.h"
 
 namespace pcrp {
 
 namespace {
 
-// Per-system refinement state (device).
-struct SysState {
-    unsigned long long dmax, xmax;   // bit patterns of max |dx|, max |x| this sweep
-    double dprev;                    // max |dx| of the previous sweep
-    int sweep, pad;
-};
-
-// Block row of the FP32 inner solve.
-struct BRow {
-    M2 D, L, U;
-    float y0, y1;
-};
-
-constexpr int kWords = 14;            // floats per block row
 constexpr int kStageNT = 256, kStageBR = 4;
 constexpr int kStageMax = kStageNT * kStageBR;   // block rows per LDS sub-system
-constexpr int kFuse = 4;              // levels per HBM pass
+constexpr int kFuse = 4;                         // levels per HBM pass
 constexpr int kPassTB = 256, kPassG = 4, kPassRPT = 4;
-
-// LDS record of one block row during a level: D^-1, L, U, Y.
-template <int N>
-__device__ __forceinline__ void put_rec(float (&s)[kWords][N], int i, const M2& Di, const BRow& r)
-{
-    s[0][i] = Di.p; s[1][i] = Di.q; s[2][i] = Di.r; s[3][i] = Di.s;
-    s[4][i] = r.L.p; s[5][i] = r.L.q; s[6][i] = r.L.r; s[7][i] = r.L.s;
-    s[8][i] = r.U.p; s[9][i] = r.U.q; s[10][i] = r.U.r; s[11][i] = r.U.s;
-    s[12][i] = r.y0; s[13][i] = r.y1;
-}
-
-// One level for block row r from the records of its neighbours il / ir
-// (hl / hr: the neighbour exists in the sub-system).
-template <int N>
-__device__ __forceinline__ BRow level(const BRow& r, const float (&s)[kWords][N], bool hl, int il,
-                                      bool hr, int ir)
-{
-    const M2 DiL{s[0][il], s[1][il], s[2][il], s[3][il]};
-    const M2 LL{s[4][il], s[5][il], s[6][il], s[7][il]};
-    const M2 UL{s[8][il], s[9][il], s[10][il], s[11][il]};
-    const M2 DiR{s[0][ir], s[1][ir], s[2][ir], s[3][ir]};
-    const M2 LR{s[4][ir], s[5][ir], s[6][ir], s[7][ir]};
-    const M2 UR{s[8][ir], s[9][ir], s[10][ir], s[11][ir]};
-    const float yl0 = s[12][il], yl1 = s[13][il], yr0 = s[12][ir], yr1 = s[13][ir];
-    const M2 al = hl ? mmul_neg(r.L, DiL) : kZero2;
-    const M2 be = hr ? mmul_neg(r.U, DiR) : kZero2;
-    BRow o;
-    o.D = madd2(r.D, al, UL, be, LR);
-    o.L = mmul(al, LL);
-    o.U = mmul(be, UR);
-    o.y0 = fmaf(be.p, yr0, fmaf(be.q, yr1, fmaf(al.p, yl0, fmaf(al.q, yl1, r.y0))));
-    o.y1 = fmaf(be.r, yr0, fmaf(be.s, yr1, fmaf(al.r, yl0, fmaf(al.s, yl1, r.y1))));
-    return o;
-}
-
-__device__ __forceinline__ BRow load_brow(const float* const* w, long long g)
-{
-    BRow r;
-    r.D = M2{w[0][g], w[1][g], w[2][g], w[3][g]};
-    r.L = M2{w[4][g], w[5][g], w[6][g], w[7][g]};
-    r.U = M2{w[8][g], w[9][g], w[10][g], w[11][g]};
-    r.y0 = w[12][g];
-    r.y1 = w[13][g];
-    return r;
-}
-
-__device__ __forceinline__ void store_brow(float* const* w, long long g, const BRow& r)
-{
-    w[0][g] = r.D.p; w[1][g] = r.D.q; w[2][g] = r.D.r; w[3][g] = r.D.s;
-    w[4][g] = r.L.p; w[5][g] = r.L.q; w[6][g] = r.L.r; w[7][g] = r.L.s;
-    w[8][g] = r.U.p; w[9][g] = r.U.q; w[10][g] = r.U.r; w[11][g] = r.U.s;
-    w[12][g] = r.y0; w[13][g] = r.y1;
-}
-
-struct Arrays14 {
-    float* p[kWords];
-};
 
 struct Layout {
     size_t state, stop, y0, lvl0, sets, bytes;
@@ -140,23 +69,8 @@ Layout plan(int n, int m)
     return L;
 }
 
-__device__ __forceinline__ unsigned long long abs_bits(double v)
-{
-    return (unsigned long long)__double_as_longlong(fabs(v));
-}
-
-__global__ void init_state_kernel(SysState* __restrict__ st, int* __restrict__ stop, int m)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m) {
-        st[j] = SysState{0ull, 0ull, 0.0, 0, 0};
-        stop[j] = 0;
-    }
-}
-
 // Level-0 block rows (block row K of system j at K*m + j) from the FP64
-// rows 2K, 2K+1 (entries outside the band zeroed; a missing row 2K+1 of an
-// odd n is the identity row); y0 keeps y.
+// rows 2K, 2K+1 of the caller's arrays; y0 keeps y.
 __global__ void prep_kernel(const double* __restrict__ a, const double* __restrict__ b,
                             const double* __restrict__ c, const double* __restrict__ d,
                             const double* __restrict__ e, const double* __restrict__ y,
@@ -171,30 +85,18 @@ __global__ void prep_kernel(const double* __restrict__ a, const double* __restri
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const long long i = 2 * K + h;
-        rw[h][0] = 0.0; rw[h][1] = 0.0; rw[h][2] = 1.0; rw[h][3] = 0.0; rw[h][4] = 0.0;
-        rw[h][5] = 0.0;
+        rw[h][0] = rw[h][1] = rw[h][3] = rw[h][4] = rw[h][5] = 0.0;
+        rw[h][2] = 1.0;   // identity row past the end of an odd n
         if (i < n) {
             const long long g = (long long)j * ss + i * rs;
-            rw[h][0] = i >= 2 ? a[g] : 0.0;
-            rw[h][1] = i >= 1 ? b[g] : 0.0;
-            rw[h][2] = c[g];
-            rw[h][3] = i + 1 < n ? d[g] : 0.0;
-            rw[h][4] = i + 2 < n ? e[g] : 0.0;
-            rw[h][5] = y[g];
+            load_row(a, b, c, d, e, y, g, i, n, rw[h]);
             y0[g] = rw[h][5];
         }
     }
-    BRow r;
-    r.L = M2{(float)rw[0][0], (float)rw[0][1], 0.f, (float)rw[1][0]};
-    r.D = M2{(float)rw[0][2], (float)rw[0][3], (float)rw[1][1], (float)rw[1][2]};
-    r.U = M2{(float)rw[0][4], 0.f, (float)rw[1][3], (float)rw[1][4]};
-    r.y0 = (float)rw[0][5];
-    r.y1 = (float)rw[1][5];
-    store_brow(w.p, el, r);
+    store_brow(w.p, el, brow_from_rows(rw));
 }
 
-// r = y0 - A x in FP64 (operation order of the one-workgroup kernel), as the
-// FP32 right-hand side of level-0 block row K.
+// r = y0 - A x in FP64, as the FP32 right-hand side of level-0 block row K.
 __global__ void residual_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                 const double* __restrict__ c, const double* __restrict__ d,
                                 const double* __restrict__ e, const double* __restrict__ y0,
@@ -213,30 +115,11 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
         const long long i = 2 * K + h;
         if (i < n) {
             const long long g = (long long)j * ss + i * rs;
-            const double xm2 = i >= 2 ? x[g - 2 * rs] : 0.0, xm1 = i >= 1 ? x[g - rs] : 0.0;
-            const double xp1 = i + 1 < n ? x[g + rs] : 0.0, xp2 = i + 2 < n ? x[g + 2 * rs] : 0.0;
-            double ax = c[g] * x[g];
-            ax = __builtin_fma(a[g], xm2, ax);
-            ax = __builtin_fma(b[g], xm1, ax);
-            ax = __builtin_fma(d[g], xp1, ax);
-            ax = __builtin_fma(e[g], xp2, ax);
-            rv[h] = (float)(y0[g] - ax);
+            rv[h] = (float)residual_row(a, b, c, d, e, y0, x, g, i, n, rs);
         }
     }
     r0[el] = rv[0];
     r1[el] = rv[1];
-}
-
-// Block -> (group, tile), tiles striped over the 8 XCDs with the group index
-// fastest (the sub-systems of a group share cache lines).
-__device__ __forceinline__ bool tile_of(int b, int groups, int tiles, int& grp, int& tile)
-{
-    constexpr int NX = 8;
-    const int xcd = b % NX, s = b / NX;
-    const int tl = s / groups;
-    grp = s - tl * groups;
-    tile = tl * NX + xcd;
-    return tile < tiles;
 }
 
 // One HBM pass: k levels over a tile of TB block rows of G sub-systems.
@@ -248,61 +131,24 @@ __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     Arrays14 in, Arrays14 out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
     const int* __restrict__ stop)
 {
-    constexpr int NTH = TB * G / RPT, RSTEP = TB / RPT;
-    __shared__ float sR[kWords][TB * G];
+    __shared__ LdsRec<TB * G> sR;
     const int H = (1 << k) - 1;
     const int TI = TB - 2 * H;
     int grp, tile;
-    if (!tile_of(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
-    const int tid = threadIdx.x, lr0 = tid / G, g = tid - lr0 * G;
+    if (!btile_of(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
+    const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
     const int q = grp * G + g;
     const int R = q / m0;
     const int Lq = (int)((nb0 - R + (1LL << Sin) - 1) >> Sin);
     const bool run = q < Qin && !stop[q - R * m0];
     const long long Qout = (long long)Qin << k;
-    const int l0 = tile * TI - H + lr0;
-    BRow r[RPT];
-    bool valid[RPT];
-#pragma unroll
-    for (int kk = 0; kk < RPT; ++kk) {
-        const int l = l0 + RSTEP * kk;
-        valid[kk] = run && l >= 0 && l < Lq;
-        if (valid[kk]) {
-            r[kk] = load_brow(in.p, (long long)q + (long long)l * Qin);
-        } else {
-            r[kk].D = kEye2;
-            r[kk].L = kZero2;
-            r[kk].U = kZero2;
-            r[kk].y0 = r[kk].y1 = 0.f;
-        }
-    }
-    for (int s = 0; s < k; ++s) {
-        const int delta = 1 << s;
-#pragma unroll
-        for (int kk = 0; kk < RPT; ++kk) put_rec(sR, tid + NTH * kk, minv(r[kk].D), r[kk]);
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < RPT; ++kk) {
-            if (valid[kk]) {
-                const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk, self = tid + NTH * kk;
-                const bool hl = l - delta >= 0, hr = l + delta < Lq;
-                // neighbours outside the tile only feed block rows that are not kept
-                const int il = (hl && lr - delta >= 0) ? (lr - delta) * G + g : self;
-                const int ir = (hr && lr + delta < TB) ? (lr + delta) * G + g : self;
-                r[kk] = level(r[kk], sR, hl, il, hr, ir);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int kk = 0; kk < RPT; ++kk) {
-        const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
-        if (valid[kk] && lr >= H && lr < H + TI) {
-            const long long go =
-                (long long)(l >> k) * Qout + (long long)(l & ((1 << k) - 1)) * Qin + q;
-            store_brow(out.p, go, r[kk]);
-        }
-    }
+    bpass_tile<TB, G, RPT>(
+        in, out, run, tile * TI - H + lr0, Lq, k, H, TI,
+        [&](int l) { return (long long)q + (long long)l * Qin; },
+        [&](int l) {
+            return (long long)(l >> k) * Qout + (long long)(l & ((1 << k) - 1)) * Qin + q;
+        },
+        sR);
 }
 
 // LDS stage: one workgroup per sub-system q = R*m0 + j (element l at q + l*Q),
@@ -312,108 +158,19 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
     Arrays14 in, int Q, int S, long long nb0, int n0, int m0, long long xrs, long long xss,
     double* __restrict__ x, SysState* __restrict__ st, const int* __restrict__ stop, int first)
 {
-    constexpr int NB = NT * BR;
-    __shared__ float sR[kWords][NB];
+    __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
     const int q = xcd_remap(blockIdx.x, Q);
     const int R = q / m0, j = q - R * m0;
     if (stop[j]) return;   // uniform over the workgroup
     const int nb = (int)((nb0 - R + (1LL << S) - 1) >> S);
-    const int t = threadIdx.x;
-    BRow r[BR];
-#pragma unroll
-    for (int b = 0; b < BR; ++b) {
-        const int l = t + NT * b;
-        if (l < nb) {
-            r[b] = load_brow(in.p, (long long)q + (long long)l * Q);
-        } else {
-            r[b].D = kEye2;
-            r[b].L = kZero2;
-            r[b].U = kZero2;
-            r[b].y0 = r[b].y1 = 0.f;
-        }
-    }
-    for (int delta = 1; delta < nb; delta <<= 1) {
-#pragma unroll
-        for (int b = 0; b < BR; ++b) put_rec(sR, t + NT * b, minv(r[b].D), r[b]);
-        __syncthreads();
-#pragma unroll
-        for (int b = 0; b < BR; ++b) {
-            const int l = t + NT * b;
-            const bool hl = l - delta >= 0, hr = l + delta < nb;
-            r[b] = level(r[b], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l);
-        }
-        __syncthreads();
-    }
-    unsigned long long dmax = 0ull, xmax = 0ull;
-#pragma unroll
-    for (int b = 0; b < BR; ++b) {
-        const int l = t + NT * b;
-        if (l < nb) {
-            const M2 Di = minv(r[b].D);
-            const float d0 = fmaf(Di.p, r[b].y0, Di.q * r[b].y1);
-            const float d1 = fmaf(Di.r, r[b].y0, Di.s * r[b].y1);
-            const long long K = R + ((long long)l << S);
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const long long i = 2 * K + h;
-                if (i < n0) {
-                    const long long g = (long long)j * xss + i * xrs;
-                    const double dx = (double)(h == 0 ? d0 : d1);
-                    const double xn = first ? dx : x[g] + dx;
-                    x[g] = xn;
-                    const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
-                    dmax = bd > dmax ? bd : dmax;
-                    xmax = bx > xmax ? bx : xmax;
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long od = __shfl_xor(dmax, off, 64), ox = __shfl_xor(xmax, off, 64);
-        dmax = od > dmax ? od : dmax;
-        xmax = ox > xmax ? ox : xmax;
-    }
-    const int lane = t & 63, wid = t >> 6;
-    if (lane == 0) { red[0][wid] = dmax; red[1][wid] = xmax; }
-    __syncthreads();
-    if (t == 0) {
-#pragma unroll
-        for (int w = 1; w < NT / 64; ++w) {
-            dmax = red[0][w] > dmax ? red[0][w] : dmax;
-            xmax = red[1][w] > xmax ? red[1][w] : xmax;
-        }
-        atomicMax(&st[j].dmax, dmax);
-        atomicMax(&st[j].xmax, xmax);
-    }
-}
-
-__global__ void decide_kernel(SysState* __restrict__ st, int* __restrict__ stop, int m,
-                              int max_sweeps, double tol, int* __restrict__ sweeps_out)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= m || stop[j]) return;
-    SysState s = st[j];
-    const double dmax = __longlong_as_double((long long)s.dmax);
-    const double xmax = __longlong_as_double((long long)s.xmax);
-    const bool bad = dmax != dmax || xmax != xmax;
-    const bool small = !(dmax > tol * xmax);
-    const bool contracting = s.sweep > 1 && dmax <= 0.1 * s.dprev &&
-                             10.0 * dmax * dmax <= tol * xmax * s.dprev;
-    const bool done = !bad && (small || contracting);
-    s.dprev = dmax;
-    if (done || bad || s.sweep >= max_sweeps) {
-        if (bad) s.sweep = max_sweeps + 1;
-        else if (!done) ++s.sweep;          // flag: tolerance not met
-        stop[j] = 1;
-        if (sweeps_out) sweeps_out[j] = s.sweep;
-    } else {
-        ++s.sweep;
-    }
-    s.dmax = 0ull;
-    s.xmax = 0ull;
-    st[j] = s;
+    bstage_solve<NT, BR>(
+        in, nb, [&](int l) { return (long long)q + (long long)l * Q; },
+        [&](int l, int h) {
+            const long long i = 2 * (R + ((long long)l << S)) + h;
+            return i < n0 ? (long long)j * xss + i * xrs : -1LL;
+        },
+        x, first != 0, &st[j], sR, red);
 }
 
 }  // namespace

@@ -212,8 +212,10 @@ extern "C" {
 
 size_t pcrp_uneven_workspace_size(const int64_t* h_offsets, const int* h_n, int m, int precision)
 {
-    if (!h_offsets || !h_n || m < 1 || precision != PCRP_PREC_F64) return 0;
-    return pcrp::uneven_f64_workspace(h_offsets, h_n, m);
+    if (!h_offsets || !h_n || m < 1) return 0;
+    if (precision == PCRP_PREC_F64) return pcrp::uneven_f64_workspace(h_offsets, h_n, m);
+    if (precision == PCRP_PREC_F32_REFINE) return pcrp::uneven_f32r_workspace(h_offsets, h_n, m);
+    return 0;
 }
 
 int pcrp_solve_uneven_f64(const double* a, const double* b, const double* c, const double* d,

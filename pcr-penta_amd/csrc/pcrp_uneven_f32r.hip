@@ -1,0 +1,348 @@
+// Uneven batches, FP32 inner solve + FP64 iterative refinement: thousands of
+// systems of different n in one call.
+//
+// No counterpart in the reference (FP64-only, one (m, n) at a time,
+// main.cu:245-260). Systems are stored contiguously, system j at rows
+// off[j] .. off[j]+n[j]-1 (the reference's per-system layout, kernel.cu:43).
+// The inner solve is the 2x2-block PCR of pcrp_block_f32.h:
+//   * every system becomes nb = ceil(n/2) level-0 block rows, back to back
+//     (block row K of system j at boff[j] + K);
+//   * a system with nb > 1024 first runs S = ceil(log2(nb / 1024)) levels as
+//     HBM pass rounds shared by all such systems (one launch per round, a
+//     table maps blocks to systems). For a single system the residue-split
+//     position of block row l of sub-system q is q + l*2^Sin, the block row's
+//     own index, so passes read and write the same positions;
+//   * then every sub-system (or whole small system) is one job of the LDS
+//     stage, which solves it and adds dx to x in FP64.
+// One refinement sweep: residual (FP64, stored as the FP32 block right-hand
+// side), pass rounds, stage, per-system stopping decision; the stop flags
+// make later launches skip converged systems.
+#include "pcr_penta.h"
+#include "pcrp_block_f32.h"
+#include "pcrp_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace pcrp {
+
+namespace {
+
+constexpr int kStageNT = 256, kStageBR = 4;
+constexpr int kStageMax = kStageNT * kStageBR;   // block rows per LDS job
+constexpr int kFuse = 4;                         // levels per HBM pass round
+
+// Pass shapes: the first round has one sub-system per system (G = 1); later
+// rounds put G neighbouring residue sub-systems side by side.
+constexpr int kTB0 = 1024, kG0 = 1;
+constexpr int kTB1 = 256, kG1 = 4;
+constexpr int kRPT = 4;
+
+struct SysInfo {
+    long long off, boff;   // first row, first level-0 block row
+    int n, nb;
+};
+
+struct PassEntry {
+    long long boff, block0;
+    int nb, k, tiles, sys;
+};
+
+struct Job {
+    int sys, R, S, nb, src, pad;
+};
+
+struct Plan {
+    std::vector<SysInfo> sys;
+    std::vector<Job> jobs;
+    std::vector<std::vector<PassEntry>> rounds;
+    std::vector<long long> round_blocks;
+    long long rows = 0, nbt = 0;
+    bool any_split = false;
+    size_t off_state = 0, off_stop = 0, off_sys = 0, off_jobs = 0, off_y0 = 0, off_lvl0 = 0,
+           off_sets = 0, bytes = 0;
+    std::vector<size_t> round_off;
+};
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int levels_for(int nb)
+{
+    int S = 0;
+    while (((nb + (1 << S) - 1) >> S) > kStageMax) ++S;
+    return S;
+}
+
+int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
+{
+    P.sys.resize(m);
+    std::vector<int> S(m);
+    for (int j = 0; j < m; ++j) {
+        if (h_n[j] < 1 || h_off[j] < 0) return PCRP_ERR_INVALID_SIZE;
+        const long long end = h_off[j] + (long long)h_n[j];
+        if (end > P.rows) P.rows = end;
+        const int nb = (h_n[j] + 1) / 2;
+        P.sys[j] = SysInfo{h_off[j], P.nbt, h_n[j], nb};
+        P.nbt += nb;
+        S[j] = levels_for(nb);
+        if (S[j] > 0) P.any_split = true;
+    }
+    if (P.rows > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
+    // stage jobs, largest sub-systems first
+    for (int j = 0; j < m; ++j) {
+        const int rounds = (S[j] + kFuse - 1) / kFuse;
+        const int src = rounds == 0 ? 0 : 1 + ((rounds - 1) & 1);
+        for (int R = 0; R < (1 << S[j]); ++R) {
+            const int nbs = (P.sys[j].nb - R + (1 << S[j]) - 1) >> S[j];
+            if (nbs > 0) P.jobs.push_back(Job{j, R, S[j], nbs, src, 0});
+        }
+    }
+    std::stable_sort(P.jobs.begin(), P.jobs.end(),
+                     [](const Job& x, const Job& y) { return x.nb > y.nb; });
+    // pass rounds: round r covers levels [r*kFuse, min(S, (r+1)*kFuse))
+    for (int r = 0;; ++r) {
+        const int Sin = r * kFuse;
+        const int TB = r == 0 ? kTB0 : kTB1, G = r == 0 ? kG0 : kG1;
+        std::vector<PassEntry> tab;
+        long long blocks = 0;
+        for (int j = 0; j < m; ++j) {
+            if (S[j] <= Sin) continue;
+            const int k = std::min(kFuse, S[j] - Sin);
+            const int H = (1 << k) - 1, TI = TB - 2 * H;
+            const int L = (P.sys[j].nb + (1 << Sin) - 1) >> Sin;
+            const int tiles = (L + TI - 1) / TI;
+            const int groups = ((1 << Sin) + G - 1) / G;
+            tab.push_back(PassEntry{P.sys[j].boff, blocks, P.sys[j].nb, k, tiles, j});
+            blocks += (long long)groups * ((tiles + 7) / 8) * 8;   // XCD-striped tiles
+        }
+        if (tab.empty()) break;
+        if (blocks > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
+        P.rounds.push_back(std::move(tab));
+        P.round_blocks.push_back(blocks);
+    }
+    size_t o = 0;
+    P.off_state = o; o = align_up(o + (size_t)m * sizeof(SysState));
+    P.off_stop = o;  o = align_up(o + (size_t)m * sizeof(int));
+    P.off_sys = o;   o = align_up(o + (size_t)m * sizeof(SysInfo));
+    P.off_jobs = o;  o = align_up(o + P.jobs.size() * sizeof(Job));
+    for (const auto& t : P.rounds) {
+        P.round_off.push_back(o);
+        o = align_up(o + t.size() * sizeof(PassEntry));
+    }
+    P.off_y0 = o;    o = align_up(o + (size_t)P.rows * sizeof(double));
+    P.off_lvl0 = o;  o = align_up(o + kWords * (size_t)P.nbt * sizeof(float));
+    P.off_sets = o;
+    if (P.any_split) o = align_up(o + 2 * kWords * (size_t)P.nbt * sizeof(float));
+    P.bytes = o;
+    return PCRP_OK;
+}
+
+// System owning level-0 block row e: last j with boff <= e.
+__device__ __forceinline__ int sys_of(const SysInfo* __restrict__ sys, int m, long long e)
+{
+    int lo = 0, hi = m - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sys[mid].boff <= e) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void uprep_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                             const double* __restrict__ c, const double* __restrict__ d,
+                             const double* __restrict__ e, const double* __restrict__ y,
+                             Arrays14 w, double* __restrict__ y0, const SysInfo* __restrict__ sys,
+                             int m, long long nbt)
+{
+    const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (el >= nbt) return;
+    const SysInfo S = sys[sys_of(sys, m, el)];
+    const long long K = el - S.boff;
+    double rw[2][6];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const long long i = 2 * K + h;
+        rw[h][0] = rw[h][1] = rw[h][3] = rw[h][4] = rw[h][5] = 0.0;
+        rw[h][2] = 1.0;   // identity row past the end of an odd n
+        if (i < S.n) {
+            const long long g = S.off + i;
+            load_row(a, b, c, d, e, y, g, i, S.n, rw[h]);
+            y0[g] = rw[h][5];
+        }
+    }
+    store_brow(w.p, el, brow_from_rows(rw));
+}
+
+__global__ void uresidual_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                 const double* __restrict__ c, const double* __restrict__ d,
+                                 const double* __restrict__ e, const double* __restrict__ y0,
+                                 const double* __restrict__ x, float* __restrict__ r0,
+                                 float* __restrict__ r1, const int* __restrict__ stop,
+                                 const SysInfo* __restrict__ sys, int m, long long nbt)
+{
+    const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (el >= nbt) return;
+    const int j = sys_of(sys, m, el);
+    if (stop[j]) return;
+    const SysInfo S = sys[j];
+    const long long K = el - S.boff;
+    float rv[2] = {0.f, 0.f};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const long long i = 2 * K + h;
+        if (i < S.n) rv[h] = (float)residual_row(a, b, c, d, e, y0, x, S.off + i, i, S.n, 1);
+    }
+    r0[el] = rv[0];
+    r1[el] = rv[1];
+}
+
+// One pass round: tab lists the systems that still need levels; a block finds
+// its system by binary search on block0. sets: 0 = level 0, 1 and 2 = ping-pong.
+template <int TB, int G, int RPT>
+__global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
+    Arrays14 src, Arrays14 dst, const PassEntry* __restrict__ tab, int ntab, int Sin,
+    const int* __restrict__ stop)
+{
+    __shared__ LdsRec<TB * G> sR;
+    const long long b = blockIdx.x;
+    int lo = 0, hi = ntab - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    }
+    const PassEntry P = tab[lo];
+    const int Qin = 1 << Sin;
+    int grp, tile;
+    if (!btile_of((int)(b - P.block0), (Qin + G - 1) / G, P.tiles, grp, tile)) return;
+    const int k = P.k;
+    const int H = (1 << k) - 1, TI = TB - 2 * H;
+    const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
+    const int q = grp * G + g;
+    const int Lq = q < Qin ? (P.nb - q + Qin - 1) >> Sin : 0;
+    const bool run = q < Qin && !stop[P.sys];
+    auto at = [&](int l) { return P.boff + q + ((long long)l << Sin); };
+    bpass_tile<TB, G, RPT>(src, dst, run, tile * TI - H + lr0, Lq, k, H, TI, at, at, sR);
+}
+
+template <int NT, int BR>
+__global__ __launch_bounds__(NT) void ustage_kernel(Arrays14 s0, Arrays14 s1, Arrays14 s2,
+                                                    const Job* __restrict__ jobs,
+                                                    const SysInfo* __restrict__ sys,
+                                                    double* __restrict__ x,
+                                                    SysState* __restrict__ st,
+                                                    const int* __restrict__ stop, int first)
+{
+    __shared__ LdsRec<NT * BR> sR;
+    __shared__ unsigned long long red[2][NT / 64];
+    const Job J = jobs[blockIdx.x];
+    if (stop[J.sys]) return;   // uniform over the workgroup
+    const SysInfo S = sys[J.sys];
+    const Arrays14& in = J.src == 0 ? s0 : (J.src == 1 ? s1 : s2);
+    bstage_solve<NT, BR>(
+        in, J.nb, [&](int l) { return S.boff + J.R + ((long long)l << J.S); },
+        [&](int l, int h) {
+            const long long i = 2 * (J.R + ((long long)l << J.S)) + h;
+            return i < S.n ? S.off + i : -1LL;
+        },
+        x, first != 0, &st[J.sys], sR, red);
+}
+
+}  // namespace
+
+size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m)
+{
+    Plan P;
+    if (plan_uneven(h_off, h_n, m, P) != PCRP_OK) return 0;
+    return P.bytes;
+}
+
+int uneven_f32r_solve(const double* a, const double* b, const double* c, const double* d,
+                      const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
+                      int max_sweeps, double tol, int* sweeps_out, void* ws, size_t ws_bytes,
+                      hipStream_t st)
+{
+    Plan P;
+    int rc = plan_uneven(h_off, h_n, m, P);
+    if (rc != PCRP_OK) return rc;
+    if (!ws || ws_bytes < P.bytes) return PCRP_ERR_WORKSPACE;
+    char* w = static_cast<char*>(ws);
+    // host tables -> the front of the workspace in one copy
+    std::vector<char> host(P.off_y0, 0);
+    std::copy(reinterpret_cast<const char*>(P.sys.data()),
+              reinterpret_cast<const char*>(P.sys.data() + P.sys.size()), host.begin() + P.off_sys);
+    std::copy(reinterpret_cast<const char*>(P.jobs.data()),
+              reinterpret_cast<const char*>(P.jobs.data() + P.jobs.size()),
+              host.begin() + P.off_jobs);
+    for (size_t r = 0; r < P.rounds.size(); ++r)
+        std::copy(reinterpret_cast<const char*>(P.rounds[r].data()),
+                  reinterpret_cast<const char*>(P.rounds[r].data() + P.rounds[r].size()),
+                  host.begin() + P.round_off[r]);
+    if (hipMemcpyAsync(w + P.off_sys, host.data() + P.off_sys, host.size() - P.off_sys,
+                       hipMemcpyHostToDevice, st) != hipSuccess)
+        return PCRP_ERR_HIP;
+    SysState* state = reinterpret_cast<SysState*>(w + P.off_state);
+    int* stop = reinterpret_cast<int*>(w + P.off_stop);
+    const SysInfo* dsys = reinterpret_cast<const SysInfo*>(w + P.off_sys);
+    const Job* djobs = reinterpret_cast<const Job*>(w + P.off_jobs);
+    double* y0 = reinterpret_cast<double*>(w + P.off_y0);
+    Arrays14 sets[3];
+    for (int v = 0; v < kWords; ++v) {
+        float* base = reinterpret_cast<float*>(w + P.off_lvl0);
+        sets[0].p[v] = base + (size_t)v * P.nbt;
+        float* ws2 = reinterpret_cast<float*>(w + P.off_sets);
+        sets[1].p[v] = P.any_split ? ws2 + (size_t)v * P.nbt : nullptr;
+        sets[2].p[v] = P.any_split ? ws2 + (size_t)(kWords + v) * P.nbt : nullptr;
+    }
+    const unsigned bblocks = (unsigned)((P.nbt + 255) / 256);
+    const unsigned sblocks = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
+    hipLaunchKernelGGL(uprep_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e, y, sets[0],
+                       y0, dsys, m, P.nbt);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    for (int it = 0; it <= max_sweeps; ++it) {
+        for (size_t r = 0; r < P.rounds.size(); ++r) {
+            const Arrays14& src = sets[r == 0 ? 0 : 1 + ((r - 1) & 1)];
+            const Arrays14& dst = sets[1 + (r & 1)];
+            const PassEntry* tab = reinterpret_cast<const PassEntry*>(w + P.round_off[r]);
+            const int ntab = (int)P.rounds[r].size();
+            const unsigned nblk = (unsigned)P.round_blocks[r];
+            if (r == 0)
+                hipLaunchKernelGGL((upass_kernel<kTB0, kG0, kRPT>), dim3(nblk),
+                                   dim3(kTB0 * kG0 / kRPT), 0, st, src, dst, tab, ntab,
+                                   (int)r * kFuse, stop);
+            else
+                hipLaunchKernelGGL((upass_kernel<kTB1, kG1, kRPT>), dim3(nblk),
+                                   dim3(kTB1 * kG1 / kRPT), 0, st, src, dst, tab, ntab,
+                                   (int)r * kFuse, stop);
+        }
+        hipLaunchKernelGGL((ustage_kernel<kStageNT, kStageBR>), dim3((unsigned)P.jobs.size()),
+                           dim3(kStageNT), 0, st, sets[0], sets[1], sets[2], djobs, dsys, y, state,
+                           stop, it == 0 ? 1 : 0);
+        hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
+                           max_sweeps, tol, sweeps_out);
+        if (it < max_sweeps)
+            hipLaunchKernelGGL(uresidual_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e,
+                               y0, y, sets[0].p[12], sets[0].p[13], stop, dsys, m, P.nbt);
+        if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    }
+    return PCRP_OK;
+}
+
+}  // namespace pcrp
+
+extern "C" {
+
+int pcrp_solve_uneven_f32r_f64(const double* a, const double* b, const double* c, const double* d,
+                               const double* e, double* y, const int64_t* h_offsets,
+                               const int* h_n, int m, int max_sweeps, double tol, int* sweeps_out,
+                               void* ws, size_t ws_bytes, void* stream)
+{
+    if (m < 1) return PCRP_ERR_INVALID_SIZE;
+    if (!a || !b || !c || !d || !e || !y || !h_offsets || !h_n) return PCRP_ERR_NULL;
+    if (max_sweeps < 0 || !(tol >= 0.0)) return PCRP_ERR_ARG;
+    return pcrp::uneven_f32r_solve(a, b, c, d, e, y, h_offsets, h_n, m, max_sweeps, tol,
+                                   sweeps_out, ws, ws_bytes, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

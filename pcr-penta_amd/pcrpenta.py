@@ -79,6 +79,9 @@ def _declare(lib):
     lib.pcrp_uneven_workspace_size.restype = ctypes.c_size_t
     lib.pcrp_solve_uneven_f64.argtypes = [_vp] * 8 + [ctypes.c_int, _vp, ctypes.c_size_t, _vp]
     lib.pcrp_solve_uneven_f64.restype = ctypes.c_int
+    lib.pcrp_solve_uneven_f32r_f64.argtypes = ([_vp] * 8 + [ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_double, _vp, _vp, ctypes.c_size_t, _vp])
+    lib.pcrp_solve_uneven_f32r_f64.restype = ctypes.c_int
     lib.pcrp_generate_uneven.argtypes = [ctypes.c_int, ctypes.c_uint64] + [_vp] * 2 + \
         [ctypes.c_int] + [_vp] * 6 + [_vp]
     lib.pcrp_generate_uneven.restype = ctypes.c_int
@@ -290,11 +293,12 @@ def uneven_layout(sizes):
     return off, n, int(n.astype(np.int64).sum())
 
 
-def uneven_workspace_size(offsets, sizes) -> int:
+def uneven_workspace_size(offsets, sizes, precision: int = PREC_F64) -> int:
     import numpy as np
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     n = np.ascontiguousarray(sizes, dtype=np.int32)
-    return int(load().pcrp_uneven_workspace_size(off.ctypes.data, n.ctypes.data, len(n), PREC_F64))
+    return int(load().pcrp_uneven_workspace_size(off.ctypes.data, n.ctypes.data, len(n),
+                                                 precision))
 
 
 def solve_uneven(a, b, c, d, e, y, offsets, sizes, stream=None, ws=None):
@@ -313,6 +317,28 @@ def solve_uneven(a, b, c, d, e, y, offsets, sizes, stream=None, ws=None):
     st = load().pcrp_solve_uneven_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], off.ctypes.data,
                                       n.ctypes.data, len(n), _ptr(ws), ws.numel(), _stream(stream))
     _check(st, "pcrp_solve_uneven_f64")
+    return y
+
+
+def solve_uneven_refined(a, b, c, d, e, y, offsets, sizes, max_sweeps: int = 8,
+                         tol: float = 1e-12, sweeps=None, stream=None, ws=None):
+    """FP32 inner solve + FP64 refinement of an uneven batch, y overwritten in
+    place. ``sweeps``: optional int32 device tensor of len(sizes) entries."""
+    import numpy as np
+    import torch
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = np.ascontiguousarray(sizes, dtype=np.int32)
+    for t in (a, b, c, d, e, y):
+        if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+            raise TypeError("pcrpenta: arrays must be contiguous float64 device tensors")
+    if ws is None:
+        nb = uneven_workspace_size(off, n, PREC_F32_REFINE)
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y.device)
+    sp = _ptr(sweeps) if sweeps is not None else None
+    st = load().pcrp_solve_uneven_f32r_f64(*[_ptr(t) for t in (a, b, c, d, e, y)],
+                                           off.ctypes.data, n.ctypes.data, len(n), max_sweeps,
+                                           tol, sp, _ptr(ws), ws.numel(), _stream(stream))
+    _check(st, "pcrp_solve_uneven_f32r_f64")
     return y
 
 

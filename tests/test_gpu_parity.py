@@ -384,6 +384,56 @@ def test_uneven_config4_sample(gpu, oracle_lib):
     assert np.array_equal(x, ref)
 
 
+def _uneven_refined_case(gpu, oracle_lib, sizes, kind, seed, max_sweeps=8):
+    import torch
+    off, n, total = gpu.uneven_layout(sizes)
+    dev = gpu.generate_uneven(kind, seed, off, n)
+    ins = [_np(t).copy() for t in dev]
+    sweeps = torch.full((len(n),), -1, dtype=torch.int32, device="cuda")
+    gpu.solve_uneven_refined(*dev, off, n, max_sweeps=max_sweeps, tol=1e-12, sweeps=sweeps)
+    torch.cuda.synchronize()
+    x = _np(dev[5])
+    errs = []
+    for o, nj in zip(off, n):
+        sl = slice(int(o), int(o) + int(nj))
+        ref = oracle_lib.gbsv(*[t[sl] for t in ins])
+        errs.append(np.abs(x[sl] - ref).max() / np.abs(ref).max())
+    for t, h in zip(dev[:5], ins[:5]):      # a..e untouched
+        assert np.array_equal(_np(t), h)
+    return np.array(errs), _np(sweeps)
+
+
+def test_uneven_refined_mixed_sizes(gpu, oracle_lib):
+    """Uneven FP32 + refinement: every system within 1e-12 of dgbsv (normwise),
+    small systems, odd sizes, and sizes that need one or two pass rounds."""
+    rng = np.random.default_rng(12)
+    sizes = [1, 2, 3, 5, 64, 100, 1000, 2047, 2048, 2049, 2051, 4097, 5000, 40000, 70001]
+    sizes += list(rng.integers(1, 6000, 50))
+    for kind in (0, 1):
+        _say("uneven-refined", len(sizes), kind)
+        errs, sw = _uneven_refined_case(gpu, oracle_lib, sizes, kind, 31 + kind)
+        assert errs.max() <= 1e-12, (kind, errs.max(), int(errs.argmax()))
+        assert sw.min() >= 1 and sw.max() <= 3, sw
+
+
+def test_uneven_refined_flags_nonconvergence(gpu, oracle_lib):
+    """max_sweeps = 0 returns the bare FP32 solve, flagged (sweeps = 1)."""
+    sizes = [10, 3000, 5000]
+    _say("uneven-refined-flag", sizes)
+    errs, sw = _uneven_refined_case(gpu, oracle_lib, sizes, 0, 2, max_sweeps=0)
+    assert (sw == 1).all(), sw
+    assert errs.max() < 1e-2 and errs.max() > 1e-9, errs
+
+
+def test_uneven_refined_config4_sample(gpu, oracle_lib):
+    """First 300 systems of the config-4 size draw, FP32 + refinement."""
+    sizes = gpu.config4_sizes()[:300]
+    _say("uneven-refined-c4", len(sizes), int(sizes.sum()))
+    errs, sw = _uneven_refined_case(gpu, oracle_lib, sizes, 0, 1)
+    print("c4 sample refined: max err", errs.max(), "sweeps", np.bincount(sw), flush=True)
+    assert errs.max() <= 1e-12 and sw.max() <= 3
+
+
 # ---------------- Heston ADI driver (hipGraph time steps) ----------------
 
 def _heston_gpu(gpu, precision, use_graph, **kw):
