@@ -144,16 +144,20 @@ __device__ __forceinline__ M2 minv(const M2& m)
 
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
 // Remap so that each XCD works on runs of C consecutive systems at a time:
-// in the interleaved layout those share cache lines, so the line is fetched
-// into one L2 instead of eight. Bijective on [0, m).
-__device__ __forceinline__ int xcd_remap(int b, int m)
+// in the interleaved layout those share cache lines (C = 16 FP64 or 32 FP32
+// values per 128-byte line), so the line is fetched into one L2 instead of
+// eight. Bijective on [0, m).
+template <int C>
+__device__ __forceinline__ int xcd_remap_c(int b, int m)
 {
-    constexpr int NX = 8, C = 16;
+    constexpr int NX = 8;
     const int full = (m / (NX * C)) * (NX * C);
     if (b >= full) return b;
     const int xcd = b % NX, s = b / NX;
     return (s / C) * (NX * C) + xcd * C + (s % C);
 }
+
+__device__ __forceinline__ int xcd_remap(int b, int m) { return xcd_remap_c<16>(b, m); }
 
 // ---- seeded generator (same bits as the oracle's oracle_gen_uniform) ------
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z)

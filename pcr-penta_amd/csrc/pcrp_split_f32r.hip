@@ -35,10 +35,10 @@ namespace pcrp {
 
 namespace {
 
-constexpr int kStageNT = 256, kStageBR = 4;
+constexpr int kStageNT = 512, kStageBR = 2;
 constexpr int kStageMax = kStageNT * kStageBR;   // block rows per LDS sub-system
 constexpr int kFuse = 4;                         // levels per HBM pass
-constexpr int kPassTB = 256, kPassG = 4, kPassRPT = 4;
+constexpr int kPassTB = 256, kPassG = 4, kPassRPT = 2;
 
 struct Layout {
     size_t state, stop, y0, lvl0, sets, bytes;
@@ -141,6 +141,7 @@ __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     const int R = q / m0;
     const int Lq = (int)((nb0 - R + (1LL << Sin) - 1) >> Sin);
     const bool run = q < Qin && !stop[q - R * m0];
+    if (!__syncthreads_or(run)) return;   // every sub-system of this tile has converged
     const long long Qout = (long long)Qin << k;
     bpass_tile<TB, G, RPT>(
         in, out, run, tile * TI - H + lr0, Lq, k, H, TI,
@@ -160,7 +161,7 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
-    const int q = xcd_remap(blockIdx.x, Q);
+    const int q = xcd_remap_c<32>(blockIdx.x, Q);   // 32 FP32 block values per line
     const int R = q / m0, j = q - R * m0;
     if (stop[j]) return;   // uniform over the workgroup
     const int nb = (int)((nb0 - R + (1LL << S) - 1) >> S);

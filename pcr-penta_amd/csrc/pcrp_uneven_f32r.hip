@@ -28,7 +28,7 @@ namespace pcrp {
 
 namespace {
 
-constexpr int kStageNT = 256, kStageBR = 4;
+constexpr int kStageNT = 512, kStageBR = 2;
 constexpr int kStageMax = kStageNT * kStageBR;   // block rows per LDS job
 constexpr int kFuse = 4;                         // levels per HBM pass round
 
@@ -36,7 +36,7 @@ constexpr int kFuse = 4;                         // levels per HBM pass round
 // rounds put G neighbouring residue sub-systems side by side.
 constexpr int kTB0 = 1024, kG0 = 1;
 constexpr int kTB1 = 256, kG1 = 4;
-constexpr int kRPT = 4;
+constexpr int kRPT = 2;
 
 struct SysInfo {
     long long off, boff;   // first row, first level-0 block row
@@ -221,6 +221,7 @@ __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
     const int q = grp * G + g;
     const int Lq = q < Qin ? (P.nb - q + Qin - 1) >> Sin : 0;
     const bool run = q < Qin && !stop[P.sys];
+    if (!__syncthreads_or(run)) return;   // the system has converged
     auto at = [&](int l) { return P.boff + q + ((long long)l << Sin); };
     bpass_tile<TB, G, RPT>(src, dst, run, tile * TI - H + lr0, Lq, k, H, TI, at, at, sR);
 }
@@ -235,7 +236,9 @@ __global__ __launch_bounds__(NT) void ustage_kernel(Arrays14 s0, Arrays14 s1, Ar
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
-    const Job J = jobs[blockIdx.x];
+    // consecutive jobs are neighbouring residues of one system (their block
+    // rows share cache lines): keep runs of 32 of them on one XCD
+    const Job J = jobs[xcd_remap_c<32>(blockIdx.x, (int)gridDim.x)];
     if (stop[J.sys]) return;   // uniform over the workgroup
     const SysInfo S = sys[J.sys];
     const Arrays14& in = J.src == 0 ? s0 : (J.src == 1 ? s1 : s2);
