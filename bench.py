@@ -31,6 +31,7 @@ WORKLOADS = {
     # (config 0/1 distribution), kind 1 = weakly dominant (config 2)
     "lds_f64_m16384_n1024": (1024, 16384, "f64", 0),
     "lds_f32r_m16384_n2048": (2048, 16384, "f32r", 0),
+    "lds_f32r_m16384_n1024": (1024, 16384, "f32r", 0),
     "hbm_f64_m16384_n2048": (2048, 16384, "f64", 0),
     "lds_f64_m256_n512": (512, 256, "f64", 0),
     "hbm_f64_m32_n1048576": (1 << 20, 32, "f64", 1),
