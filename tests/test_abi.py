@@ -116,3 +116,27 @@ def test_uneven_argument_checks(lib):
                                      need - 1, None) == 4
     big = np.array([5000], dtype=np.int32)
     assert lib.pcrp_uneven_workspace_size(off.ctypes.data, big.ctypes.data, 1, 0) >= 2 * 6 * 8 * 5000
+
+
+def test_uneven_refined_argument_checks(lib):
+    """The uneven FP32 + refinement entry point validates like the FP64 one,
+    plus the refinement arguments; its workspace covers the block arrays."""
+    import numpy as np
+    off = np.array([0, 10], dtype=np.int64)
+    n = np.array([10, 0], dtype=np.int32)
+    p = _fake(6)
+    f = lib.pcrp_solve_uneven_f32r_f64
+    assert f(*p, off.ctypes.data, n.ctypes.data, 2, 4, 1e-12, None, None, 0, None) == 1
+    n[1] = 5
+    assert f(*p, None, n.ctypes.data, 2, 4, 1e-12, None, None, 0, None) == 2
+    assert f(*p, off.ctypes.data, n.ctypes.data, 0, 4, 1e-12, None, None, 0, None) == 1
+    assert f(*p, off.ctypes.data, n.ctypes.data, 2, -1, 1e-12, None, None, 0, None) == 6
+    assert f(*p, off.ctypes.data, n.ctypes.data, 2, 4, -1.0, None, None, 0, None) == 6
+    need = lib.pcrp_uneven_workspace_size(off.ctypes.data, n.ctypes.data, 2, 1)
+    assert need > 0
+    assert f(*p, off.ctypes.data, n.ctypes.data, 2, 4, 1e-12, None, _fake(1)[0], need - 1,
+             None) == 4
+    big = np.array([5000], dtype=np.int32)
+    # y copy (8 B/row) + level-0 and two ping-pong sets of 14 FP32 words per block row
+    assert lib.pcrp_uneven_workspace_size(off.ctypes.data, big.ctypes.data, 1, 1) >= \
+        8 * 5000 + 3 * 14 * 4 * 2500
