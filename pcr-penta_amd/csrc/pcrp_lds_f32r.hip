@@ -56,7 +56,10 @@ __device__ __forceinline__ double block_max(double v, double* red)
 // NT threads, BR block rows per thread (block row k = t + NT*q), LMAX levels
 // (>= ceil(log2(NT*BR))).
 template <int NT, int BR, int LMAX>
-__global__ __launch_bounds__(NT) void lds_f32r_kernel(
+// One block row per thread at 512 threads: capped at 128 VGPRs (4 waves per
+// SIMD, two workgroups per CU), which spills a few registers to scratch but
+// doubles the waves per CU (measured 1.37x faster at n = 1024).
+__global__ __launch_bounds__(NT, (BR == 1 && NT == 512) ? 4 : 1) void lds_f32r_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
     int n, int m, long long rs, long long ss, int max_sweeps, double tol,
