@@ -35,20 +35,29 @@ namespace pcrp {
 
 namespace {
 
+// Workgroup max of u and of v (wave64 shuffles, then across waves through
+// LDS, one barrier). red is rewritten only after the caller's next barrier.
 template <int NT>
-__device__ __forceinline__ double block_max(double v, double* red)
+__device__ __forceinline__ void block_max2(double& u, double& v, double (&red)[2][NT / 64])
 {
-    // wave64 reduction then across waves through LDS
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) {
+        u = fmax(u, __shfl_xor(u, off, 64));
+        v = fmax(v, __shfl_xor(v, off, 64));
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = v;
+    if (lane == 0) {
+        red[0][wid] = u;
+        red[1][wid] = v;
+    }
     __syncthreads();
-    double r = red[0];
+    u = red[0][0];
+    v = red[1][0];
 #pragma unroll
-    for (int w = 1; w < NT / 64; ++w) r = fmax(r, red[w]);
-    __syncthreads();
-    return r;
+    for (int w = 1; w < NT / 64; ++w) {
+        u = fmax(u, red[0][w]);
+        v = fmax(v, red[1][w]);
+    }
 }
 
 }  // namespace
@@ -194,8 +203,8 @@ __global__ __launch_bounds__(NT, (BR == 1 && NT == 512) ? 4 : 1) void lds_f32r_k
                 }
             }
         }
-        dmax = block_max<NT>(dmax, sRed[0]);
-        xmax = block_max<NT>(xmax, sRed[1]);
+        // (the next write to sRed follows at least the residual's barrier)
+        block_max2<NT>(dmax, xmax, sRed);
         // Stop when the correction is below tol, or when the error left after
         // applying it, estimated from the observed contraction
         // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k| with a safety factor of 10,
@@ -239,7 +248,8 @@ __global__ __launch_bounds__(NT, (BR == 1 && NT == 512) ? 4 : 1) void lds_f32r_k
                 }
             }
         }
-        __syncthreads();
+        // no barrier here: sX is next written by the next residual, behind the
+        // next sweep's barriers, and the sweep writes only sY
     }
 #pragma unroll
     for (int q = 0; q < BR; ++q) {
