@@ -114,6 +114,17 @@ def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys, prec):
     }
 
 
+def measured_traffic(workload):
+    """HBM bytes of one solve (every solver kernel of one step) measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE and calibrated against a plain copy;
+    None for workloads not profiled."""
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as f:
+        return json.load(f).get(workload, {}).get("bytes_per_step")
+
+
 def bench_uneven(args, m, prec, kind):
     """Config 4: one uneven batch (contiguous systems back to back), seed 1."""
     import numpy as np
@@ -205,7 +216,8 @@ def bench_uneven(args, m, prec, kind):
                                "sweeps_hist": torch.bincount(sweeps.long()).tolist()}
                               if refined else None)},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(args.workload),
                      "solve_ms": solve_s * 1e3,
                      "note": "whole solve (pass rounds + persistent LDS launch) per step"},
     }
@@ -373,11 +385,7 @@ def main():
         sweep_hist = torch.bincount(sweeps.long()).tolist()
 
     achieved = BYTES_PER_UNKNOWN * unknowns / avg_kernel_s
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(args.workload, {}).get("bytes_per_launch")
+    traffic = measured_traffic(args.workload)
     out = {
         "metric": "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12",
         "value": value,

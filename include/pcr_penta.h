@@ -76,24 +76,26 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
                    void *ws, size_t ws_bytes, void *stream);
 
 /*
- * FP32 PCR + FP64 iterative refinement (no counterpart in the reference,
- * which is FP64-only). The FP32 factorisation is computed once per system;
- * x0 = PCR32(y); then up to max_sweeps times: r = y - A x (FP64),
- * dx = PCR32(r), x += dx (FP64). A system stops when ||dx_k|| <= tol ||x||,
- * or when the error left after the update, estimated from the observed
- * contraction as 10 ||dx_k||^2 / ||dx_{k-1}|| (k >= 2), is <= tol ||x|| (and
- * the contraction is <= 0.1). Norms are max-norms. Inputs and output are FP64.
- * sweeps_out: optional DEVICE array of m ints receiving each system's number
- * of refinement sweeps (max_sweeps + 1 flags a system that did not stop).
+ * FP32 inner solve + FP64 iterative refinement (no counterpart in the
+ * reference, which is FP64-only): x0 = solve32(y); then up to max_sweeps
+ * times: r = y - A x (FP64), dx = solve32(r), x += dx (FP64). A system stops
+ * when ||dx_k|| <= tol ||x||, or when the error left after the update,
+ * estimated from the observed contraction as 10 ||dx_k||^2 / ||dx_{k-1}||
+ * (k >= 2), is <= tol ||x|| (and the contraction is <= 0.1). Norms are
+ * max-norms. Inputs and output are FP64. sweeps_out: optional DEVICE array of
+ * m ints receiving each system's number of refinement sweeps (max_sweeps + 1
+ * flags a system that did not stop, or whose correction or iterate is NaN).
+ * solve32 is PCR on the 2x2-block tridiagonal form of the matrix (rows 2k,
+ * 2k+1 as one block row, diagonal-block pivots), which keeps FP32 error near
+ * 3e-7 where PCR-Penta's own off-diagonal pivots lose up to 1e-2.
  * Sizes n <= 2048 run in one LDS-resident kernel per system (the FP32
  * factorisation is computed once and its multipliers kept in registers).
  * Larger n need a workspace (pcrp_workspace_size with PCRP_PREC_F32_REFINE):
- * every sweep re-runs the FP32 split solve (FP32 HBM passes + FP32 LDS
- * stage, the same split scheme as pcrp_solve_f64) on the FP32 residual, with
- * per-system stop flags so converged systems drop out of later sweeps.
- * A zero or below-2^-100 FP32 pivot gives a zero multiplier (the refinement
- * corrects the difference). A NaN correction stops the system with
- * sweeps_out = max_sweeps + 1.
+ * every sweep re-runs the FP32 block solve split like pcrp_solve_f64 (HBM
+ * passes, then an LDS stage per residue sub-system) on the FP32 residual,
+ * with per-system stop flags so converged systems drop out of later sweeps.
+ * A block pivot with determinant below 2^-100 gets a zero inverse (the
+ * refinement corrects the difference, or flags the system).
  */
 int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
                         const double *d, const double *e, double *y,

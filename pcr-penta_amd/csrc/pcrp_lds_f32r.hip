@@ -198,7 +198,10 @@ __global__ __launch_bounds__(NT, (BR == 1 && NT == 512) ? 4 : 1) void lds_f32r_k
                 if (i < n) {
                     const double dx = (double)(h == 0 ? dx0 : dx1);
                     X[q][h] += dx;
-                    dmax = fmax(dmax, fabs(dx));
+                    // a NaN correction or iterate never passes the stopping
+                    // rule (fmax would drop it): the system is flagged
+                    const bool nan = !(dx == dx) || !(X[q][h] == X[q][h]);
+                    dmax = fmax(dmax, nan ? INFINITY : fabs(dx));
                     xmax = fmax(xmax, fabs(X[q][h]));
                 }
             }
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(NT, (BR == 1 && NT == 512) ? 4 : 1) void lds_f32r_k
         // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k| with a safety factor of 10,
         // is below tol (and rho <= 0.1). rho needs two refinement corrections:
         // |dx_1| / |x_0| measures the first solve's error, not the contraction.
-        const bool small = !(dmax > tol * xmax);   // also stops on NaN
+        const bool small = dmax <= tol * xmax;
         const bool contracting = sweep > 1 && dmax <= 0.1 * dprev &&
                                  10.0 * dmax * dmax <= tol * xmax * dprev;
         const bool done = small || contracting;

@@ -217,6 +217,25 @@ def test_refined_config1_shape(gpu, oracle_lib):
     print("sweeps histogram", np.bincount(sw), flush=True)
 
 
+@pytest.mark.parametrize("n", [64, 1000, 3000])
+def test_refined_flags_nan_system(gpu, oracle_lib, n):
+    """A system with a NaN coefficient is flagged (sweeps = max_sweeps + 1)
+    and does not disturb the others (one-workgroup and split paths)."""
+    import torch
+    m = 4
+    _say("refined-nan", n, m)
+    dev = gpu.generate(0, 17, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    dev[2].view(n, m)[n // 2, 1] = float("nan")      # c of system 1
+    x, sw = _refined(gpu, dev, n, m, I, max_sweeps=4)
+    assert sw[1] == 5, sw
+    others = [0, 2, 3]
+    assert all(1 <= sw[j] <= 3 for j in others), sw
+    ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+    xs, rs = x.reshape(n, m)[:, others], ref.reshape(n, m)[:, others]
+    assert np.abs(xs - rs).max() / np.abs(rs).max() <= 1e-12
+
+
 def test_refined_split_reports_nonconvergence(gpu, oracle_lib):
     """Split refinement path: max_sweeps = 0 gives the bare FP32 split solve,
     flagged with sweeps = 1, both layouts."""
