@@ -255,6 +255,8 @@ __device__ __forceinline__ void bstage_solve(const Arrays14& in, int nb, InAt in
     }
 }
 
+// stop[0 .. m-1]: per-system stop flags; stop[m]: systems still running, so
+// every kernel of a sweep after the last system stopped returns at once.
 __global__ void init_state_kernel(SysState* __restrict__ st, int* __restrict__ stop, int m)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -262,6 +264,7 @@ __global__ void init_state_kernel(SysState* __restrict__ st, int* __restrict__ s
         st[j] = SysState{0ull, 0ull, 0.0, 0, 0};
         stop[j] = 0;
     }
+    if (j == 0) stop[m] = m;
 }
 
 // Stopping rule of the one-workgroup refinement kernel, per system: stop when
@@ -287,6 +290,7 @@ __global__ void decide_kernel(SysState* __restrict__ st, int* __restrict__ stop,
         if (bad) s.sweep = max_sweeps + 1;
         else if (!done) ++s.sweep;          // flag: tolerance not met
         stop[j] = 1;
+        atomicSub(&stop[m], 1);
         if (sweeps_out) sweeps_out[j] = s.sweep;
     } else {
         ++s.sweep;

@@ -61,7 +61,7 @@ Layout plan(int n, int m)
     L.per = (size_t)m * ((size_t)((L.nb0 + (1LL << L.S) - 1) >> L.S) << L.S);
     size_t o = 0;
     L.state = o; o = align_up(o + (size_t)m * sizeof(SysState));
-    L.stop = o;  o = align_up(o + (size_t)m * sizeof(int));
+    L.stop = o;  o = align_up(o + (size_t)(m + 1) * sizeof(int));   // flags + running count
     L.y0 = o;    o = align_up(o + N * sizeof(double));
     L.lvl0 = o;  o = align_up(o + kWords * NB * sizeof(float));
     L.sets = o;  o = align_up(o + 2 * kWords * L.per * sizeof(float));
@@ -104,6 +104,7 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
                                 float* __restrict__ r1, const int* __restrict__ stop, int n, int m,
                                 long long rs, long long ss, long long nb0)
 {
+    if (stop[m] == 0) return;   // every system has stopped
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (el >= nb0 * m) return;
     const long long K = el / m;
@@ -132,6 +133,7 @@ __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     const int* __restrict__ stop)
 {
     __shared__ LdsRec<TB * G> sR;
+    if (stop[m0] == 0) return;   // every system has stopped
     const int H = (1 << k) - 1;
     const int TI = TB - 2 * H;
     int grp, tile;
@@ -161,6 +163,7 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
+    if (stop[m0] == 0) return;   // every system has stopped
     const int q = xcd_remap_c<32>(blockIdx.x, Q);   // 32 FP32 block values per line
     const int R = q / m0, j = q - R * m0;
     if (stop[j]) return;   // uniform over the workgroup

@@ -123,7 +123,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
     }
     size_t o = 0;
     P.off_state = o; o = align_up(o + (size_t)m * sizeof(SysState));
-    P.off_stop = o;  o = align_up(o + (size_t)m * sizeof(int));
+    P.off_stop = o;  o = align_up(o + (size_t)(m + 1) * sizeof(int));   // flags + running count
     P.off_sys = o;   o = align_up(o + (size_t)m * sizeof(SysInfo));
     P.off_jobs = o;  o = align_up(o + P.jobs.size() * sizeof(Job));
     for (const auto& t : P.rounds) {
@@ -181,6 +181,7 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
                                  float* __restrict__ r1, const int* __restrict__ stop,
                                  const SysInfo* __restrict__ sys, int m, long long nbt)
 {
+    if (stop[m] == 0) return;   // every system has stopped
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (el >= nbt) return;
     const int j = sys_of(sys, m, el);
@@ -202,9 +203,10 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
     Arrays14 src, Arrays14 dst, const PassEntry* __restrict__ tab, int ntab, int Sin,
-    const int* __restrict__ stop)
+    const int* __restrict__ stop, int m)
 {
     __shared__ LdsRec<TB * G> sR;
+    if (stop[m] == 0) return;   // every system has stopped
     const long long b = blockIdx.x;
     int lo = 0, hi = ntab - 1;
     while (lo < hi) {
@@ -232,10 +234,12 @@ __global__ __launch_bounds__(NT) void ustage_kernel(Arrays14 s0, Arrays14 s1, Ar
                                                     const SysInfo* __restrict__ sys,
                                                     double* __restrict__ x,
                                                     SysState* __restrict__ st,
-                                                    const int* __restrict__ stop, int first)
+                                                    const int* __restrict__ stop, int m,
+                                                    int first)
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
+    if (stop[m] == 0) return;   // every system has stopped
     // consecutive jobs are neighbouring residues of one system (their block
     // rows share cache lines): keep runs of 32 of them on one XCD
     const Job J = jobs[xcd_remap_c<32>(blockIdx.x, (int)gridDim.x)];
@@ -313,15 +317,15 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
             if (r == 0)
                 hipLaunchKernelGGL((upass_kernel<kTB0, kG0, kRPT>), dim3(nblk),
                                    dim3(kTB0 * kG0 / kRPT), 0, st, src, dst, tab, ntab,
-                                   (int)r * kFuse, stop);
+                                   (int)r * kFuse, stop, m);
             else
                 hipLaunchKernelGGL((upass_kernel<kTB1, kG1, kRPT>), dim3(nblk),
                                    dim3(kTB1 * kG1 / kRPT), 0, st, src, dst, tab, ntab,
-                                   (int)r * kFuse, stop);
+                                   (int)r * kFuse, stop, m);
         }
         hipLaunchKernelGGL((ustage_kernel<kStageNT, kStageBR>), dim3((unsigned)P.jobs.size()),
                            dim3(kStageNT), 0, st, sets[0], sets[1], sets[2], djobs, dsys, y, state,
-                           stop, it == 0 ? 1 : 0);
+                           stop, m, it == 0 ? 1 : 0);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
                            max_sweeps, tol, sweeps_out);
         if (it < max_sweeps)
