@@ -149,6 +149,19 @@ __device__ __forceinline__ int sys_of(const SysInfo* __restrict__ sys, int m, lo
     return lo;
 }
 
+// The same for the block rows of a one-dimensional grid: one binary search
+// per workgroup (on its first block row, uniform, so scalar loads), then a
+// short walk: a workgroup spans few systems (each has at least one block row,
+// and the batches this serves have tens to thousands).
+__device__ __forceinline__ int sys_of_row(const SysInfo* __restrict__ sys, int m, long long nbt,
+                                          long long e)
+{
+    const long long e0 = (long long)blockIdx.x * blockDim.x;
+    int j = sys_of(sys, m, e0 < nbt ? e0 : nbt - 1);
+    while (j + 1 < m && sys[j + 1].boff <= e) ++j;
+    return j;
+}
+
 __global__ void uprep_kernel(const double* __restrict__ a, const double* __restrict__ b,
                              const double* __restrict__ c, const double* __restrict__ d,
                              const double* __restrict__ e, const double* __restrict__ y,
@@ -157,7 +170,7 @@ __global__ void uprep_kernel(const double* __restrict__ a, const double* __restr
 {
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (el >= nbt) return;
-    const SysInfo S = sys[sys_of(sys, m, el)];
+    const SysInfo S = sys[sys_of_row(sys, m, nbt, el)];
     const long long K = el - S.boff;
     double rw[2][6];
 #pragma unroll
@@ -184,7 +197,7 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
     if (stop[m] == 0) return;   // every system has stopped
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (el >= nbt) return;
-    const int j = sys_of(sys, m, el);
+    const int j = sys_of_row(sys, m, nbt, el);
     if (stop[j]) return;
     const SysInfo S = sys[j];
     const long long K = el - S.boff;
