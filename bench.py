@@ -78,36 +78,47 @@ def apply_split(args):
         pcrpenta.split_tune_set(lds_max, *args.split)
 
 
+CPU_SECONDS = 10.0   # CPU-baseline sample: repeat the host solve for about this long
+
+
+def timed_repeats(fn, seconds=CPU_SECONDS, min_runs=3):
+    """Run fn() after one warm-up until `seconds` have passed (at least
+    min_runs times); returns (runs, elapsed seconds, last result)."""
+    out = fn()
+    runs, t0 = 0, time.perf_counter()
+    while runs < min_runs or time.perf_counter() - t0 < seconds:
+        out = fn()
+        runs += 1
+    return runs, time.perf_counter() - t0, out
+
+
 def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys, prec):
-    """The oracle's FP64 PCR-Penta over the first `nsys` systems of the same
-    workload, all host threads (OpenMP), median of 3 after one warm-up; also
-    checks the GPU's result on those systems (bit-identical for FP64, error
-    against the FP64 banded-LU oracle for FP32+refine)."""
+    """The oracle's FP64 PCR-Penta on the host (all threads, OpenMP over
+    systems): the first `nsys` systems of the same workload, solved over and
+    over for about CPU_SECONDS; also checks the GPU's result on those systems
+    (bit-identical for FP64, error against the FP64 banded-LU oracle for
+    FP32+refine)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle
     oracle.load()
     arrs = oracle.gen_batch(kind, seed, n, nsys, oracle.INTERLEAVED)
     threads = oracle.max_threads()
-    times = []
-    x = None
-    for _ in range(4):
-        t0 = time.perf_counter()
-        x = oracle.pcr_batch(*arrs, n, nsys, oracle.INTERLEAVED, threads)
-        times.append(time.perf_counter() - t0)
-    t = sorted(times[1:])[1]
+    runs, t, x = timed_repeats(
+        lambda: oracle.pcr_batch(*arrs, n, nsys, oracle.INTERLEAVED, threads))
     agree = err = None
     if gpu_x_sample is not None:
         agree = bool(np.array_equal(x.reshape(n, nsys), gpu_x_sample))
         xg = oracle.gbsv_batch(*arrs, n, nsys, oracle.INTERLEAVED)
         err = oracle.normwise_relerr(gpu_x_sample.reshape(-1), xg, n, nsys, oracle.INTERLEAVED)
     return {
-        "value": n * nsys / t,
+        "value": n * nsys * runs / t,
         "unit": "unknowns/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {nsys} of the {m} systems (n={n}, same generator), FP64 PCR-Penta "
-                  f"restated from kernel.cu:32-154, gcc -O2 + OpenMP over systems, median of 3",
+        "sample": f"first {nsys} of the {m} systems (n={n}, same generator) solved {runs} times "
+                  f"in {t:.1f} s, FP64 PCR-Penta restated from kernel.cu:32-154, gcc -O2 + "
+                  f"OpenMP over systems",
         "seconds": t,
         "bit_identical_to_gpu": agree,
         "gpu_relerr_vs_dgbsv": err,
@@ -230,12 +241,7 @@ def bench_uneven(args, m, prec, kind):
         cnt = min(cnt, m)
         sub_total = int(off[cnt - 1] + nn[cnt - 1])
         ins = [t[:sub_total].cpu().numpy() for t in (a, b, c, d, e, y0)]
-        times = []
-        for _ in range(4):
-            t1 = time.perf_counter()
-            x = oracle.pcr_uneven(*ins, off[:cnt], nn[:cnt])
-            times.append(time.perf_counter() - t1)
-        tc = sorted(times[1:])[1]
+        runs, tc, x = timed_repeats(lambda: oracle.pcr_uneven(*ins, off[:cnt], nn[:cnt]))
         xg = y_check[:sub_total].cpu().numpy()
         check = {"bit_identical_to_gpu": bool(np.array_equal(x, xg))}
         if refined:
@@ -246,10 +252,11 @@ def bench_uneven(args, m, prec, kind):
                 errs.append(float(np.abs(xg[sl] - ref).max() / np.abs(ref).max()))
             check = {"gpu_max_relerr_vs_dgbsv": max(errs)}
         out["cpu_baseline"] = {
-            "value": sub_total / tc, "unit": "unknowns/s", "cores": oracle.max_threads(),
+            "value": sub_total * runs / tc, "unit": "unknowns/s", "cores": oracle.max_threads(),
             "kind": "port",
-            "sample": f"first {cnt} systems of the draw ({sub_total} unknowns), FP64 PCR-Penta "
-                      f"restated from kernel.cu:32-154, OpenMP dynamic schedule over systems",
+            "sample": f"first {cnt} systems of the draw ({sub_total} unknowns) solved {runs} "
+                      f"times in {tc:.1f} s, FP64 PCR-Penta restated from kernel.cu:32-154, "
+                      f"OpenMP dynamic schedule over systems",
             "seconds": tc, **check,
         }
     print(json.dumps(out), flush=True)
