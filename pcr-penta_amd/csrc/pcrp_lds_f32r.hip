@@ -65,10 +65,7 @@ __device__ __forceinline__ void block_max2(double& u, double& v, double (&red)[2
 // NT threads, BR block rows per thread (block row k = t + NT*q), LMAX levels
 // (>= ceil(log2(NT*BR))).
 template <int NT, int BR, int LMAX>
-// One block row per thread at 512 threads: capped at 128 VGPRs (4 waves per
-// SIMD, two workgroups per CU), which spills a few registers to scratch but
-// doubles the waves per CU (measured 1.37x faster at n = 1024).
-__global__ __launch_bounds__(NT, (BR == 1 && NT == 512) ? 4 : 1) void lds_f32r_kernel(
+__global__ __launch_bounds__(NT) void lds_f32r_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
     int n, int m, long long rs, long long ss, int max_sweeps, double tol,
@@ -285,8 +282,13 @@ int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
     const int n = A.n;
     if (n <= 128) return launch_f32r<64, 1>(A, st);
     if (n <= 256) return launch_f32r<128, 1>(A, st);
-    if (n <= 512) return launch_f32r<256, 1>(A, st);
-    if (n <= 1024) return launch_f32r<512, 1>(A, st);
+    // From n = 257 on, two block rows per thread: the cached multipliers of
+    // every level stay in registers without spilling (one block row per
+    // thread at 512 threads needed a 128-register cap for two workgroups per
+    // CU and spilled: 11% slower at n = 1024), and at n = 512 the smaller
+    // workgroups are 8% faster on 1024 systems (equal on 16384).
+    if (n <= 512) return launch_f32r<128, 2>(A, st);
+    if (n <= 1024) return launch_f32r<256, 2>(A, st);
     if (n <= 2048) return launch_f32r<512, 2>(A, st);
     return -2;
 }
