@@ -73,15 +73,24 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
 {
     constexpr int NB = NT * BR;        // block rows
     constexpr int NMAX = 2 * NB;       // rows
-    __shared__ double sA[5][NMAX];     // exact FP64 matrix (residual)
+    // exact FP64 matrix (residual); rows in parity-split order, row i at
+    // lds_pos(i) = (i & 1) * NB + i / 2, so the lanes of a wave (consecutive
+    // block rows k, rows 2k + h) touch consecutive doubles: no bank conflicts
+    // a..e and y: the residuals read no global memory (re-reading y there
+    // cost 22% at n = 1024, waves waiting on the strided loads)
+    __shared__ double sA[6][NMAX];
     // factor: D^-1, L, U of every block row (one 16-byte vector each);
-    // sweeps: FP64 x (aliases it)
+    // sweeps: FP64 x in sF[0..1] and the workgroup-max scratch in sF[2]
+    // (aliases: n = 1024 takes exactly 80 KB, two workgroups per CU, and
+    // n = 2048 exactly the CU's 160 KB)
     __shared__ float4 sF[3][NB];
     __shared__ float2 sY[2][NB];       // right-hand side exchange, ping-pong
-    __shared__ double sRed[2][NT / 64];
-    double* const sX = reinterpret_cast<double*>(&sF[0][0]);   // NMAX doubles <= 3*NB float4
-    static_assert(NMAX * sizeof(double) <= 3 * NB * sizeof(float4), "x fits in the factor buffer");
+    double* const sX = reinterpret_cast<double*>(&sF[0][0]);
+    static_assert(NMAX * sizeof(double) <= 2 * NB * sizeof(float4), "x fits in sF[0..1]");
+    static_assert(2 * (NT / 64) * sizeof(double) <= NB * sizeof(float4), "scratch fits in sF[2]");
+    auto& sRed = *reinterpret_cast<double (*)[2][NT / 64]>(&sF[2][0]);
 
+    const auto lds_pos = [](int i) { return (i & 1) * NB + (i >> 1); };
     const int t = threadIdx.x;
     const int sys = xcd_remap(blockIdx.x, m);
     const long long base = (long long)sys * ss;
@@ -106,7 +115,9 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
                 if (i < 1) b = 0.0;
                 if (i > n - 2) d = 0.0;
                 if (i > n - 3) e = 0.0;
-                sA[0][i] = a; sA[1][i] = b; sA[2][i] = c; sA[3][i] = d; sA[4][i] = e;
+                const int p = lds_pos(i);
+                sA[0][p] = a; sA[1][p] = b; sA[2][p] = c; sA[3][p] = d; sA[4][p] = e;
+                sA[5][p] = y;
             }
             rw[h][0] = a; rw[h][1] = b; rw[h][2] = c; rw[h][3] = d; rw[h][4] = e;
             X[q][h] = 0.0;
@@ -220,13 +231,13 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
             break;
         }
         ++sweep;
-        // r = y - A x, exact FP64 (y re-read: it is overwritten only at the end)
+        // r = y - A x, exact FP64 (y is overwritten only at the end)
 #pragma unroll
         for (int q = 0; q < BR; ++q) {
             const int k = t + NT * q;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-                if (2 * k + h < n) sX[2 * k + h] = X[q][h];
+                if (2 * k + h < n) sX[h * NB + k] = X[q][h];
         }
         __syncthreads();
 #pragma unroll
@@ -236,15 +247,17 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
             for (int h = 0; h < 2; ++h) {
                 const int i = 2 * k + h;
                 if (i < n) {
-                    const double xm2 = i >= 2 ? sX[i - 2] : 0.0, xm1 = i >= 1 ? sX[i - 1] : 0.0;
-                    const double xp1 = i + 1 < n ? sX[i + 1] : 0.0;
-                    const double xp2 = i + 2 < n ? sX[i + 2] : 0.0;
-                    double ax = sA[2][i] * X[q][h];
-                    ax = __builtin_fma(sA[0][i], xm2, ax);
-                    ax = __builtin_fma(sA[1][i], xm1, ax);
-                    ax = __builtin_fma(sA[3][i], xp1, ax);
-                    ax = __builtin_fma(sA[4][i], xp2, ax);
-                    v[q][h] = (float)(gy[base + (long long)i * rs] - ax);
+                    const double xm2 = i >= 2 ? sX[lds_pos(i - 2)] : 0.0;
+                    const double xm1 = i >= 1 ? sX[lds_pos(i - 1)] : 0.0;
+                    const double xp1 = i + 1 < n ? sX[lds_pos(i + 1)] : 0.0;
+                    const double xp2 = i + 2 < n ? sX[lds_pos(i + 2)] : 0.0;
+                    const int p = lds_pos(i);
+                    double ax = sA[2][p] * X[q][h];
+                    ax = __builtin_fma(sA[0][p], xm2, ax);
+                    ax = __builtin_fma(sA[1][p], xm1, ax);
+                    ax = __builtin_fma(sA[3][p], xp1, ax);
+                    ax = __builtin_fma(sA[4][p], xp2, ax);
+                    v[q][h] = (float)(sA[5][p] - ax);
                 }
             }
         }
