@@ -20,16 +20,22 @@
 namespace pcrp {
 namespace {
 
-constexpr int kWords = 14;   // floats per block row: D, L, U (row-major) and Y
-
 struct BRow {
     M2 D, L, U;
     float y0, y1;
 };
 
-struct Arrays14 {
-    float* p[kWords];
+// Block rows in global memory: block row g's matrix part is the 48-byte
+// record m[3g .. 3g+2] (D, L, U, each row-major) and its right-hand side
+// y[g]. A record is three 16-byte loads, so a workgroup that walks one
+// sub-system (records far apart) issues 4 memory requests per block row
+// instead of 14 for one array per word; the residual writes y alone,
+// coalesced.
+struct BRows {
+    float4* m;
+    float2* y;
 };
+constexpr size_t kBRowBytes = 3 * sizeof(float4) + sizeof(float2);   // 56
 
 // Per-system refinement state (device).
 struct SysState {
@@ -98,23 +104,37 @@ __device__ __forceinline__ BRow level(const BRow& r, const LdsRec<N>& s, bool hl
     return o;
 }
 
-__device__ __forceinline__ BRow load_brow(const float* const* w, long long g)
+__device__ __forceinline__ BRow load_brow(const BRows& w, long long g)
 {
     BRow r;
-    r.D = M2{w[0][g], w[1][g], w[2][g], w[3][g]};
-    r.L = M2{w[4][g], w[5][g], w[6][g], w[7][g]};
-    r.U = M2{w[8][g], w[9][g], w[10][g], w[11][g]};
-    r.y0 = w[12][g];
-    r.y1 = w[13][g];
+    r.D = m2(w.m[3 * g]);
+    r.L = m2(w.m[3 * g + 1]);
+    r.U = m2(w.m[3 * g + 2]);
+    const float2 y = w.y[g];
+    r.y0 = y.x;
+    r.y1 = y.y;
     return r;
 }
 
-__device__ __forceinline__ void store_brow(float* const* w, long long g, const BRow& r)
+__device__ __forceinline__ void store_brow(const BRows& w, long long g, const BRow& r)
 {
-    w[0][g] = r.D.p; w[1][g] = r.D.q; w[2][g] = r.D.r; w[3][g] = r.D.s;
-    w[4][g] = r.L.p; w[5][g] = r.L.q; w[6][g] = r.L.r; w[7][g] = r.L.s;
-    w[8][g] = r.U.p; w[9][g] = r.U.q; w[10][g] = r.U.r; w[11][g] = r.U.s;
-    w[12][g] = r.y0; w[13][g] = r.y1;
+    w.m[3 * g] = f4(r.D);
+    w.m[3 * g + 1] = f4(r.L);
+    w.m[3 * g + 2] = f4(r.U);
+    w.y[g] = make_float2(r.y0, r.y1);
+}
+
+// Bytes of a set of nb block rows (records, then right-hand sides), rounded
+// up to 256 so that sets carved one after the other stay aligned.
+__host__ inline size_t brows_bytes(size_t nb) { return (nb * kBRowBytes + 255) / 256 * 256; }
+
+// Carve a set of nb block rows out of workspace memory at p (256-byte
+// aligned); returns where the next set starts.
+__host__ inline char* carve_brows(char* p, size_t nb, BRows& w)
+{
+    w.m = reinterpret_cast<float4*>(p);
+    w.y = reinterpret_cast<float2*>(p + nb * 3 * sizeof(float4));
+    return p + brows_bytes(nb);
 }
 
 __device__ __forceinline__ unsigned long long abs_bits(double v)
@@ -145,7 +165,7 @@ __host__ __forceinline__ long long bblocks_for(int groups, int tiles)
 // rows stored. in_at(l) / out_at(l): element of block row l in / out.
 // Each thread owns RPT slots (rows lr0 + (TB/RPT)*kk of the same slot g).
 template <int TB, int G, int RPT, typename InAt, typename OutAt>
-__device__ __forceinline__ void bpass_tile(const Arrays14& in, const Arrays14& out, bool run,
+__device__ __forceinline__ void bpass_tile(const BRows& in, const BRows& out, bool run,
                                            int l0, int Lq, int k, int H, int TI, InAt in_at,
                                            OutAt out_at, LdsRec<TB * G>& sR)
 {
@@ -157,7 +177,7 @@ __device__ __forceinline__ void bpass_tile(const Arrays14& in, const Arrays14& o
     for (int kk = 0; kk < RPT; ++kk) {
         const int l = l0 + RSTEP * kk;
         valid[kk] = run && l >= 0 && l < Lq;
-        r[kk] = valid[kk] ? load_brow(in.p, in_at(l)) : identity_brow();
+        r[kk] = valid[kk] ? load_brow(in, in_at(l)) : identity_brow();
     }
     for (int s = 0; s < k; ++s) {
         const int delta = 1 << s;
@@ -180,7 +200,7 @@ __device__ __forceinline__ void bpass_tile(const Arrays14& in, const Arrays14& o
 #pragma unroll
     for (int kk = 0; kk < RPT; ++kk) {
         const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
-        if (valid[kk] && lr >= H && lr < H + TI) store_brow(out.p, out_at(l), r[kk]);
+        if (valid[kk] && lr >= H && lr < H + TI) store_brow(out, out_at(l), r[kk]);
     }
 }
 
@@ -189,7 +209,7 @@ __device__ __forceinline__ void bpass_tile(const Arrays14& in, const Arrays14& o
 // elements x_at(l, h) returns (negative: no such row), and the workgroup's
 // max |dx| and max |x| go to *st by atomic max. All threads call it together.
 template <int NT, int BR, typename InAt, typename XAt>
-__device__ __forceinline__ void bstage_solve(const Arrays14& in, int nb, InAt in_at, XAt x_at,
+__device__ __forceinline__ void bstage_solve(const BRows& in, int nb, InAt in_at, XAt x_at,
                                              double* __restrict__ x, bool first, SysState* st,
                                              LdsRec<NT * BR>& sR,
                                              unsigned long long (&red)[2][NT / 64])
@@ -199,7 +219,7 @@ __device__ __forceinline__ void bstage_solve(const Arrays14& in, int nb, InAt in
 #pragma unroll
     for (int b = 0; b < BR; ++b) {
         const int l = t + NT * b;
-        r[b] = l < nb ? load_brow(in.p, in_at(l)) : identity_brow();
+        r[b] = l < nb ? load_brow(in, in_at(l)) : identity_brow();
     }
     for (int delta = 1; delta < nb; delta <<= 1) {
 #pragma unroll

@@ -63,8 +63,8 @@ Layout plan(int n, int m)
     L.state = o; o = align_up(o + (size_t)m * sizeof(SysState));
     L.stop = o;  o = align_up(o + (size_t)(m + 1) * sizeof(int));   // flags + running count
     L.y0 = o;    o = align_up(o + N * sizeof(double));
-    L.lvl0 = o;  o = align_up(o + kWords * NB * sizeof(float));
-    L.sets = o;  o = align_up(o + 2 * kWords * L.per * sizeof(float));
+    L.lvl0 = o;  o = align_up(o + brows_bytes(NB));
+    L.sets = o;  o = align_up(o + 2 * brows_bytes(L.per));
     L.bytes = o;
     return L;
 }
@@ -74,7 +74,7 @@ Layout plan(int n, int m)
 __global__ void prep_kernel(const double* __restrict__ a, const double* __restrict__ b,
                             const double* __restrict__ c, const double* __restrict__ d,
                             const double* __restrict__ e, const double* __restrict__ y,
-                            Arrays14 w, double* __restrict__ y0, int n, int m, long long rs,
+                            BRows w, double* __restrict__ y0, int n, int m, long long rs,
                             long long ss, long long nb0)
 {
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -93,15 +93,15 @@ __global__ void prep_kernel(const double* __restrict__ a, const double* __restri
             y0[g] = rw[h][5];
         }
     }
-    store_brow(w.p, el, brow_from_rows(rw));
+    store_brow(w, el, brow_from_rows(rw));
 }
 
 // r = y0 - A x in FP64, as the FP32 right-hand side of level-0 block row K.
 __global__ void residual_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                 const double* __restrict__ c, const double* __restrict__ d,
                                 const double* __restrict__ e, const double* __restrict__ y0,
-                                const double* __restrict__ x, float* __restrict__ r0,
-                                float* __restrict__ r1, const int* __restrict__ stop, int n, int m,
+                                const double* __restrict__ x, float2* __restrict__ r,
+                                const int* __restrict__ stop, int n, int m,
                                 long long rs, long long ss, long long nb0)
 {
     if (stop[m] == 0) return;   // every system has stopped
@@ -119,8 +119,7 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
             rv[h] = (float)residual_row(a, b, c, d, e, y0, x, g, i, n, rs);
         }
     }
-    r0[el] = rv[0];
-    r1[el] = rv[1];
+    r[el] = make_float2(rv[0], rv[1]);
 }
 
 // One HBM pass: k levels over a tile of TB block rows of G sub-systems.
@@ -129,7 +128,7 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
 // + q at l >> k).
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
-    Arrays14 in, Arrays14 out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
+    BRows in, BRows out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
     const int* __restrict__ stop)
 {
     __shared__ LdsRec<TB * G> sR;
@@ -158,7 +157,7 @@ __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
 // the remaining levels, X = D^-1 Y, x += dx at rows 2K, 2K+1, K = R + 2^S l.
 template <int NT, int BR>
 __global__ __launch_bounds__(NT) void bstage_kernel(
-    Arrays14 in, int Q, int S, long long nb0, int n0, int m0, long long xrs, long long xss,
+    BRows in, int Q, int S, long long nb0, int n0, int m0, long long xrs, long long xss,
     double* __restrict__ x, SysState* __restrict__ st, const int* __restrict__ stop, int first)
 {
     __shared__ LdsRec<NT * BR> sR;
@@ -197,12 +196,9 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
     int* stop = reinterpret_cast<int*>(w + P.stop);
     double* y0 = reinterpret_cast<double*>(w + P.y0);
     const long long NB = P.nb0 * m0;
-    Arrays14 lvl0, set0, set1;
-    for (int v = 0; v < kWords; ++v) {
-        lvl0.p[v] = reinterpret_cast<float*>(w + P.lvl0) + (size_t)v * NB;
-        set0.p[v] = reinterpret_cast<float*>(w + P.sets) + (size_t)v * P.per;
-        set1.p[v] = reinterpret_cast<float*>(w + P.sets) + (size_t)(kWords + v) * P.per;
-    }
+    BRows lvl0, set0, set1;
+    carve_brows(w + P.lvl0, (size_t)NB, lvl0);
+    carve_brows(carve_brows(w + P.sets, P.per, set0), P.per, set1);
     const unsigned bblocks = (unsigned)((NB + 255) / 256);
     const unsigned sblocks = (unsigned)((m0 + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0);
@@ -210,7 +206,7 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
                        A.y, lvl0, y0, n0, m0, A.rs, A.ss, P.nb0);
     if (hipGetLastError() != hipSuccess) return -1;
     for (int it = 0; it <= A.max_sweeps; ++it) {
-        Arrays14 in = lvl0;
+        BRows in = lvl0;
         int Sin = 0, Qin = m0;
         bool to0 = true;
         while (Sin < S) {
@@ -221,7 +217,7 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
             const int groups = (Qin + kPassG - 1) / kPassG;
             const long long blocks = (long long)groups * ((tiles + 7) / 8) * 8;
             if (blocks > 0x7fffffffLL) return -2;
-            Arrays14 out = to0 ? set0 : set1;
+            BRows out = to0 ? set0 : set1;
             hipLaunchKernelGGL((bpass_kernel<kPassTB, kPassG, kPassRPT>), dim3((unsigned)blocks),
                                dim3(kPassTB * kPassG / kPassRPT), 0, st, in, out, P.nb0, m0, Sin,
                                k, Qin, tiles, stop);
@@ -237,7 +233,7 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
                            A.max_sweeps, A.tol, A.sweeps_out);
         if (it < A.max_sweeps)
             hipLaunchKernelGGL(residual_kernel, dim3(bblocks), dim3(256), 0, st, A.a, A.b, A.c,
-                               A.d, A.e, y0, A.y, lvl0.p[12], lvl0.p[13], stop, n0, m0, A.rs,
+                               A.d, A.e, y0, A.y, lvl0.y, stop, n0, m0, A.rs,
                                A.ss, P.nb0);
         if (hipGetLastError() != hipSuccess) return -1;
     }

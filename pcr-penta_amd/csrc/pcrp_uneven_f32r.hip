@@ -131,9 +131,9 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
         o = align_up(o + t.size() * sizeof(PassEntry));
     }
     P.off_y0 = o;    o = align_up(o + (size_t)P.rows * sizeof(double));
-    P.off_lvl0 = o;  o = align_up(o + kWords * (size_t)P.nbt * sizeof(float));
+    P.off_lvl0 = o;  o = align_up(o + brows_bytes((size_t)P.nbt));
     P.off_sets = o;
-    if (P.any_split) o = align_up(o + 2 * kWords * (size_t)P.nbt * sizeof(float));
+    if (P.any_split) o = align_up(o + 2 * brows_bytes((size_t)P.nbt));
     P.bytes = o;
     return PCRP_OK;
 }
@@ -165,7 +165,7 @@ __device__ __forceinline__ int sys_of_row(const SysInfo* __restrict__ sys, int m
 __global__ void uprep_kernel(const double* __restrict__ a, const double* __restrict__ b,
                              const double* __restrict__ c, const double* __restrict__ d,
                              const double* __restrict__ e, const double* __restrict__ y,
-                             Arrays14 w, double* __restrict__ y0, const SysInfo* __restrict__ sys,
+                             BRows w, double* __restrict__ y0, const SysInfo* __restrict__ sys,
                              int m, long long nbt)
 {
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,14 +184,14 @@ __global__ void uprep_kernel(const double* __restrict__ a, const double* __restr
             y0[g] = rw[h][5];
         }
     }
-    store_brow(w.p, el, brow_from_rows(rw));
+    store_brow(w, el, brow_from_rows(rw));
 }
 
 __global__ void uresidual_kernel(const double* __restrict__ a, const double* __restrict__ b,
                                  const double* __restrict__ c, const double* __restrict__ d,
                                  const double* __restrict__ e, const double* __restrict__ y0,
-                                 const double* __restrict__ x, float* __restrict__ r0,
-                                 float* __restrict__ r1, const int* __restrict__ stop,
+                                 const double* __restrict__ x, float2* __restrict__ r,
+                                 const int* __restrict__ stop,
                                  const SysInfo* __restrict__ sys, int m, long long nbt)
 {
     if (stop[m] == 0) return;   // every system has stopped
@@ -207,15 +207,14 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
         const long long i = 2 * K + h;
         if (i < S.n) rv[h] = (float)residual_row(a, b, c, d, e, y0, x, S.off + i, i, S.n, 1);
     }
-    r0[el] = rv[0];
-    r1[el] = rv[1];
+    r[el] = make_float2(rv[0], rv[1]);
 }
 
 // One pass round: tab lists the systems that still need levels; a block finds
 // its system by binary search on block0. sets: 0 = level 0, 1 and 2 = ping-pong.
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
-    Arrays14 src, Arrays14 dst, const PassEntry* __restrict__ tab, int ntab, int Sin,
+    BRows src, BRows dst, const PassEntry* __restrict__ tab, int ntab, int Sin,
     const int* __restrict__ stop, int m)
 {
     __shared__ LdsRec<TB * G> sR;
@@ -242,7 +241,7 @@ __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
 }
 
 template <int NT, int BR>
-__global__ __launch_bounds__(NT) void ustage_kernel(Arrays14 s0, Arrays14 s1, Arrays14 s2,
+__global__ __launch_bounds__(NT) void ustage_kernel(BRows s0, BRows s1, BRows s2,
                                                     const Job* __restrict__ jobs,
                                                     const SysInfo* __restrict__ sys,
                                                     double* __restrict__ x,
@@ -258,7 +257,7 @@ __global__ __launch_bounds__(NT) void ustage_kernel(Arrays14 s0, Arrays14 s1, Ar
     const Job J = jobs[xcd_remap_c<32>(blockIdx.x, (int)gridDim.x)];
     if (stop[J.sys]) return;   // uniform over the workgroup
     const SysInfo S = sys[J.sys];
-    const Arrays14& in = J.src == 0 ? s0 : (J.src == 1 ? s1 : s2);
+    const BRows& in = J.src == 0 ? s0 : (J.src == 1 ? s1 : s2);
     bstage_solve<NT, BR>(
         in, J.nb, [&](int l) { return S.boff + J.R + ((long long)l << J.S); },
         [&](int l, int h) {
@@ -306,14 +305,10 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
     const SysInfo* dsys = reinterpret_cast<const SysInfo*>(w + P.off_sys);
     const Job* djobs = reinterpret_cast<const Job*>(w + P.off_jobs);
     double* y0 = reinterpret_cast<double*>(w + P.off_y0);
-    Arrays14 sets[3];
-    for (int v = 0; v < kWords; ++v) {
-        float* base = reinterpret_cast<float*>(w + P.off_lvl0);
-        sets[0].p[v] = base + (size_t)v * P.nbt;
-        float* ws2 = reinterpret_cast<float*>(w + P.off_sets);
-        sets[1].p[v] = P.any_split ? ws2 + (size_t)v * P.nbt : nullptr;
-        sets[2].p[v] = P.any_split ? ws2 + (size_t)(kWords + v) * P.nbt : nullptr;
-    }
+    BRows sets[3] = {};
+    carve_brows(w + P.off_lvl0, (size_t)P.nbt, sets[0]);
+    if (P.any_split)
+        carve_brows(carve_brows(w + P.off_sets, (size_t)P.nbt, sets[1]), (size_t)P.nbt, sets[2]);
     const unsigned bblocks = (unsigned)((P.nbt + 255) / 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
@@ -322,8 +317,8 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     for (int it = 0; it <= max_sweeps; ++it) {
         for (size_t r = 0; r < P.rounds.size(); ++r) {
-            const Arrays14& src = sets[r == 0 ? 0 : 1 + ((r - 1) & 1)];
-            const Arrays14& dst = sets[1 + (r & 1)];
+            const BRows& src = sets[r == 0 ? 0 : 1 + ((r - 1) & 1)];
+            const BRows& dst = sets[1 + (r & 1)];
             const PassEntry* tab = reinterpret_cast<const PassEntry*>(w + P.round_off[r]);
             const int ntab = (int)P.rounds[r].size();
             const unsigned nblk = (unsigned)P.round_blocks[r];
@@ -343,7 +338,7 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
                            max_sweeps, tol, sweeps_out);
         if (it < max_sweeps)
             hipLaunchKernelGGL(uresidual_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e,
-                               y0, y, sets[0].p[12], sets[0].p[13], stop, dsys, m, P.nbt);
+                               y0, y, sets[0].y, stop, dsys, m, P.nbt);
         if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     }
     return PCRP_OK;
