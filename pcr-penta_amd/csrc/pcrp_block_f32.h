@@ -163,11 +163,13 @@ __host__ __forceinline__ long long bblocks_for(int groups, int tiles)
 // One tile of one HBM pass: block rows l = tile*TI - H + lr (lr < TB) of the
 // current sub-system (this thread's slot g of G), k levels in LDS, interior
 // rows stored. in_at(l) / out_at(l): element of block row l in / out.
+// y_only: the output records already hold this pass's matrix part (the
+// matrix is the same in every refinement sweep), so only Y is stored.
 // Each thread owns RPT slots (rows lr0 + (TB/RPT)*kk of the same slot g).
 template <int TB, int G, int RPT, typename InAt, typename OutAt>
 __device__ __forceinline__ void bpass_tile(const BRows& in, const BRows& out, bool run,
-                                           int l0, int Lq, int k, int H, int TI, InAt in_at,
-                                           OutAt out_at, LdsRec<TB * G>& sR)
+                                           bool y_only, int l0, int Lq, int k, int H, int TI,
+                                           InAt in_at, OutAt out_at, LdsRec<TB * G>& sR)
 {
     constexpr int NTH = TB * G / RPT, RSTEP = TB / RPT;
     const int tid = threadIdx.x, lr0 = tid / G, g = tid - lr0 * G;
@@ -200,7 +202,10 @@ __device__ __forceinline__ void bpass_tile(const BRows& in, const BRows& out, bo
 #pragma unroll
     for (int kk = 0; kk < RPT; ++kk) {
         const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
-        if (valid[kk] && lr >= H && lr < H + TI) store_brow(out, out_at(l), r[kk]);
+        if (valid[kk] && lr >= H && lr < H + TI) {
+            if (y_only) out.y[out_at(l)] = make_float2(r[kk].y0, r[kk].y1);
+            else store_brow(out, out_at(l), r[kk]);
+        }
     }
 }
 

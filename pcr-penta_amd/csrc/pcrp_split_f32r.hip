@@ -17,7 +17,10 @@
 //   residual : r = y - A x in FP64 from the FP64 inputs, stored as the FP32
 //              block right-hand side (first sweep: r = y, x = 0);
 //   passes + stage (factor and solve together: recomputing the FP32 factor
-//              moves fewer bytes than storing its multipliers);
+//              moves fewer bytes than storing its multipliers; each pass
+//              has its own output set, so from the second sweep on a pass
+//              reads the matrix records sweep 0 left there and stores only
+//              the right-hand side);
 //   decide   : the stopping rule of the one-workgroup kernel: stop when
 //              max|dx| <= tol max|x|, or when the observed contraction
 //              rho = |dx_k| / |dx_{k-1}| (from the second correction on)
@@ -42,9 +45,9 @@ constexpr int kPassTB = 256, kPassG = 4, kPassRPT = 2;
 
 struct Layout {
     size_t state, stop, y0, lvl0, sets, bytes;
-    size_t per;        // elements per pass array
+    size_t per;        // block rows per pass output set
     long long nb0;     // block rows per system
-    int S;
+    int S, rounds;     // HBM levels, passes (one output set each)
 };
 
 constexpr size_t kAlign = 256;
@@ -63,8 +66,9 @@ Layout plan(int n, int m)
     L.state = o; o = align_up(o + (size_t)m * sizeof(SysState));
     L.stop = o;  o = align_up(o + (size_t)(m + 1) * sizeof(int));   // flags + running count
     L.y0 = o;    o = align_up(o + N * sizeof(double));
+    L.rounds = (L.S + kFuse - 1) / kFuse;
     L.lvl0 = o;  o = align_up(o + brows_bytes(NB));
-    L.sets = o;  o = align_up(o + 2 * brows_bytes(L.per));
+    L.sets = o;  o = align_up(o + (size_t)L.rounds * brows_bytes(L.per));
     L.bytes = o;
     return L;
 }
@@ -129,7 +133,7 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     BRows in, BRows out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
-    const int* __restrict__ stop)
+    const int* __restrict__ stop, int y_only)
 {
     __shared__ LdsRec<TB * G> sR;
     if (stop[m0] == 0) return;   // every system has stopped
@@ -145,7 +149,7 @@ __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     if (!__syncthreads_or(run)) return;   // every sub-system of this tile has converged
     const long long Qout = (long long)Qin << k;
     bpass_tile<TB, G, RPT>(
-        in, out, run, tile * TI - H + lr0, Lq, k, H, TI,
+        in, out, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI,
         [&](int l) { return (long long)q + (long long)l * Qin; },
         [&](int l) {
             return (long long)(l >> k) * Qout + (long long)(l & ((1 << k) - 1)) * Qin + q;
@@ -196,9 +200,11 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
     int* stop = reinterpret_cast<int*>(w + P.stop);
     double* y0 = reinterpret_cast<double*>(w + P.y0);
     const long long NB = P.nb0 * m0;
-    BRows lvl0, set0, set1;
+    BRows lvl0, sets[32];
+    if (P.rounds > 32) return -2;
     carve_brows(w + P.lvl0, (size_t)NB, lvl0);
-    carve_brows(carve_brows(w + P.sets, P.per, set0), P.per, set1);
+    char* p = w + P.sets;
+    for (int r = 0; r < P.rounds; ++r) p = carve_brows(p, P.per, sets[r]);
     const unsigned bblocks = (unsigned)((NB + 255) / 256);
     const unsigned sblocks = (unsigned)((m0 + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0);
@@ -208,8 +214,7 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
     for (int it = 0; it <= A.max_sweeps; ++it) {
         BRows in = lvl0;
         int Sin = 0, Qin = m0;
-        bool to0 = true;
-        while (Sin < S) {
+        for (int r = 0; Sin < S; ++r) {
             const int k = (S - Sin < kFuse) ? S - Sin : kFuse;
             const int H = (1 << k) - 1, TI = kPassTB - 2 * H;
             const long long L = (P.nb0 + (1LL << Sin) - 1) >> Sin;
@@ -217,14 +222,15 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
             const int groups = (Qin + kPassG - 1) / kPassG;
             const long long blocks = (long long)groups * ((tiles + 7) / 8) * 8;
             if (blocks > 0x7fffffffLL) return -2;
-            BRows out = to0 ? set0 : set1;
+            // from the second sweep on, pass r's output set already holds the
+            // matrix records of sweep 0: only the right-hand side is stored
+            const BRows out = sets[r];
             hipLaunchKernelGGL((bpass_kernel<kPassTB, kPassG, kPassRPT>), dim3((unsigned)blocks),
                                dim3(kPassTB * kPassG / kPassRPT), 0, st, in, out, P.nb0, m0, Sin,
-                               k, Qin, tiles, stop);
+                               k, Qin, tiles, stop, it > 0 ? 1 : 0);
             Sin += k;
             Qin <<= k;
             in = out;
-            to0 = !to0;
         }
         hipLaunchKernelGGL((bstage_kernel<kStageNT, kStageBR>), dim3((unsigned)Qin),
                            dim3(kStageNT), 0, st, in, Qin, S, P.nb0, n0, m0, A.rs, A.ss, A.y,

@@ -52,13 +52,18 @@ struct Job {
     int sys, R, S, nb, src, pad;
 };
 
+// Level-0 block rows (s[0]) and the output set of each pass round (s[r + 1]).
+constexpr int kMaxRounds = 7;   // 28 HBM levels: block rows up to 2^38
+struct BRowSets {
+    BRows s[kMaxRounds + 1];
+};
+
 struct Plan {
     std::vector<SysInfo> sys;
     std::vector<Job> jobs;
     std::vector<std::vector<PassEntry>> rounds;
     std::vector<long long> round_blocks;
     long long rows = 0, nbt = 0;
-    bool any_split = false;
     size_t off_state = 0, off_stop = 0, off_sys = 0, off_jobs = 0, off_y0 = 0, off_lvl0 = 0,
            off_sets = 0, bytes = 0;
     std::vector<size_t> round_off;
@@ -86,13 +91,13 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
         P.sys[j] = SysInfo{h_off[j], P.nbt, h_n[j], nb};
         P.nbt += nb;
         S[j] = levels_for(nb);
-        if (S[j] > 0) P.any_split = true;
     }
     if (P.rows > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
     // stage jobs, largest sub-systems first
     for (int j = 0; j < m; ++j) {
-        const int rounds = (S[j] + kFuse - 1) / kFuse;
-        const int src = rounds == 0 ? 0 : 1 + ((rounds - 1) & 1);
+        // the stage reads the output set of the system's last pass round
+        // (set r + 1 holds round r's output; set 0 the level-0 block rows)
+        const int src = (S[j] + kFuse - 1) / kFuse;
         for (int R = 0; R < (1 << S[j]); ++R) {
             const int nbs = (P.sys[j].nb - R + (1 << S[j]) - 1) >> S[j];
             if (nbs > 0) P.jobs.push_back(Job{j, R, S[j], nbs, src, 0});
@@ -117,6 +122,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
             blocks += (long long)groups * ((tiles + 7) / 8) * 8;   // XCD-striped tiles
         }
         if (tab.empty()) break;
+        if (r >= kMaxRounds) return PCRP_ERR_INVALID_SIZE;
         if (blocks > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
         P.rounds.push_back(std::move(tab));
         P.round_blocks.push_back(blocks);
@@ -133,7 +139,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
     P.off_y0 = o;    o = align_up(o + (size_t)P.rows * sizeof(double));
     P.off_lvl0 = o;  o = align_up(o + brows_bytes((size_t)P.nbt));
     P.off_sets = o;
-    if (P.any_split) o = align_up(o + 2 * brows_bytes((size_t)P.nbt));
+    o = align_up(o + P.rounds.size() * brows_bytes((size_t)P.nbt));
     P.bytes = o;
     return PCRP_OK;
 }
@@ -211,11 +217,12 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
 }
 
 // One pass round: tab lists the systems that still need levels; a block finds
-// its system by binary search on block0. sets: 0 = level 0, 1 and 2 = ping-pong.
+// its system by binary search on block0. Round r reads set r (0: level 0) and
+// writes set r + 1.
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
     BRows src, BRows dst, const PassEntry* __restrict__ tab, int ntab, int Sin,
-    const int* __restrict__ stop, int m)
+    const int* __restrict__ stop, int m, int y_only)
 {
     __shared__ LdsRec<TB * G> sR;
     if (stop[m] == 0) return;   // every system has stopped
@@ -237,11 +244,12 @@ __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
     const bool run = q < Qin && !stop[P.sys];
     if (!__syncthreads_or(run)) return;   // the system has converged
     auto at = [&](int l) { return P.boff + q + ((long long)l << Sin); };
-    bpass_tile<TB, G, RPT>(src, dst, run, tile * TI - H + lr0, Lq, k, H, TI, at, at, sR);
+    bpass_tile<TB, G, RPT>(src, dst, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI, at, at,
+                           sR);
 }
 
 template <int NT, int BR>
-__global__ __launch_bounds__(NT) void ustage_kernel(BRows s0, BRows s1, BRows s2,
+__global__ __launch_bounds__(NT) void ustage_kernel(BRowSets sets,
                                                     const Job* __restrict__ jobs,
                                                     const SysInfo* __restrict__ sys,
                                                     double* __restrict__ x,
@@ -257,7 +265,7 @@ __global__ __launch_bounds__(NT) void ustage_kernel(BRows s0, BRows s1, BRows s2
     const Job J = jobs[xcd_remap_c<32>(blockIdx.x, (int)gridDim.x)];
     if (stop[J.sys]) return;   // uniform over the workgroup
     const SysInfo S = sys[J.sys];
-    const BRows& in = J.src == 0 ? s0 : (J.src == 1 ? s1 : s2);
+    const BRows in = sets.s[J.src];
     bstage_solve<NT, BR>(
         in, J.nb, [&](int l) { return S.boff + J.R + ((long long)l << J.S); },
         [&](int l, int h) {
@@ -305,40 +313,43 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
     const SysInfo* dsys = reinterpret_cast<const SysInfo*>(w + P.off_sys);
     const Job* djobs = reinterpret_cast<const Job*>(w + P.off_jobs);
     double* y0 = reinterpret_cast<double*>(w + P.off_y0);
-    BRows sets[3] = {};
-    carve_brows(w + P.off_lvl0, (size_t)P.nbt, sets[0]);
-    if (P.any_split)
-        carve_brows(carve_brows(w + P.off_sets, (size_t)P.nbt, sets[1]), (size_t)P.nbt, sets[2]);
+    BRowSets sets = {};
+    carve_brows(w + P.off_lvl0, (size_t)P.nbt, sets.s[0]);
+    char* p = w + P.off_sets;
+    for (size_t r = 0; r < P.rounds.size(); ++r) p = carve_brows(p, (size_t)P.nbt, sets.s[r + 1]);
     const unsigned bblocks = (unsigned)((P.nbt + 255) / 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
-    hipLaunchKernelGGL(uprep_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e, y, sets[0],
+    hipLaunchKernelGGL(uprep_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e, y, sets.s[0],
                        y0, dsys, m, P.nbt);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     for (int it = 0; it <= max_sweeps; ++it) {
         for (size_t r = 0; r < P.rounds.size(); ++r) {
-            const BRows& src = sets[r == 0 ? 0 : 1 + ((r - 1) & 1)];
-            const BRows& dst = sets[1 + (r & 1)];
+            // from the second sweep on, round r's output set already holds the
+            // matrix records of sweep 0: only the right-hand side is stored
+            const BRows& src = sets.s[r];
+            const BRows& dst = sets.s[r + 1];
+            const int y_only = it > 0 ? 1 : 0;
             const PassEntry* tab = reinterpret_cast<const PassEntry*>(w + P.round_off[r]);
             const int ntab = (int)P.rounds[r].size();
             const unsigned nblk = (unsigned)P.round_blocks[r];
             if (r == 0)
                 hipLaunchKernelGGL((upass_kernel<kTB0, kG0, kRPT>), dim3(nblk),
                                    dim3(kTB0 * kG0 / kRPT), 0, st, src, dst, tab, ntab,
-                                   (int)r * kFuse, stop, m);
+                                   (int)r * kFuse, stop, m, y_only);
             else
                 hipLaunchKernelGGL((upass_kernel<kTB1, kG1, kRPT>), dim3(nblk),
                                    dim3(kTB1 * kG1 / kRPT), 0, st, src, dst, tab, ntab,
-                                   (int)r * kFuse, stop, m);
+                                   (int)r * kFuse, stop, m, y_only);
         }
         hipLaunchKernelGGL((ustage_kernel<kStageNT, kStageBR>), dim3((unsigned)P.jobs.size()),
-                           dim3(kStageNT), 0, st, sets[0], sets[1], sets[2], djobs, dsys, y, state,
+                           dim3(kStageNT), 0, st, sets, djobs, dsys, y, state,
                            stop, m, it == 0 ? 1 : 0);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
                            max_sweeps, tol, sweeps_out);
         if (it < max_sweeps)
             hipLaunchKernelGGL(uresidual_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e,
-                               y0, y, sets[0].y, stop, dsys, m, P.nbt);
+                               y0, y, sets.s[0].y, stop, dsys, m, P.nbt);
         if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     }
     return PCRP_OK;
