@@ -10,7 +10,8 @@
 //   passes : S levels in HBM, k_fuse levels per pass over tiles of TB block
 //            rows of G neighbouring sub-systems, halo 2^k - 1 block rows per
 //            side; output regrouped by residue as in the FP64 passes, so
-//            sub-system (R, j) of block rows R + 2^S l lands at q = R*m + j;
+//            sub-system (R, j) of block rows R + 2^S l is q = R*m + j; the
+//            last pass stores each sub-system's block rows consecutively;
 //   stage  : one workgroup per sub-system, the remaining levels in LDS, then
 //            X = D^-1 Y, and x += dx (FP64) at the caller's elements.
 // One refinement sweep (every kernel skips systems already stopped):
@@ -130,10 +131,13 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
 // Current sub-system q = R*m0 + j holds block rows R + 2^Sin l of system j,
 // element l at q + l*Qin; output regrouped by residue (q' = (l mod 2^k)*Qin
 // + q at l >> k).
+// The last pass writes sub-system-major instead (sub-system q' of block rows
+// l' at q' * Lst + l'), so the stage, one workgroup per sub-system, reads
+// consecutive records.
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     BRows in, BRows out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
-    const int* __restrict__ stop, int y_only)
+    const int* __restrict__ stop, int y_only, int last, long long Lst)
 {
     __shared__ LdsRec<TB * G> sR;
     if (stop[m0] == 0) return;   // every system has stopped
@@ -152,17 +156,20 @@ __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
         in, out, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI,
         [&](int l) { return (long long)q + (long long)l * Qin; },
         [&](int l) {
-            return (long long)(l >> k) * Qout + (long long)(l & ((1 << k) - 1)) * Qin + q;
+            const long long qo = (long long)(l & ((1 << k) - 1)) * Qin + q;
+            return last ? qo * Lst + (l >> k) : (long long)(l >> k) * Qout + qo;
         },
         sR);
 }
 
-// LDS stage: one workgroup per sub-system q = R*m0 + j (element l at q + l*Q),
-// the remaining levels, X = D^-1 Y, x += dx at rows 2K, 2K+1, K = R + 2^S l.
+// LDS stage: one workgroup per sub-system q = R*m0 + j (element l at
+// q * Lst + l), the remaining levels, X = D^-1 Y, x += dx at rows 2K, 2K+1,
+// K = R + 2^S l.
 template <int NT, int BR>
 __global__ __launch_bounds__(NT) void bstage_kernel(
-    BRows in, int Q, int S, long long nb0, int n0, int m0, long long xrs, long long xss,
-    double* __restrict__ x, SysState* __restrict__ st, const int* __restrict__ stop, int first)
+    BRows in, int Q, int S, long long Lst, long long nb0, int n0, int m0, long long xrs,
+    long long xss, double* __restrict__ x, SysState* __restrict__ st,
+    const int* __restrict__ stop, int first)
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
     if (stop[j]) return;   // uniform over the workgroup
     const int nb = (int)((nb0 - R + (1LL << S) - 1) >> S);
     bstage_solve<NT, BR>(
-        in, nb, [&](int l) { return (long long)q + (long long)l * Q; },
+        in, nb, [&](int l) { return (long long)q * Lst + l; },
         [&](int l, int h) {
             const long long i = 2 * (R + ((long long)l << S)) + h;
             return i < n0 ? (long long)j * xss + i * xrs : -1LL;
@@ -205,6 +212,8 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
     carve_brows(w + P.lvl0, (size_t)NB, lvl0);
     char* p = w + P.sets;
     for (int r = 0; r < P.rounds; ++r) p = carve_brows(p, P.per, sets[r]);
+    if (S < 1) return -2;   // the split path always runs at least one pass
+    const long long Lst = (P.nb0 + (1LL << S) - 1) >> S;   // block rows per stage sub-system
     const unsigned bblocks = (unsigned)((NB + 255) / 256);
     const unsigned sblocks = (unsigned)((m0 + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0);
@@ -227,13 +236,13 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
             const BRows out = sets[r];
             hipLaunchKernelGGL((bpass_kernel<kPassTB, kPassG, kPassRPT>), dim3((unsigned)blocks),
                                dim3(kPassTB * kPassG / kPassRPT), 0, st, in, out, P.nb0, m0, Sin,
-                               k, Qin, tiles, stop, it > 0 ? 1 : 0);
+                               k, Qin, tiles, stop, it > 0 ? 1 : 0, Sin + k == S ? 1 : 0, Lst);
             Sin += k;
             Qin <<= k;
             in = out;
         }
         hipLaunchKernelGGL((bstage_kernel<kStageNT, kStageBR>), dim3((unsigned)Qin),
-                           dim3(kStageNT), 0, st, in, Qin, S, P.nb0, n0, m0, A.rs, A.ss, A.y,
+                           dim3(kStageNT), 0, st, in, Qin, S, Lst, P.nb0, n0, m0, A.rs, A.ss, A.y,
                            state, stop, it == 0 ? 1 : 0);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0,
                            A.max_sweeps, A.tol, A.sweeps_out);
