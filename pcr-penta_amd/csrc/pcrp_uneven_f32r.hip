@@ -13,7 +13,11 @@
 //     position of block row l of sub-system q is q + l*2^Sin, the block row's
 //     own index, so passes read and write the same positions;
 //   * then every sub-system (or whole small system) is one job of the LDS
-//     stage, which solves it and adds dx to x in FP64.
+//     stage, which solves it and adds dx to x in FP64. A system's last pass
+//     round stores each of its stage sub-systems consecutively (sub-system
+//     R at pboff[j] + R * lst[j], lst[j] = ceil(nb / 2^S)), so a stage
+//     workgroup reads consecutive records; the pass output sets give every
+//     system 2^S * lst[j] >= nb positions from pboff[j] on.
 // One refinement sweep: residual (FP64, stored as the FP32 block right-hand
 // side), pass rounds, stage, per-system stopping decision; the stop flags
 // make later launches skip converged systems.
@@ -40,12 +44,13 @@ constexpr int kRPT = 2;
 
 struct SysInfo {
     long long off, boff;   // first row, first level-0 block row
-    int n, nb;
+    long long pboff;       // first position in the pass output sets
+    int n, nb, lst, pad;   // lst: block rows per stage sub-system (split systems)
 };
 
 struct PassEntry {
-    long long boff, block0;
-    int nb, k, tiles, sys;
+    long long ibase, obase, block0;   // input / output position of block row 0
+    int nb, k, tiles, sys, last, lst;
 };
 
 struct Job {
@@ -63,7 +68,7 @@ struct Plan {
     std::vector<Job> jobs;
     std::vector<std::vector<PassEntry>> rounds;
     std::vector<long long> round_blocks;
-    long long rows = 0, nbt = 0;
+    long long rows = 0, nbt = 0, nbt_pad = 0;
     size_t off_state = 0, off_stop = 0, off_sys = 0, off_jobs = 0, off_y0 = 0, off_lvl0 = 0,
            off_sets = 0, bytes = 0;
     std::vector<size_t> round_off;
@@ -88,9 +93,11 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
         const long long end = h_off[j] + (long long)h_n[j];
         if (end > P.rows) P.rows = end;
         const int nb = (h_n[j] + 1) / 2;
-        P.sys[j] = SysInfo{h_off[j], P.nbt, h_n[j], nb};
-        P.nbt += nb;
         S[j] = levels_for(nb);
+        const int lst = (nb + (1 << S[j]) - 1) >> S[j];
+        P.sys[j] = SysInfo{h_off[j], P.nbt, P.nbt_pad, h_n[j], nb, lst, 0};
+        P.nbt += nb;
+        P.nbt_pad += S[j] > 0 ? ((long long)lst << S[j]) : nb;
     }
     if (P.rows > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
     // stage jobs, largest sub-systems first
@@ -118,7 +125,9 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
             const int L = (P.sys[j].nb + (1 << Sin) - 1) >> Sin;
             const int tiles = (L + TI - 1) / TI;
             const int groups = ((1 << Sin) + G - 1) / G;
-            tab.push_back(PassEntry{P.sys[j].boff, blocks, P.sys[j].nb, k, tiles, j});
+            tab.push_back(PassEntry{r == 0 ? P.sys[j].boff : P.sys[j].pboff, P.sys[j].pboff,
+                                    blocks, P.sys[j].nb, k, tiles, j, Sin + k == S[j] ? 1 : 0,
+                                    P.sys[j].lst});
             blocks += (long long)groups * ((tiles + 7) / 8) * 8;   // XCD-striped tiles
         }
         if (tab.empty()) break;
@@ -139,7 +148,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, Plan& P)
     P.off_y0 = o;    o = align_up(o + (size_t)P.rows * sizeof(double));
     P.off_lvl0 = o;  o = align_up(o + brows_bytes((size_t)P.nbt));
     P.off_sets = o;
-    o = align_up(o + P.rounds.size() * brows_bytes((size_t)P.nbt));
+    o = align_up(o + P.rounds.size() * brows_bytes((size_t)P.nbt_pad));
     P.bytes = o;
     return PCRP_OK;
 }
@@ -243,9 +252,16 @@ __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
     const int Lq = q < Qin ? (P.nb - q + Qin - 1) >> Sin : 0;
     const bool run = q < Qin && !stop[P.sys];
     if (!__syncthreads_or(run)) return;   // the system has converged
-    auto at = [&](int l) { return P.boff + q + ((long long)l << Sin); };
-    bpass_tile<TB, G, RPT>(src, dst, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI, at, at,
-                           sR);
+    // block row l of sub-system q is the system's block row q + l*2^Sin; the
+    // last round regroups its output by the stage's sub-system
+    auto in_at = [&](int l) { return P.ibase + q + ((long long)l << Sin); };
+    auto out_at = [&](int l) {
+        if (!P.last) return P.obase + q + ((long long)l << Sin);
+        const long long qo = q + ((long long)(l & ((1 << k) - 1)) << Sin);
+        return P.obase + qo * P.lst + (l >> k);
+    };
+    bpass_tile<TB, G, RPT>(src, dst, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI, in_at,
+                           out_at, sR);
 }
 
 template <int NT, int BR>
@@ -267,7 +283,10 @@ __global__ __launch_bounds__(NT) void ustage_kernel(BRowSets sets,
     const SysInfo S = sys[J.sys];
     const BRows in = sets.s[J.src];
     bstage_solve<NT, BR>(
-        in, J.nb, [&](int l) { return S.boff + J.R + ((long long)l << J.S); },
+        in, J.nb,
+        [&](int l) {
+            return J.src == 0 ? S.boff + l : S.pboff + (long long)J.R * S.lst + l;
+        },
         [&](int l, int h) {
             const long long i = 2 * (J.R + ((long long)l << J.S)) + h;
             return i < S.n ? S.off + i : -1LL;
@@ -316,7 +335,8 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
     BRowSets sets = {};
     carve_brows(w + P.off_lvl0, (size_t)P.nbt, sets.s[0]);
     char* p = w + P.off_sets;
-    for (size_t r = 0; r < P.rounds.size(); ++r) p = carve_brows(p, (size_t)P.nbt, sets.s[r + 1]);
+    for (size_t r = 0; r < P.rounds.size(); ++r)
+        p = carve_brows(p, (size_t)P.nbt_pad, sets.s[r + 1]);
     const unsigned bblocks = (unsigned)((P.nbt + 255) / 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
