@@ -35,17 +35,18 @@ namespace pcrp {
 
 namespace {
 
-// Workgroup max of u and of v (wave64 shuffles, then across waves through
-// LDS, one barrier). red is rewritten only after the caller's next barrier.
+// Max of u and of v over the NT threads of one system (wave64 shuffles, then
+// across its waves through LDS, one workgroup barrier); t: the thread's index
+// within its system. red is rewritten only after the caller's next barrier.
 template <int NT>
-__device__ __forceinline__ void block_max2(double& u, double& v, double (&red)[2][NT / 64])
+__device__ __forceinline__ void block_max2(double& u, double& v, double (&red)[2][NT / 64], int t)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         u = fmax(u, __shfl_xor(u, off, 64));
         v = fmax(v, __shfl_xor(v, off, 64));
     }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = t & 63, wid = t >> 6;
     if (lane == 0) {
         red[0][wid] = u;
         red[1][wid] = v;
@@ -62,10 +63,18 @@ __device__ __forceinline__ void block_max2(double& u, double& v, double (&red)[2
 
 }  // namespace
 
-// NT threads, BR block rows per thread (block row k = t + NT*q), LMAX levels
-// (>= ceil(log2(NT*BR))).
-template <int NT, int BR, int LMAX>
-__global__ __launch_bounds__(NT) void lds_f32r_kernel(
+// NT threads per system, BR block rows per thread (block row k = t + NT*q),
+// LMAX levels (>= ceil(log2(NT*BR))), G systems per workgroup.
+// G > 1 (interleaved batches, m a multiple of G, 16-byte aligned arrays):
+// the workgroup solves the neighbouring systems j0 .. j0+G-1, whose values of
+// one row sit side by side, so it loads them with 16-byte accesses (one per
+// two systems) instead of one 8-byte access per system and row: the
+// interleaved layout's one-element-per-line gather is what bounds the load
+// phase. The G systems then run the same solve on their own LDS regions;
+// every barrier is workgroup-wide and a system that has converged idles
+// until the last one stops.
+template <int NT, int BR, int LMAX, int G>
+__global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
     int n, int m, long long rs, long long ss, int max_sweeps, double tol,
@@ -73,28 +82,63 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
 {
     constexpr int NB = NT * BR;        // block rows
     constexpr int NMAX = 2 * NB;       // rows
-    // exact FP64 matrix (residual); rows in parity-split order, row i at
+    static_assert(G == 1 || G % 2 == 0, "systems are loaded in pairs");
+    // exact FP64 matrix and y (residual); rows in parity-split order, row i at
     // lds_pos(i) = (i & 1) * NB + i / 2, so the lanes of a wave (consecutive
-    // block rows k, rows 2k + h) touch consecutive doubles: no bank conflicts
-    // a..e and y: the residuals read no global memory (re-reading y there
-    // cost 22% at n = 1024, waves waiting on the strided loads)
-    __shared__ double sA[6][NMAX];
+    // block rows k, rows 2k + h) touch consecutive doubles: no bank conflicts.
+    // The residuals read no global memory (re-reading y there cost 22% at
+    // n = 1024, waves waiting on the strided loads).
+    __shared__ double sA[G][6][NMAX];
     // factor: D^-1, L, U of every block row (one 16-byte vector each);
-    // sweeps: FP64 x in sF[0..1] and the workgroup-max scratch in sF[2]
-    // (aliases: n = 1024 takes exactly 80 KB, two workgroups per CU, and
-    // n = 2048 exactly the CU's 160 KB)
-    __shared__ float4 sF[3][NB];
-    __shared__ float2 sY[2][NB];       // right-hand side exchange, ping-pong
-    double* const sX = reinterpret_cast<double*>(&sF[0][0]);
-    static_assert(NMAX * sizeof(double) <= 2 * NB * sizeof(float4), "x fits in sF[0..1]");
-    static_assert(2 * (NT / 64) * sizeof(double) <= NB * sizeof(float4), "scratch fits in sF[2]");
-    auto& sRed = *reinterpret_cast<double (*)[2][NT / 64]>(&sF[2][0]);
+    // sweeps: FP64 x in sF[.][0..1] and the max-reduction scratch in sF[.][2]
+    // (aliases: one system of n = 1024 takes exactly 80 KB, of n = 2048 the
+    // CU's 160 KB)
+    __shared__ float4 sF[G][3][NB];
+    __shared__ float2 sY[G][2][NB];    // right-hand side exchange, ping-pong
+    static_assert(NMAX * sizeof(double) <= 2 * NB * sizeof(float4), "x fits in sF[.][0..1]");
+    static_assert(2 * (NT / 64) * sizeof(double) + sizeof(int) <= NB * sizeof(float4),
+                  "scratch and stop flag fit in sF[.][2]");
 
     const auto lds_pos = [](int i) { return (i & 1) * NB + (i >> 1); };
-    const int t = threadIdx.x;
-    const int sys = xcd_remap(blockIdx.x, m);
+    const int gs = G == 1 ? 0 : (int)threadIdx.x / NT;   // system slot
+    const int t = (int)threadIdx.x - gs * NT;
+    const int grp = xcd_remap(blockIdx.x, m / G);
+    const int sys = grp * G + gs;
     const long long base = (long long)sys * ss;
     const int nb = (n + 1) >> 1;
+    double(&A)[6][NMAX] = sA[gs];
+    float4(&F)[3][NB] = sF[gs];
+    float2(&Yx)[2][NB] = sY[gs];
+    double* const sX = reinterpret_cast<double*>(&F[0][0]);
+    auto& sRed = *reinterpret_cast<double (*)[2][NT / 64]>(&F[2][0]);
+
+    // out-of-range couplings are not part of the matrix
+    const auto band = [n](int f, int i, double v) {
+        if (f == 0 && i < 2) return 0.0;
+        if (f == 1 && i < 1) return 0.0;
+        if (f == 3 && i > n - 2) return 0.0;
+        if (f == 4 && i > n - 3) return 0.0;
+        return v;
+    };
+    if constexpr (G > 1) {
+        // rows 0..n-1 of the six arrays, G neighbouring values at i*rs + j0
+        const long long j0 = (long long)grp * G;
+        const double* const arr[6] = {ga, gb, gc, gd, ge, gy};
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            for (int i = (int)threadIdx.x; i < n; i += NT * G) {
+                const double2* src = reinterpret_cast<const double2*>(arr[f] + (long long)i * rs + j0);
+                const int p = lds_pos(i);
+#pragma unroll
+                for (int h = 0; h < G / 2; ++h) {
+                    const double2 w = src[h];
+                    sA[2 * h][f][p] = band(f, i, w.x);
+                    sA[2 * h + 1][f][p] = band(f, i, w.y);
+                }
+            }
+        }
+        __syncthreads();
+    }
 
     double X[BR][2];
     float v[BR][2];   // FP32 right-hand side of the next inner solve (first: y)
@@ -108,16 +152,16 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
             const int i = 2 * k + h;
             double a = 0.0, b = 0.0, c = 1.0, d = 0.0, e = 0.0, y = 0.0;
             if (i < n) {
-                const long long g = base + (long long)i * rs;
-                a = ga[g]; b = gb[g]; c = gc[g]; d = gd[g]; e = ge[g]; y = gy[g];
-                // out-of-range couplings are not part of the matrix
-                if (i < 2) a = 0.0;
-                if (i < 1) b = 0.0;
-                if (i > n - 2) d = 0.0;
-                if (i > n - 3) e = 0.0;
                 const int p = lds_pos(i);
-                sA[0][p] = a; sA[1][p] = b; sA[2][p] = c; sA[3][p] = d; sA[4][p] = e;
-                sA[5][p] = y;
+                if constexpr (G == 1) {
+                    const long long g = base + (long long)i * rs;
+                    a = band(0, i, ga[g]); b = band(1, i, gb[g]); c = gc[g];
+                    d = band(3, i, gd[g]); e = band(4, i, ge[g]); y = gy[g];
+                    A[0][p] = a; A[1][p] = b; A[2][p] = c; A[3][p] = d; A[4][p] = e;
+                    A[5][p] = y;
+                } else {
+                    a = A[0][p]; b = A[1][p]; c = A[2][p]; d = A[3][p]; e = A[4][p]; y = A[5][p];
+                }
             }
             rw[h][0] = a; rw[h][1] = b; rw[h][2] = c; rw[h][3] = d; rw[h][4] = e;
             X[q][h] = 0.0;
@@ -139,9 +183,9 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
 #pragma unroll
             for (int q = 0; q < BR; ++q) {
                 const int k = t + NT * q;
-                sF[0][k] = f4(minv(D[q]));
-                sF[1][k] = f4(L[q]);
-                sF[2][k] = f4(U[q]);
+                F[0][k] = f4(minv(D[q]));
+                F[1][k] = f4(L[q]);
+                F[2][k] = f4(U[q]);
             }
             __syncthreads();
 #pragma unroll
@@ -149,8 +193,8 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
                 const int k = t + NT * q;
                 const bool hl = k - delta >= 0, hr = k + delta < nb;
                 const int kl = hl ? k - delta : k, kr = hr ? k + delta : k;
-                const M2 DiL = m2(sF[0][kl]), LL = m2(sF[1][kl]), UL = m2(sF[2][kl]);
-                const M2 DiR = m2(sF[0][kr]), LR = m2(sF[1][kr]), UR = m2(sF[2][kr]);
+                const M2 DiL = m2(F[0][kl]), LL = m2(F[1][kl]), UL = m2(F[2][kl]);
+                const M2 DiR = m2(F[0][kr]), LR = m2(F[1][kr]), UR = m2(F[2][kr]);
                 const M2 al = hl ? mmul_neg(L[q], DiL) : kZero2;
                 const M2 be = hr ? mmul_neg(U[q], DiR) : kZero2;
                 D[q] = madd2(D[q], al, UL, be, LR);
@@ -169,95 +213,123 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
     // ---------------- solve + refinement sweeps ----------------
     int sweep = 0;
     double dprev = 0.0;
+    bool fin = false;   // this system has stopped (uniform over its threads)
     for (;;) {
 #pragma unroll
         for (int s = 0; s < LMAX; ++s) {
             if (s < levels) {
                 const int delta = 1 << s, p = s & 1;
+                if (!fin) {
 #pragma unroll
-                for (int q = 0; q < BR; ++q) {
-                    const int k = t + NT * q;
-                    sY[p][k] = make_float2(v[q][0], v[q][1]);
+                    for (int q = 0; q < BR; ++q) {
+                        const int k = t + NT * q;
+                        Yx[p][k] = make_float2(v[q][0], v[q][1]);
+                    }
                 }
                 __syncthreads();
+                if (!fin) {
 #pragma unroll
-                for (int q = 0; q < BR; ++q) {
-                    const int k = t + NT * q;
-                    const int kl = k - delta >= 0 ? k - delta : k;
-                    const int kr = k + delta < nb ? k + delta : k;
-                    const float2 yl = sY[p][kl], yr = sY[p][kr];
-                    const float l0 = yl.x, l1 = yl.y, r0 = yr.x, r1 = yr.y;
-                    const M2& al = cA[s][q];
-                    const M2& be = cB[s][q];
-                    v[q][0] = fmaf(be.p, r0, fmaf(be.q, r1, fmaf(al.p, l0, fmaf(al.q, l1, v[q][0]))));
-                    v[q][1] = fmaf(be.r, r0, fmaf(be.s, r1, fmaf(al.r, l0, fmaf(al.s, l1, v[q][1]))));
+                    for (int q = 0; q < BR; ++q) {
+                        const int k = t + NT * q;
+                        const int kl = k - delta >= 0 ? k - delta : k;
+                        const int kr = k + delta < nb ? k + delta : k;
+                        const float2 yl = Yx[p][kl], yr = Yx[p][kr];
+                        const float l0 = yl.x, l1 = yl.y, r0 = yr.x, r1 = yr.y;
+                        const M2& al = cA[s][q];
+                        const M2& be = cB[s][q];
+                        v[q][0] = fmaf(be.p, r0, fmaf(be.q, r1, fmaf(al.p, l0, fmaf(al.q, l1, v[q][0]))));
+                        v[q][1] = fmaf(be.r, r0, fmaf(be.s, r1, fmaf(al.r, l0, fmaf(al.s, l1, v[q][1]))));
+                    }
                 }
             }
         }
         double dmax = 0.0, xmax = 0.0;
+        if (!fin) {
 #pragma unroll
-        for (int q = 0; q < BR; ++q) {
-            const int k = t + NT * q;
-            const float dx0 = fmaf(Fin[q].p, v[q][0], Fin[q].q * v[q][1]);
-            const float dx1 = fmaf(Fin[q].r, v[q][0], Fin[q].s * v[q][1]);
+            for (int q = 0; q < BR; ++q) {
+                const int k = t + NT * q;
+                const float dx0 = fmaf(Fin[q].p, v[q][0], Fin[q].q * v[q][1]);
+                const float dx1 = fmaf(Fin[q].r, v[q][0], Fin[q].s * v[q][1]);
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int i = 2 * k + h;
-                if (i < n) {
-                    const double dx = (double)(h == 0 ? dx0 : dx1);
-                    X[q][h] += dx;
-                    // a NaN correction or iterate never passes the stopping
-                    // rule (fmax would drop it): the system is flagged
-                    const bool nan = !(dx == dx) || !(X[q][h] == X[q][h]);
-                    dmax = fmax(dmax, nan ? INFINITY : fabs(dx));
-                    xmax = fmax(xmax, fabs(X[q][h]));
+                for (int h = 0; h < 2; ++h) {
+                    const int i = 2 * k + h;
+                    if (i < n) {
+                        const double dx = (double)(h == 0 ? dx0 : dx1);
+                        X[q][h] += dx;
+                        // a NaN correction or iterate never passes the stopping
+                        // rule (fmax would drop it): the system is flagged
+                        const bool nan = !(dx == dx) || !(X[q][h] == X[q][h]);
+                        dmax = fmax(dmax, nan ? INFINITY : fabs(dx));
+                        xmax = fmax(xmax, fabs(X[q][h]));
+                    }
                 }
             }
         }
         // (the next write to sRed follows at least the residual's barrier)
-        block_max2<NT>(dmax, xmax, sRed);
-        // Stop when the correction is below tol, or when the error left after
-        // applying it, estimated from the observed contraction
-        // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k| with a safety factor of 10,
-        // is below tol (and rho <= 0.1). rho needs two refinement corrections:
-        // |dx_1| / |x_0| measures the first solve's error, not the contraction.
-        const bool small = dmax <= tol * xmax;
-        const bool contracting = sweep > 1 && dmax <= 0.1 * dprev &&
-                                 10.0 * dmax * dmax <= tol * xmax * dprev;
-        const bool done = small || contracting;
-        dprev = dmax;
-        if (done || sweep >= max_sweeps) {
-            if (!done) ++sweep;                   // flag: tolerance not met
-            break;
+        block_max2<NT>(dmax, xmax, sRed, t);
+        if (!fin) {
+            // Stop when the correction is below tol, or when the error left
+            // after applying it, estimated from the observed contraction
+            // rho = |dx_k| / |dx_{k-1}| as rho * |dx_k| with a safety factor
+            // of 10, is below tol (and rho <= 0.1). rho needs two refinement
+            // corrections: |dx_1| / |x_0| measures the first solve's error,
+            // not the contraction.
+            const bool small = dmax <= tol * xmax;
+            const bool contracting = sweep > 1 && dmax <= 0.1 * dprev &&
+                                     10.0 * dmax * dmax <= tol * xmax * dprev;
+            const bool done = small || contracting;
+            dprev = dmax;
+            if (done || sweep >= max_sweeps) {
+                if (!done) ++sweep;               // flag: tolerance not met
+                fin = true;
+            } else {
+                ++sweep;
+            }
         }
-        ++sweep;
+        if constexpr (G == 1) {
+            if (fin) break;
+        } else {
+            // every system of the workgroup stopped? Each system's flag sits in
+            // its own idle sF[.][2] words, after the max-reduction scratch.
+            constexpr int kFlag = 4 * (NT / 64);   // int index past sRed's doubles
+            if (t == 0) reinterpret_cast<int*>(&F[2][0])[kFlag] = fin ? 1 : 0;
+            __syncthreads();
+            bool all = true;
+#pragma unroll
+            for (int g = 0; g < G; ++g) all = all && reinterpret_cast<const int*>(&sF[g][2][0])[kFlag] != 0;
+            if (all) break;
+        }
         // r = y - A x, exact FP64 (y is overwritten only at the end)
+        if (!fin) {
 #pragma unroll
-        for (int q = 0; q < BR; ++q) {
-            const int k = t + NT * q;
+            for (int q = 0; q < BR; ++q) {
+                const int k = t + NT * q;
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
-                if (2 * k + h < n) sX[h * NB + k] = X[q][h];
+                for (int h = 0; h < 2; ++h)
+                    if (2 * k + h < n) sX[h * NB + k] = X[q][h];
+            }
         }
         __syncthreads();
+        if (!fin) {
 #pragma unroll
-        for (int q = 0; q < BR; ++q) {
-            const int k = t + NT * q;
+            for (int q = 0; q < BR; ++q) {
+                const int k = t + NT * q;
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int i = 2 * k + h;
-                if (i < n) {
-                    const double xm2 = i >= 2 ? sX[lds_pos(i - 2)] : 0.0;
-                    const double xm1 = i >= 1 ? sX[lds_pos(i - 1)] : 0.0;
-                    const double xp1 = i + 1 < n ? sX[lds_pos(i + 1)] : 0.0;
-                    const double xp2 = i + 2 < n ? sX[lds_pos(i + 2)] : 0.0;
-                    const int p = lds_pos(i);
-                    double ax = sA[2][p] * X[q][h];
-                    ax = __builtin_fma(sA[0][p], xm2, ax);
-                    ax = __builtin_fma(sA[1][p], xm1, ax);
-                    ax = __builtin_fma(sA[3][p], xp1, ax);
-                    ax = __builtin_fma(sA[4][p], xp2, ax);
-                    v[q][h] = (float)(sA[5][p] - ax);
+                for (int h = 0; h < 2; ++h) {
+                    const int i = 2 * k + h;
+                    if (i < n) {
+                        const double xm2 = i >= 2 ? sX[lds_pos(i - 2)] : 0.0;
+                        const double xm1 = i >= 1 ? sX[lds_pos(i - 1)] : 0.0;
+                        const double xp1 = i + 1 < n ? sX[lds_pos(i + 1)] : 0.0;
+                        const double xp2 = i + 2 < n ? sX[lds_pos(i + 2)] : 0.0;
+                        const int p = lds_pos(i);
+                        double ax = A[2][p] * X[q][h];
+                        ax = __builtin_fma(A[0][p], xm2, ax);
+                        ax = __builtin_fma(A[1][p], xm1, ax);
+                        ax = __builtin_fma(A[3][p], xp1, ax);
+                        ax = __builtin_fma(A[4][p], xp2, ax);
+                        v[q][h] = (float)(A[5][p] - ax);
+                    }
                 }
             }
         }
@@ -276,33 +348,62 @@ __global__ __launch_bounds__(NT) void lds_f32r_kernel(
     if (sweeps_out && t == 0) sweeps_out[sys] = sweep;
 }
 
-template <int NT, int BR>
+template <int NT, int BR, int G>
 static int launch_f32r(const SolveArgsF32R& A, hipStream_t st)
 {
     constexpr int LMAX = (NT * BR >= 1024) ? 10 : (NT * BR >= 512) ? 9 : (NT * BR >= 256) ? 8
                        : (NT * BR >= 128) ? 7 : 6;
     static_assert((1 << LMAX) >= NT * BR, "LMAX covers ceil(log2 block rows)");
-    hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX>), dim3(A.m), dim3(NT), 0, st, A.a, A.b,
-                       A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.max_sweeps, A.tol,
+    hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, G>), dim3(A.m / G), dim3(NT * G), 0, st,
+                       A.a, A.b, A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.max_sweeps, A.tol,
                        A.sweeps_out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int lds_f32r_max_n() { return 2048; }
 
+// Grouped loading needs an interleaved batch (rows m apart, systems adjacent)
+// whose size is a multiple of the group and six 16-byte aligned arrays.
+static bool groupable(const SolveArgsF32R& A, int G)
+{
+    const auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    return A.ss == 1 && A.rs == A.m && A.m % G == 0 && al(A.a) && al(A.b) && al(A.c) &&
+           al(A.d) && al(A.e) && al(A.y);
+}
+
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
 {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return -1;
+    }
     const int n = A.n;
-    if (n <= 128) return launch_f32r<64, 1>(A, st);
-    if (n <= 256) return launch_f32r<128, 1>(A, st);
+    if (n <= 128) return launch_f32r<64, 1, 1>(A, st);
+    if (n <= 256) return launch_f32r<128, 1, 1>(A, st);
     // From n = 257 on, two block rows per thread: the cached multipliers of
     // every level stay in registers without spilling (one block row per
     // thread at 512 threads needed a 128-register cap for two workgroups per
-    // CU and spilled: 11% slower at n = 1024), and at n = 512 the smaller
-    // workgroups are 8% faster on 1024 systems (equal on 16384).
-    if (n <= 512) return launch_f32r<128, 2>(A, st);
-    if (n <= 1024) return launch_f32r<256, 2>(A, st);
-    if (n <= 2048) return launch_f32r<512, 2>(A, st);
+    // CU and spilled: 11% slower at n = 1024).
+    // Pairs of neighbouring systems per workgroup on interleaved batches
+    // with more systems than CUs (so that one system per workgroup would put
+    // two or more workgroups on some CUs). Measured on MI355X against one
+    // system per workgroup: n <= 512 (80 KB per pair, two pairs per CU)
+    // 7.5% faster at m = 1024 and 2% at m = 16384, but 9% slower at m = 256
+    // (half the CUs idle); n <= 1024 7% faster at m = 300 and 3.5% at
+    // m = 512, but only up to m = 2 CUs: a 160 KB pair holds the whole CU,
+    // so in a longer batch no second workgroup's levels hide its loads (25%
+    // slower at m = 16384). Groups of 4..16 systems of n <= 256 measured
+    // slower at m = 16384.
+    if (n <= 512)
+        return groupable(A, 2) && A.m > cus ? launch_f32r<128, 2, 2>(A, st)
+                                            : launch_f32r<128, 2, 1>(A, st);
+    if (n <= 1024)
+        return groupable(A, 2) && A.m > cus && A.m <= 2 * cus ? launch_f32r<256, 2, 2>(A, st)
+                                                              : launch_f32r<256, 2, 1>(A, st);
+    if (n <= 2048) return launch_f32r<512, 2, 1>(A, st);
     return -2;
 }
 

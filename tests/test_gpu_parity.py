@@ -236,6 +236,36 @@ def test_refined_flags_nan_system(gpu, oracle_lib, n):
     assert np.abs(xs - rs).max() / np.abs(rs).max() <= 1e-12
 
 
+@pytest.mark.parametrize("n", [257, 300, 512, 600, 1000, 1024])
+def test_refined_grouped_load_matches_single(gpu, oracle_lib, n):
+    """Interleaved batches of an even number of systems, more than the GPU
+    has CUs, load pairs of neighbouring systems per workgroup with 16-byte
+    accesses (257 <= n <= 1024); the same systems in the contiguous layout go
+    one per workgroup. The arithmetic of a system does not depend on that,
+    so x and the sweep counts agree bit for bit, also when one system of a
+    pair is NaN and stops with a flag while its neighbour keeps iterating."""
+    import torch
+    group = 2
+    m = torch.cuda.get_device_properties(0).multi_processor_count + 2
+    _say("refined-grouped", n, m)
+    dev = gpu.generate(1, 40 + n, n, m, I)
+    dev[2].view(n, m)[n // 3, group + 1] = float("nan")   # c of system group + 1
+    host = [_np(t).copy() for t in dev]
+    xi, swi = _refined(gpu, dev, n, m, I, max_sweeps=4)
+    cont = [torch.from_numpy(np.ascontiguousarray(h.reshape(n, m).T).reshape(-1)).cuda()
+            for h in host]
+    xc, swc = _refined(gpu, cont, n, m, C, max_sweeps=4)
+    xc_i = xc.reshape(m, n).T.reshape(-1)
+    assert np.array_equal(swi, swc), (swi, swc)
+    assert swi[group + 1] == 5 and all(1 <= swi[j] <= 3 for j in range(m) if j != group + 1), swi
+    keep = [j for j in range(m) if j != group + 1]
+    xs = xi.reshape(n, m)[:, keep]
+    assert np.array_equal(xs.view(np.int64), xc_i.reshape(n, m)[:, keep].view(np.int64))
+    ref = oracle_lib.gbsv_batch(*host, n, m, I)
+    rs = ref.reshape(n, m)[:, keep]
+    assert np.abs(xs - rs).max() / np.abs(rs).max() <= 1e-12
+
+
 def test_refined_split_reports_nonconvergence(gpu, oracle_lib):
     """Split refinement path: max_sweeps = 0 gives the bare FP32 split solve,
     flagged with sweeps = 1, both layouts."""
