@@ -121,6 +121,13 @@ int lds_tuned_variant(int prec, int n)
     }
     return lds_f64_default_variant(n);
 }
+
+int lds_tuned_variant(int prec, int n, int m)
+{
+    const int b = bucket_of(n);
+    if (b < kBuckets && g_tuned[prec][b] >= 0) return lds_tuned_variant(prec, n);
+    return lds_f64_default_variant_for_batch(n, m);
+}
 }  // namespace pcrp
 
 extern "C" {
@@ -161,7 +168,7 @@ int pcrp_solve_f64(const double* a, const double* b, const double* c, const doub
     if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (split_f64_levels(n, split_f64_get().lds_max) == 0) {
-        const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, n));
+        const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, n, m));
         return v->launch(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
     }
     if (!ws || ws_bytes < split_f64_workspace(n, m)) return PCRP_ERR_WORKSPACE;

@@ -62,6 +62,7 @@ struct LdsVariantF64 {
 int lds_f64_variant_count();
 const LdsVariantF64* lds_f64_variant(int id);
 int lds_f64_default_variant(int n);
+int lds_f64_default_variant_for_batch(int n, int m);
 int lds_f64_lds_bytes(const LdsVariantF64& v);
 int lds_f64_max_n();
 
@@ -79,6 +80,8 @@ struct SolveArgsF32R {
 
 // Variant the tuning table assigns to an LDS solve of size n (pcrp_api.hip).
 int lds_tuned_variant(int prec, int n);
+// The same for a whole batch: without a tuned entry, the batch-size-aware default.
+int lds_tuned_variant(int prec, int n, int m);
 
 // Split path tunables: sub-systems up to lds_max rows go to the LDS kernel;
 // HBM passes fuse up to k_fuse steps; shape indexes the pass tile shapes.

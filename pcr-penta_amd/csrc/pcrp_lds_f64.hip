@@ -579,4 +579,26 @@ int lds_f64_default_variant(int n)
     return best;
 }
 
+// Default for a whole batch of m systems: at 512 < n <= 1024 a batch of at
+// most two workgroups per CU is latency-bound, and one row per thread
+// (1024 threads) finishes each system sooner (MI355X, n = 1024: 5-10% faster
+// up to m = 512, 2-10% slower from m = 1024 on than 256 x 4).
+int lds_f64_default_variant_for_batch(int n, int m)
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = -1;
+    }
+    if (n > 512 && n <= 1024 && cus > 0 && m <= 2 * cus) {
+        for (int id = 0; id < lds_f64_variant_count(); ++id) {
+            const LdsVariantF64& v = kVariants[id];
+            if (v.kind == KIND_REDUNDANT && v.nt == 1024 && v.rpt == 1) return id;
+        }
+    }
+    return lds_f64_default_variant(n);
+}
+
 }  // namespace pcrp
