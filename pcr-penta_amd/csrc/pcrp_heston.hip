@@ -36,17 +36,23 @@ namespace {
 struct Grid {
     int ns, nv;
     double hS, hv, smax, vmax, kappa, thv, sigma, rho, r, K, dt, th;
+    // reciprocals of the difference-quotient denominators (host-computed, so
+    // the stencil coefficients take multiplications, not FP64 divisions)
+    double r12hS2, r12hS, rhS2, r2hS, rhv2, r2hv, rhv, r4hShv;
 };
 
 __host__ __device__ inline double S_at(const Grid& g, int i) { return i * g.hS; }
 __host__ __device__ inline double v_at(const Grid& g, int j) { return j * g.hv; }
 
+// far-S boundary value Smax - K e^{-r tau} at time-to-maturity tau
+__device__ inline double far_s(const Grid& g, double tau) { return g.smax - g.K * exp(-g.r * tau); }
+
 // value of the grid function at node (i, j), i in 0..ns+1, j in 0..nv
-// (boundary nodes from the boundary conditions at time-to-maturity tau)
-__device__ inline double node(const Grid& g, const double* U, int i, int j, double tau)
+// (boundary nodes from the boundary conditions; bS = far_s(tau))
+__device__ inline double node(const Grid& g, const double* U, int i, int j, double bS)
 {
     if (i == 0) return 0.0;
-    if (i == g.ns + 1) return g.smax - g.K * exp(-g.r * tau);
+    if (i == g.ns + 1) return bS;
     if (j == g.nv) return S_at(g, i);
     return U[(long long)j * g.ns + (i - 1)];
 }
@@ -57,14 +63,14 @@ __host__ __device__ inline void a1_coef(const Grid& g, int i, int j, double w[5]
     const double S = S_at(g, i), v = v_at(g, j);
     const double D = 0.5 * v * S * S, C = g.r * S;
     if (i >= 2 && i <= g.ns - 1) {
-        const double d2 = D / (12.0 * g.hS * g.hS), d1 = C / (12.0 * g.hS);
+        const double d2 = D * g.r12hS2, d1 = C * g.r12hS;
         w[0] = -d2 + d1;
         w[1] = 16.0 * d2 - 8.0 * d1;
         w[2] = -30.0 * d2 - 0.5 * g.r;
         w[3] = 16.0 * d2 + 8.0 * d1;
         w[4] = -d2 - d1;
     } else {
-        const double d2 = D / (g.hS * g.hS), d1 = C / (2.0 * g.hS);
+        const double d2 = D * g.rhS2, d1 = C * g.r2hS;
         w[0] = 0.0;
         w[1] = d2 - d1;
         w[2] = -2.0 * d2 - 0.5 * g.r;
@@ -77,7 +83,7 @@ __host__ __device__ inline void a1_coef(const Grid& g, int i, int j, double w[5]
 __host__ __device__ inline void a2_coef(const Grid& g, int j, double w[5])
 {
     const double v = v_at(g, j);
-    const double diff = 0.5 * g.sigma * g.sigma * v / (g.hv * g.hv);
+    const double diff = 0.5 * g.sigma * g.sigma * v * g.rhv2;
     const double mu = g.kappa * (g.thv - v);
     w[0] = w[1] = w[3] = w[4] = 0.0;
     w[2] = -0.5 * g.r;
@@ -88,18 +94,18 @@ __host__ __device__ inline void a2_coef(const Grid& g, int j, double w[5])
     }
     if (mu >= 0.0) {
         if (j + 2 <= g.nv) {   // second-order forward
-            const double f = mu / (2.0 * g.hv);
+            const double f = mu * g.r2hv;
             w[2] += -3.0 * f; w[3] += 4.0 * f; w[4] += -f;
         } else {               // first-order forward next to v = vmax
-            const double f = mu / g.hv;
+            const double f = mu * g.rhv;
             w[2] += -f; w[3] += f;
         }
     } else {
         if (j >= 2) {          // second-order backward
-            const double f = mu / (2.0 * g.hv);
+            const double f = mu * g.r2hv;
             w[2] += 3.0 * f; w[1] += -4.0 * f; w[0] += f;
         } else {               // first-order backward at j = 1
-            const double f = mu / g.hv;
+            const double f = mu * g.rhv;
             w[2] += f; w[1] += -f;
         }
     }
@@ -142,61 +148,55 @@ __global__ void payoff_kernel(Grid g, double* U, int* step)
     U[idx] = s > 0.0 ? s : 0.0;
 }
 
-// A1 U (with boundary nodes at tau) at unknown (i, j)
-__device__ inline double apply_a1(const Grid& g, const double* U, int i, int j, double tau)
+// A1 U (with boundary nodes at tau) at unknown (i, j); w: a1_coef(i, j)
+__device__ inline double apply_a1(const Grid& g, const double* U, int i, int j, double bS,
+                                  const double (&w)[5])
 {
-    double w[5];
-    a1_coef(g, i, j, w);
-    double acc = w[2] * node(g, U, i, j, tau);
-    acc += w[1] * node(g, U, i - 1, j, tau);
-    acc += w[3] * node(g, U, i + 1, j, tau);
-    if (w[0] != 0.0) acc += w[0] * node(g, U, i - 2, j, tau);
-    if (w[4] != 0.0) acc += w[4] * node(g, U, i + 2, j, tau);
+    double acc = w[2] * node(g, U, i, j, bS);
+    acc += w[1] * node(g, U, i - 1, j, bS);
+    acc += w[3] * node(g, U, i + 1, j, bS);
+    if (w[0] != 0.0) acc += w[0] * node(g, U, i - 2, j, bS);
+    if (w[4] != 0.0) acc += w[4] * node(g, U, i + 2, j, bS);
     return acc;
 }
 
 // boundary part b1(tau) of A1 at (i, j): terms on boundary nodes only
-__device__ inline double bnd_a1(const Grid& g, int i, int j, double tau)
+__device__ inline double bnd_a1(const Grid& g, int i, int j, double bS, const double (&w)[5])
 {
-    double w[5];
-    a1_coef(g, i, j, w);
     double acc = 0.0;
     for (int o = -2; o <= 2; ++o) {
         const int ii = i + o;
         if (w[o + 2] != 0.0 && (ii == 0 || ii == g.ns + 1))
-            acc += w[o + 2] * node(g, nullptr, ii, j, tau);
+            acc += w[o + 2] * node(g, nullptr, ii, j, bS);
     }
     return acc;
 }
 
-__device__ inline double apply_a2(const Grid& g, const double* U, int i, int j, double tau)
+__device__ inline double apply_a2(const Grid& g, const double* U, int i, int j, double bS,
+                                  const double (&w)[5])
 {
-    double w[5];
-    a2_coef(g, j, w);
-    double acc = w[2] * node(g, U, i, j, tau);
+    double acc = w[2] * node(g, U, i, j, bS);
     for (int o = -2; o <= 2; ++o) {
         if (o == 0 || w[o + 2] == 0.0) continue;
-        acc += w[o + 2] * node(g, U, i, j + o, tau);
+        acc += w[o + 2] * node(g, U, i, j + o, bS);
     }
     return acc;
 }
 
-__device__ inline double bnd_a2(const Grid& g, int i, int j, double tau)
+__device__ inline double bnd_a2(const Grid& g, int i, int j, double bS, const double (&w)[5])
 {
-    double w[5];
-    a2_coef(g, j, w);
     double acc = 0.0;
     for (int o = 1; o <= 2; ++o)
-        if (w[o + 2] != 0.0 && j + o == g.nv) acc += w[o + 2] * node(g, nullptr, i, j + o, tau);
+        if (w[o + 2] != 0.0 && j + o == g.nv) acc += w[o + 2] * node(g, nullptr, i, j + o, bS);
     return acc;
 }
 
-__device__ inline double apply_a0(const Grid& g, const double* U, int i, int j, double tau)
+__device__ inline double apply_a0(const Grid& g, const double* U, int i, int j, double bS)
 {
     if (j == 0) return 0.0;   // v = 0
-    const double c = g.rho * g.sigma * v_at(g, j) * S_at(g, i) / (4.0 * g.hS * g.hv);
-    const double x = node(g, U, i + 1, j + 1, tau) - node(g, U, i + 1, j - 1, tau) -
-                     node(g, U, i - 1, j + 1, tau) + node(g, U, i - 1, j - 1, tau);
+    const double c = g.rho * g.sigma * v_at(g, j) * S_at(g, i) * g.r4hShv;
+    const double x = node(g, U, i + 1, j + 1, bS) - node(g, U, i + 1, j - 1, bS) -
+                     node(g, U, i - 1, j + 1, bS) + node(g, U, i - 1, j - 1, bS);
     return c * x;
 }
 
@@ -207,14 +207,18 @@ __global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, 
     if (idx >= (long long)g.ns * g.nv) return;
     const int j = (int)(idx / g.ns), i = (int)(idx % g.ns) + 1;
     const double tau0 = (*step) * g.dt, tau1 = tau0 + g.dt;
+    const double bS0 = far_s(g, tau0), bS1 = far_s(g, tau1);
     const double u = U[idx];
-    const double l0 = apply_a0(g, U, i, j, tau0);
-    const double l1 = apply_a1(g, U, i, j, tau0);
-    const double l2 = apply_a2(g, U, i, j, tau0);
+    double w1[5], w2[5];   // each stencil's coefficients once per node
+    a1_coef(g, i, j, w1);
+    a2_coef(g, j, w2);
+    const double l0 = apply_a0(g, U, i, j, bS0);
+    const double l1 = apply_a1(g, U, i, j, bS0, w1);
+    const double l2 = apply_a2(g, U, i, j, bS0, w2);
     const double y0 = u + g.dt * (l0 + l1 + l2);
     const double th = g.th * g.dt;
-    R1[idx] = y0 - th * l1 + th * bnd_a1(g, i, j, tau1);
-    W[idx] = -th * l2 + th * bnd_a2(g, i, j, tau1);
+    R1[idx] = y0 - th * l1 + th * bnd_a1(g, i, j, bS1, w1);
+    W[idx] = -th * l2 + th * bnd_a2(g, i, j, bS1, w2);
 }
 
 __global__ void combine_kernel(Grid g, const double* Y1, const double* W, double* X, int* step)
@@ -234,6 +238,14 @@ Grid make_grid(const pcrp_heston_params* p)
     g.hv = p->vmax / p->nv;
     g.kappa = p->kappa; g.thv = p->theta_v; g.sigma = p->sigma; g.rho = p->rho; g.r = p->r;
     g.K = p->strike; g.dt = p->dt; g.th = p->theta_scheme;
+    g.r12hS2 = 1.0 / (12.0 * g.hS * g.hS);
+    g.r12hS = 1.0 / (12.0 * g.hS);
+    g.rhS2 = 1.0 / (g.hS * g.hS);
+    g.r2hS = 1.0 / (2.0 * g.hS);
+    g.rhv2 = 1.0 / (g.hv * g.hv);
+    g.r2hv = 1.0 / (2.0 * g.hv);
+    g.rhv = 1.0 / g.hv;
+    g.r4hShv = 1.0 / (4.0 * g.hS * g.hv);
     return g;
 }
 
