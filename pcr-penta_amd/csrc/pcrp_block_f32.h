@@ -154,6 +154,35 @@ __device__ __forceinline__ bool btile_of(int b, int groups, int tiles, int& grp,
     return tile < tiles;
 }
 
+// Workgroups of `kernel` (block_threads threads) resident on the whole GPU at
+// once, rounded down to a multiple of 8 so that logical block b and
+// b + grid run on the same XCD; 0 if the runtime cannot say. The sweep
+// kernels walk their logical blocks in steps of the grid. Sweeps 0 .. 2 (x0
+// and the two corrections a dominant system needs) launch one workgroup per
+// logical block; later sweeps, which every system has usually left, launch
+// only this many, so such a launch costs one round of workgroups that read
+// the stop count and exit (the resident grid for every sweep measured 6-8%
+// slower on the active sweeps: no dynamic balancing of the tiles).
+constexpr int kFullGridSweeps = 3;
+template <typename K>
+__host__ inline int resident_grid(K kernel, int block_threads)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(kernel),
+                                                     block_threads, 0) != hipSuccess)
+        return 0;
+    const int g = cus * (per > 0 ? per : 1) / 8 * 8;
+    return g > 0 ? g : 8;
+}
+
+// Launch grid for `logical` blocks walked by at most `resident` workgroups.
+__host__ inline unsigned walk_grid(long long logical, int resident)
+{
+    return (unsigned)(resident > 0 && logical > resident ? resident : logical);
+}
+
 __host__ __forceinline__ long long bblocks_for(int groups, int tiles)
 {
     constexpr int NX = 8;

@@ -110,21 +110,22 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
                                 long long rs, long long ss, long long nb0)
 {
     if (stop[m] == 0) return;   // every system has stopped
-    const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (el >= nb0 * m) return;
-    const long long K = el / m;
-    const int j = (int)(el - K * m);
-    if (stop[j]) return;
-    float rv[2] = {0.f, 0.f};
+    const long long total = nb0 * m, step = (long long)gridDim.x * blockDim.x;
+    for (long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x; el < total; el += step) {
+        const long long K = el / m;
+        const int j = (int)(el - K * m);
+        if (stop[j]) continue;
+        float rv[2] = {0.f, 0.f};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const long long i = 2 * K + h;
-        if (i < n) {
-            const long long g = (long long)j * ss + i * rs;
-            rv[h] = (float)residual_row(a, b, c, d, e, y0, x, g, i, n, rs);
+        for (int h = 0; h < 2; ++h) {
+            const long long i = 2 * K + h;
+            if (i < n) {
+                const long long g = (long long)j * ss + i * rs;
+                rv[h] = (float)residual_row(a, b, c, d, e, y0, x, g, i, n, rs);
+            }
         }
+        r[el] = make_float2(rv[0], rv[1]);
     }
-    r[el] = make_float2(rv[0], rv[1]);
 }
 
 // One HBM pass: k levels over a tile of TB block rows of G sub-systems.
@@ -137,29 +138,32 @@ __global__ void residual_kernel(const double* __restrict__ a, const double* __re
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void bpass_kernel(
     BRows in, BRows out, long long nb0, int m0, int Sin, int k, int Qin, int tiles,
-    const int* __restrict__ stop, int y_only, int last, long long Lst)
+    long long nblocks, const int* __restrict__ stop, int y_only, int last, long long Lst)
 {
     __shared__ LdsRec<TB * G> sR;
     if (stop[m0] == 0) return;   // every system has stopped
     const int H = (1 << k) - 1;
     const int TI = TB - 2 * H;
-    int grp, tile;
-    if (!btile_of(blockIdx.x, (Qin + G - 1) / G, tiles, grp, tile)) return;
-    const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
-    const int q = grp * G + g;
-    const int R = q / m0;
-    const int Lq = (int)((nb0 - R + (1LL << Sin) - 1) >> Sin);
-    const bool run = q < Qin && !stop[q - R * m0];
-    if (!__syncthreads_or(run)) return;   // every sub-system of this tile has converged
-    const long long Qout = (long long)Qin << k;
-    bpass_tile<TB, G, RPT>(
-        in, out, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI,
-        [&](int l) { return (long long)q + (long long)l * Qin; },
-        [&](int l) {
-            const long long qo = (long long)(l & ((1 << k) - 1)) * Qin + q;
-            return last ? qo * Lst + (l >> k) : (long long)(l >> k) * Qout + qo;
-        },
-        sR);
+    // logical blocks b, b + grid, ... (grid a multiple of 8: same XCD)
+    for (long long b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        int grp, tile;
+        if (!btile_of((int)b, (Qin + G - 1) / G, tiles, grp, tile)) continue;
+        const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
+        const int q = grp * G + g;
+        const int R = q / m0;
+        const int Lq = (int)((nb0 - R + (1LL << Sin) - 1) >> Sin);
+        const bool run = q < Qin && !stop[q - R * m0];
+        if (!__syncthreads_or(run)) continue;   // every sub-system of this tile has converged
+        const long long Qout = (long long)Qin << k;
+        bpass_tile<TB, G, RPT>(
+            in, out, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI,
+            [&](int l) { return (long long)q + (long long)l * Qin; },
+            [&](int l) {
+                const long long qo = (long long)(l & ((1 << k) - 1)) * Qin + q;
+                return last ? qo * Lst + (l >> k) : (long long)(l >> k) * Qout + qo;
+            },
+            sR);
+    }
 }
 
 // LDS stage: one workgroup per sub-system q = R*m0 + j (element l at
@@ -174,17 +178,20 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
     if (stop[m0] == 0) return;   // every system has stopped
-    const int q = xcd_remap_c<32>(blockIdx.x, Q);   // 32 FP32 block values per line
-    const int R = q / m0, j = q - R * m0;
-    if (stop[j]) return;   // uniform over the workgroup
-    const int nb = (int)((nb0 - R + (1LL << S) - 1) >> S);
-    bstage_solve<NT, BR>(
-        in, nb, [&](int l) { return (long long)q * Lst + l; },
-        [&](int l, int h) {
-            const long long i = 2 * (R + ((long long)l << S)) + h;
-            return i < n0 ? (long long)j * xss + i * xrs : -1LL;
-        },
-        x, first != 0, &st[j], sR, red);
+    // logical blocks b, b + grid, ... (grid a multiple of 8: same XCD)
+    for (int b = blockIdx.x; b < Q; b += gridDim.x) {
+        const int q = xcd_remap_c<32>(b, Q);   // neighbouring sub-systems share x's lines
+        const int R = q / m0, j = q - R * m0;
+        if (stop[j]) continue;   // uniform over the workgroup
+        const int nb = (int)((nb0 - R + (1LL << S) - 1) >> S);
+        bstage_solve<NT, BR>(
+            in, nb, [&](int l) { return (long long)q * Lst + l; },
+            [&](int l, int h) {
+                const long long i = 2 * (R + ((long long)l << S)) + h;
+                return i < n0 ? (long long)j * xss + i * xrs : -1LL;
+            },
+            x, first != 0, &st[j], sR, red);
+    }
 }
 
 }  // namespace
@@ -213,6 +220,10 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
     char* p = w + P.sets;
     for (int r = 0; r < P.rounds; ++r) p = carve_brows(p, P.per, sets[r]);
     if (S < 1) return -2;   // the split path always runs at least one pass
+    constexpr int kPassThreads = kPassTB * kPassG / kPassRPT;
+    static const int gPass = resident_grid(bpass_kernel<kPassTB, kPassG, kPassRPT>, kPassThreads);
+    static const int gStage = resident_grid(bstage_kernel<kStageNT, kStageBR>, kStageNT);
+    static const int gRes = resident_grid(residual_kernel, 256);
     const long long Lst = (P.nb0 + (1LL << S) - 1) >> S;   // block rows per stage sub-system
     const unsigned bblocks = (unsigned)((NB + 255) / 256);
     const unsigned sblocks = (unsigned)((m0 + 255) / 256);
@@ -234,20 +245,25 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
             // from the second sweep on, pass r's output set already holds the
             // matrix records of sweep 0: only the right-hand side is stored
             const BRows out = sets[r];
-            hipLaunchKernelGGL((bpass_kernel<kPassTB, kPassG, kPassRPT>), dim3((unsigned)blocks),
-                               dim3(kPassTB * kPassG / kPassRPT), 0, st, in, out, P.nb0, m0, Sin,
-                               k, Qin, tiles, stop, it > 0 ? 1 : 0, Sin + k == S ? 1 : 0, Lst);
+            const unsigned grid = walk_grid(blocks, it < kFullGridSweeps ? 0 : gPass);
+            hipLaunchKernelGGL((bpass_kernel<kPassTB, kPassG, kPassRPT>), dim3(grid),
+                               dim3(kPassThreads), 0, st, in, out, P.nb0, m0, Sin, k, Qin, tiles,
+                               blocks, stop, it > 0 ? 1 : 0, Sin + k == S ? 1 : 0, Lst);
             Sin += k;
             Qin <<= k;
             in = out;
         }
-        hipLaunchKernelGGL((bstage_kernel<kStageNT, kStageBR>), dim3((unsigned)Qin),
-                           dim3(kStageNT), 0, st, in, Qin, S, Lst, P.nb0, n0, m0, A.rs, A.ss, A.y,
-                           state, stop, it == 0 ? 1 : 0);
+        const unsigned sgrid = walk_grid(Qin, it < kFullGridSweeps ? 0 : gStage);
+        hipLaunchKernelGGL((bstage_kernel<kStageNT, kStageBR>), dim3(sgrid), dim3(kStageNT), 0, st,
+                           in, Qin, S, Lst, P.nb0, n0, m0, A.rs, A.ss, A.y, state, stop,
+                           it == 0 ? 1 : 0);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m0,
                            A.max_sweeps, A.tol, A.sweeps_out);
         if (it < A.max_sweeps)
-            hipLaunchKernelGGL(residual_kernel, dim3(bblocks), dim3(256), 0, st, A.a, A.b, A.c,
+            hipLaunchKernelGGL(residual_kernel,
+                               dim3(walk_grid(bblocks, it + 1 < kFullGridSweeps ? 0 : gRes)),
+                               dim3(256), 0, st,
+                               A.a, A.b, A.c,
                                A.d, A.e, y0, A.y, lvl0.y, stop, n0, m0, A.rs,
                                A.ss, P.nb0);
         if (hipGetLastError() != hipSuccess) return -1;

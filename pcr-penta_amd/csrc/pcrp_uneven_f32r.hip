@@ -164,14 +164,13 @@ __device__ __forceinline__ int sys_of(const SysInfo* __restrict__ sys, int m, lo
     return lo;
 }
 
-// The same for the block rows of a one-dimensional grid: one binary search
-// per workgroup (on its first block row, uniform, so scalar loads), then a
+// The same for the block rows e of a workgroup's range starting at e0: one
+// binary search per workgroup (on e0, uniform, so scalar loads), then a
 // short walk: a workgroup spans few systems (each has at least one block row,
 // and the batches this serves have tens to thousands).
 __device__ __forceinline__ int sys_of_row(const SysInfo* __restrict__ sys, int m, long long nbt,
-                                          long long e)
+                                          long long e0, long long e)
 {
-    const long long e0 = (long long)blockIdx.x * blockDim.x;
     int j = sys_of(sys, m, e0 < nbt ? e0 : nbt - 1);
     while (j + 1 < m && sys[j + 1].boff <= e) ++j;
     return j;
@@ -185,7 +184,7 @@ __global__ void uprep_kernel(const double* __restrict__ a, const double* __restr
 {
     const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (el >= nbt) return;
-    const SysInfo S = sys[sys_of_row(sys, m, nbt, el)];
+    const SysInfo S = sys[sys_of_row(sys, m, nbt, (long long)blockIdx.x * blockDim.x, el)];
     const long long K = el - S.boff;
     double rw[2][6];
 #pragma unroll
@@ -210,19 +209,22 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
                                  const SysInfo* __restrict__ sys, int m, long long nbt)
 {
     if (stop[m] == 0) return;   // every system has stopped
-    const long long el = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (el >= nbt) return;
-    const int j = sys_of_row(sys, m, nbt, el);
-    if (stop[j]) return;
-    const SysInfo S = sys[j];
-    const long long K = el - S.boff;
-    float rv[2] = {0.f, 0.f};
+    for (long long e0 = (long long)blockIdx.x * blockDim.x; e0 < nbt;
+         e0 += (long long)gridDim.x * blockDim.x) {
+        const long long el = e0 + threadIdx.x;
+        if (el >= nbt) continue;
+        const int j = sys_of_row(sys, m, nbt, e0, el);
+        if (stop[j]) continue;
+        const SysInfo S = sys[j];
+        const long long K = el - S.boff;
+        float rv[2] = {0.f, 0.f};
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const long long i = 2 * K + h;
-        if (i < S.n) rv[h] = (float)residual_row(a, b, c, d, e, y0, x, S.off + i, i, S.n, 1);
+        for (int h = 0; h < 2; ++h) {
+            const long long i = 2 * K + h;
+            if (i < S.n) rv[h] = (float)residual_row(a, b, c, d, e, y0, x, S.off + i, i, S.n, 1);
+        }
+        r[el] = make_float2(rv[0], rv[1]);
     }
-    r[el] = make_float2(rv[0], rv[1]);
 }
 
 // One pass round: tab lists the systems that still need levels; a block finds
@@ -231,37 +233,39 @@ __global__ void uresidual_kernel(const double* __restrict__ a, const double* __r
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void upass_kernel(
     BRows src, BRows dst, const PassEntry* __restrict__ tab, int ntab, int Sin,
-    const int* __restrict__ stop, int m, int y_only)
+    long long nblocks, const int* __restrict__ stop, int m, int y_only)
 {
     __shared__ LdsRec<TB * G> sR;
     if (stop[m] == 0) return;   // every system has stopped
-    const long long b = blockIdx.x;
-    int lo = 0, hi = ntab - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    // logical blocks b, b + grid, ... (grid a multiple of 8: same XCD)
+    for (long long b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        int lo = 0, hi = ntab - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tab[mid].block0 <= b) lo = mid; else hi = mid - 1;
+        }
+        const PassEntry P = tab[lo];
+        const int Qin = 1 << Sin;
+        int grp, tile;
+        if (!btile_of((int)(b - P.block0), (Qin + G - 1) / G, P.tiles, grp, tile)) continue;
+        const int k = P.k;
+        const int H = (1 << k) - 1, TI = TB - 2 * H;
+        const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
+        const int q = grp * G + g;
+        const int Lq = q < Qin ? (P.nb - q + Qin - 1) >> Sin : 0;
+        const bool run = q < Qin && !stop[P.sys];
+        if (!__syncthreads_or(run)) continue;   // the system has converged
+        // block row l of sub-system q is the system's block row q + l*2^Sin;
+        // the last round regroups its output by the stage's sub-system
+        auto in_at = [&](int l) { return P.ibase + q + ((long long)l << Sin); };
+        auto out_at = [&](int l) {
+            if (!P.last) return P.obase + q + ((long long)l << Sin);
+            const long long qo = q + ((long long)(l & ((1 << k) - 1)) << Sin);
+            return P.obase + qo * P.lst + (l >> k);
+        };
+        bpass_tile<TB, G, RPT>(src, dst, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI,
+                               in_at, out_at, sR);
     }
-    const PassEntry P = tab[lo];
-    const int Qin = 1 << Sin;
-    int grp, tile;
-    if (!btile_of((int)(b - P.block0), (Qin + G - 1) / G, P.tiles, grp, tile)) return;
-    const int k = P.k;
-    const int H = (1 << k) - 1, TI = TB - 2 * H;
-    const int lr0 = threadIdx.x / G, g = threadIdx.x - lr0 * G;
-    const int q = grp * G + g;
-    const int Lq = q < Qin ? (P.nb - q + Qin - 1) >> Sin : 0;
-    const bool run = q < Qin && !stop[P.sys];
-    if (!__syncthreads_or(run)) return;   // the system has converged
-    // block row l of sub-system q is the system's block row q + l*2^Sin; the
-    // last round regroups its output by the stage's sub-system
-    auto in_at = [&](int l) { return P.ibase + q + ((long long)l << Sin); };
-    auto out_at = [&](int l) {
-        if (!P.last) return P.obase + q + ((long long)l << Sin);
-        const long long qo = q + ((long long)(l & ((1 << k) - 1)) << Sin);
-        return P.obase + qo * P.lst + (l >> k);
-    };
-    bpass_tile<TB, G, RPT>(src, dst, run, y_only != 0, tile * TI - H + lr0, Lq, k, H, TI, in_at,
-                           out_at, sR);
 }
 
 template <int NT, int BR>
@@ -271,27 +275,30 @@ __global__ __launch_bounds__(NT) void ustage_kernel(BRowSets sets,
                                                     double* __restrict__ x,
                                                     SysState* __restrict__ st,
                                                     const int* __restrict__ stop, int m,
-                                                    int first)
+                                                    int njobs, int first)
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
     if (stop[m] == 0) return;   // every system has stopped
-    // consecutive jobs are neighbouring residues of one system (their block
-    // rows share cache lines): keep runs of 32 of them on one XCD
-    const Job J = jobs[xcd_remap_c<32>(blockIdx.x, (int)gridDim.x)];
-    if (stop[J.sys]) return;   // uniform over the workgroup
-    const SysInfo S = sys[J.sys];
-    const BRows in = sets.s[J.src];
-    bstage_solve<NT, BR>(
-        in, J.nb,
-        [&](int l) {
-            return J.src == 0 ? S.boff + l : S.pboff + (long long)J.R * S.lst + l;
-        },
-        [&](int l, int h) {
-            const long long i = 2 * (J.R + ((long long)l << J.S)) + h;
-            return i < S.n ? S.off + i : -1LL;
-        },
-        x, first != 0, &st[J.sys], sR, red);
+    // logical blocks b, b + grid, ... (grid a multiple of 8: same XCD)
+    for (int b = blockIdx.x; b < njobs; b += gridDim.x) {
+        // consecutive jobs are neighbouring residues of one system (their
+        // block rows share cache lines): keep runs of 32 of them on one XCD
+        const Job J = jobs[xcd_remap_c<32>(b, njobs)];
+        if (stop[J.sys]) continue;   // uniform over the workgroup
+        const SysInfo S = sys[J.sys];
+        const BRows in = sets.s[J.src];
+        bstage_solve<NT, BR>(
+            in, J.nb,
+            [&](int l) {
+                return J.src == 0 ? S.boff + l : S.pboff + (long long)J.R * S.lst + l;
+            },
+            [&](int l, int h) {
+                const long long i = 2 * (J.R + ((long long)l << J.S)) + h;
+                return i < S.n ? S.off + i : -1LL;
+            },
+            x, first != 0, &st[J.sys], sR, red);
+    }
 }
 
 }  // namespace
@@ -339,6 +346,10 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
         p = carve_brows(p, (size_t)P.nbt_pad, sets.s[r + 1]);
     const unsigned bblocks = (unsigned)((P.nbt + 255) / 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
+    static const int gPass0 = resident_grid(upass_kernel<kTB0, kG0, kRPT>, kTB0 * kG0 / kRPT);
+    static const int gPass1 = resident_grid(upass_kernel<kTB1, kG1, kRPT>, kTB1 * kG1 / kRPT);
+    static const int gStage = resident_grid(ustage_kernel<kStageNT, kStageBR>, kStageNT);
+    static const int gRes = resident_grid(uresidual_kernel, 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
     hipLaunchKernelGGL(uprep_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e, y, sets.s[0],
                        y0, dsys, m, P.nbt);
@@ -353,23 +364,31 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
             const PassEntry* tab = reinterpret_cast<const PassEntry*>(w + P.round_off[r]);
             const int ntab = (int)P.rounds[r].size();
             const unsigned nblk = (unsigned)P.round_blocks[r];
+            const int full = it < kFullGridSweeps;
             if (r == 0)
-                hipLaunchKernelGGL((upass_kernel<kTB0, kG0, kRPT>), dim3(nblk),
+                hipLaunchKernelGGL((upass_kernel<kTB0, kG0, kRPT>),
+                                   dim3(walk_grid(nblk, full ? 0 : gPass0)),
                                    dim3(kTB0 * kG0 / kRPT), 0, st, src, dst, tab, ntab,
-                                   (int)r * kFuse, stop, m, y_only);
+                                   (int)r * kFuse, (long long)nblk, stop, m, y_only);
             else
-                hipLaunchKernelGGL((upass_kernel<kTB1, kG1, kRPT>), dim3(nblk),
+                hipLaunchKernelGGL((upass_kernel<kTB1, kG1, kRPT>),
+                                   dim3(walk_grid(nblk, full ? 0 : gPass1)),
                                    dim3(kTB1 * kG1 / kRPT), 0, st, src, dst, tab, ntab,
-                                   (int)r * kFuse, stop, m, y_only);
+                                   (int)r * kFuse, (long long)nblk, stop, m, y_only);
         }
-        hipLaunchKernelGGL((ustage_kernel<kStageNT, kStageBR>), dim3((unsigned)P.jobs.size()),
-                           dim3(kStageNT), 0, st, sets, djobs, dsys, y, state,
-                           stop, m, it == 0 ? 1 : 0);
+        hipLaunchKernelGGL((ustage_kernel<kStageNT, kStageBR>),
+                           dim3(walk_grid((long long)P.jobs.size(),
+                                          it < kFullGridSweeps ? 0 : gStage)),
+                           dim3(kStageNT), 0,
+                           st, sets, djobs, dsys, y, state, stop, m, (int)P.jobs.size(),
+                           it == 0 ? 1 : 0);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
                            max_sweeps, tol, sweeps_out);
         if (it < max_sweeps)
-            hipLaunchKernelGGL(uresidual_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e,
-                               y0, y, sets.s[0].y, stop, dsys, m, P.nbt);
+            hipLaunchKernelGGL(uresidual_kernel,
+                               dim3(walk_grid(bblocks, it + 1 < kFullGridSweeps ? 0 : gRes)),
+                               dim3(256), 0, st,
+                               a, b, c, d, e, y0, y, sets.s[0].y, stop, dsys, m, P.nbt);
         if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     }
     return PCRP_OK;
