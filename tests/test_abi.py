@@ -137,6 +137,7 @@ def test_uneven_refined_argument_checks(lib):
     assert f(*p, off.ctypes.data, n.ctypes.data, 2, 4, 1e-12, None, _fake(1)[0], need - 1,
              None) == 4
     big = np.array([5000], dtype=np.int32)
-    # y copy (8 B/row) + level-0 and two ping-pong sets of 14 FP32 words per block row
+    # y copy (8 B/row) + level-0 block rows and the one pass round's output set
+    # (n = 5000: 2500 block rows, 2 HBM levels), 56 B per block row each
     assert lib.pcrp_uneven_workspace_size(off.ctypes.data, big.ctypes.data, 1, 1) >= \
-        8 * 5000 + 3 * 14 * 4 * 2500
+        8 * 5000 + 2 * 56 * 2500
