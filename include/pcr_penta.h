@@ -89,11 +89,15 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
  * 2k+1 as one block row, diagonal-block pivots), which keeps FP32 error near
  * 3e-7 where PCR-Penta's own off-diagonal pivots lose up to 1e-2.
  * Sizes n <= 2048 run in one LDS-resident kernel per system (the FP32
- * factorisation is computed once and its multipliers kept in registers).
+ * factorisation is computed once and its multipliers kept in registers;
+ * interleaved batches with more systems than CUs may take two neighbouring
+ * systems per workgroup, with the same results).
  * Larger n need a workspace (pcrp_workspace_size with PCRP_PREC_F32_REFINE):
  * every sweep re-runs the FP32 block solve split like pcrp_solve_f64 (HBM
- * passes, then an LDS stage per residue sub-system) on the FP32 residual,
- * with per-system stop flags so converged systems drop out of later sweeps.
+ * passes, then an LDS stage per residue sub-system) on the FP32 residual
+ * (from the second sweep on the passes reuse the matrix records the first
+ * one left in the workspace), with per-system stop flags so converged
+ * systems drop out of later sweeps.
  * A block pivot with determinant below 2^-100 gets a zero inverse (the
  * refinement corrects the difference, or flags the system).
  */
