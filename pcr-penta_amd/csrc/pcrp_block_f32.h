@@ -85,16 +85,18 @@ __device__ __forceinline__ void put_rec(LdsRec<N>& s, int i, const M2& Di, const
 }
 
 // One level for block row r from the records of its neighbours il / ir
-// (hl / hr: the neighbour exists in the sub-system).
+// (hl / hr: the neighbour exists in the sub-system); am: running maximum of
+// the level's multiplier entries.
 template <int N>
 __device__ __forceinline__ BRow level(const BRow& r, const LdsRec<N>& s, bool hl, int il, bool hr,
-                                      int ir)
+                                      int ir, float& am)
 {
     const M2 DiL = m2(s.Di[il]), LL = m2(s.L[il]), UL = m2(s.U[il]);
     const M2 DiR = m2(s.Di[ir]), LR = m2(s.L[ir]), UR = m2(s.U[ir]);
     const float2 yl = s.Y[il], yr = s.Y[ir];
     const M2 al = hl ? mmul_neg(r.L, DiL) : kZero2;
     const M2 be = hr ? mmul_neg(r.U, DiR) : kZero2;
+    am = fmaxf(am, fmaxf(maxabs2(al), maxabs2(be)));
     BRow o;
     o.D = madd2(r.D, al, UL, be, LR);
     o.L = mmul(al, LL);
@@ -223,7 +225,8 @@ __device__ __forceinline__ void bpass_tile(const BRows& in, const BRows& out, bo
                 // neighbours outside the tile only feed block rows that are not kept
                 const int il = (hl && lr - delta >= 0) ? (lr - delta) * G + g : self;
                 const int ir = (hr && lr + delta < TB) ? (lr + delta) * G + g : self;
-                r[kk] = level(r[kk], sR, hl, il, hr, ir);
+                float unused = 0.f;   // passes run every level the host launched
+                r[kk] = level(r[kk], sR, hl, il, hr, ir, unused);
             }
         }
         __syncthreads();
@@ -246,7 +249,8 @@ template <int NT, int BR, typename InAt, typename XAt>
 __device__ __forceinline__ void bstage_solve(const BRows& in, int nb, InAt in_at, XAt x_at,
                                              double* __restrict__ x, bool first, SysState* st,
                                              LdsRec<NT * BR>& sR,
-                                             unsigned long long (&red)[2][NT / 64])
+                                             unsigned long long (&red)[2][NT / 64],
+                                             int (&amx)[NT / 64])
 {
     const int t = threadIdx.x;
     BRow r[BR];
@@ -255,17 +259,25 @@ __device__ __forceinline__ void bstage_solve(const BRows& in, int nb, InAt in_at
         const int l = t + NT * b;
         r[b] = l < nb ? load_brow(in, in_at(l)) : identity_brow();
     }
+    // truncated once a level's multipliers are all below kTruncF32
     for (int delta = 1; delta < nb; delta <<= 1) {
 #pragma unroll
         for (int b = 0; b < BR; ++b) put_rec(sR, t + NT * b, minv(r[b].D), r[b]);
         __syncthreads();
+        float am = 0.f;
 #pragma unroll
         for (int b = 0; b < BR; ++b) {
             const int l = t + NT * b;
             const bool hl = l - delta >= 0, hr = l + delta < nb;
-            r[b] = level(r[b], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l);
+            r[b] = level(r[b], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l, am);
         }
+        const int big = __any(am > kTruncF32);   // one flag per wave
+        if ((t & 63) == 0) amx[t >> 6] = big;
         __syncthreads();
+        int any = 0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) any |= amx[w];
+        if (!any) break;   // uniform: the remaining levels are negligible
     }
     unsigned long long dmax = 0ull, xmax = 0ull;
 #pragma unroll
@@ -325,7 +337,7 @@ __global__ void init_state_kernel(SysState* __restrict__ st, int* __restrict__ s
 // max|dx| <= tol max|x|, or when the observed contraction rho = |dx_k| /
 // |dx_{k-1}| (from the second correction on) puts the error left, 10 rho
 // |dx_k|, below tol with rho <= 0.1. Not done after max_sweeps corrections,
-// or a NaN correction: sweeps = max_sweeps + 1.
+// or a NaN or infinite correction: sweeps = max_sweeps + 1.
 __global__ void decide_kernel(SysState* __restrict__ st, int* __restrict__ stop, int m,
                               int max_sweeps, double tol, int* __restrict__ sweeps_out)
 {
@@ -334,7 +346,7 @@ __global__ void decide_kernel(SysState* __restrict__ st, int* __restrict__ stop,
     SysState s = st[j];
     const double dmax = __longlong_as_double((long long)s.dmax);
     const double xmax = __longlong_as_double((long long)s.xmax);
-    const bool bad = dmax != dmax || xmax != xmax;
+    const bool bad = !(dmax < INFINITY) || !(xmax < INFINITY);   // NaN or overflow
     const bool small = !(dmax > tol * xmax);
     const bool contracting = s.sweep > 1 && dmax <= 0.1 * s.dprev &&
                              10.0 * dmax * dmax <= tol * xmax * s.dprev;

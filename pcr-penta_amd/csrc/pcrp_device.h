@@ -131,6 +131,22 @@ __device__ __forceinline__ M2 madd2(const M2& d, const M2& x, const M2& y, const
               fmaf(z.r, w.q, fmaf(z.s, w.s, fmaf(x.r, y.q, fmaf(x.s, y.s, d.s))))};
 }
 
+__device__ __forceinline__ float maxabs2(const M2& m)
+{
+    return fmaxf(fmaxf(fabsf(m.p), fabsf(m.q)), fmaxf(fabsf(m.r), fabsf(m.s)));
+}
+
+// Truncated (incomplete) cyclic reduction for the FP32 inner solve: once
+// every multiplier entry of a level (alpha = -L D^-1, beta = -U D^-1) is at
+// most this, the couplings left after it are at most that times the ones
+// before, and the next level's multipliers about its square (quadratic
+// decay of the off-diagonal blocks of block cyclic reduction on diagonally
+// dominant matrices, Heller, SIAM J. Numer. Anal. 13 (1976) 484-496): the
+// remaining levels would change X by ~2^-40 relative, far below the FP32
+// solve's own ~3e-7, so X = D^-1 Y right after that level. Systems whose
+// couplings do not decay run every level.
+constexpr float kTruncF32 = 0x1p-20f;
+
 __device__ __forceinline__ float4 f4(const M2& m) { return make_float4(m.p, m.q, m.r, m.s); }
 __device__ __forceinline__ M2 m2(const float4& v) { return M2{v.x, v.y, v.z, v.w}; }
 

@@ -173,12 +173,15 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
     }
 
     // ---------------- factor: two barriers per level ----------------
+    // Truncated once a level's multipliers are all below kTruncF32 (every
+    // system of the workgroup); the sweeps then run the same levels.
     M2 cA[LMAX][BR], cB[LMAX][BR];     // alpha, beta per level
     int levels = 0;
     while ((1 << levels) < nb) ++levels;
 #pragma unroll
     for (int s = 0; s < LMAX; ++s) {
         if (s < levels) {
+            float am = 0.f;   // largest multiplier entry of this thread's block rows
             const int delta = 1 << s;
 #pragma unroll
             for (int q = 0; q < BR; ++q) {
@@ -202,8 +205,20 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
                 U[q] = mmul(be, UR);
                 cA[s][q] = al;
                 cB[s][q] = be;
+                am = fmaxf(am, fmaxf(maxabs2(al), maxabs2(be)));
             }
+            // one flag per wave (a ballot, no shuffles) in sY[.][1], idle
+            // until the sweeps
+            const int big = __any(am > kTruncF32);
+            if ((t & 63) == 0) reinterpret_cast<int*>(&Yx[1][0])[t >> 6] = big;
             __syncthreads();
+            int any = 0;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int w = 0; w < NT / 64; ++w)
+                    any |= reinterpret_cast<const int*>(&sY[g][1][0])[w];
+            if (!any) levels = s + 1;   // uniform: the remaining levels are negligible
         }
     }
     M2 Fin[BR];
@@ -277,7 +292,9 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
             const bool small = dmax <= tol * xmax;
             const bool contracting = sweep > 1 && dmax <= 0.1 * dprev &&
                                      10.0 * dmax * dmax <= tol * xmax * dprev;
-            const bool done = small || contracting;
+            // dmax = INF marks a NaN (or an overflow); INF <= 0.1 * INF would
+            // pass the contraction test
+            const bool done = dmax < INFINITY && xmax < INFINITY && (small || contracting);
             dprev = dmax;
             if (done || sweep >= max_sweeps) {
                 if (!done) ++sweep;               // flag: tolerance not met

@@ -177,6 +177,7 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
+    __shared__ int amx[NT / 64];
     if (stop[m0] == 0) return;   // every system has stopped
     // logical blocks b, b + grid, ... (grid a multiple of 8: same XCD)
     for (int b = blockIdx.x; b < Q; b += gridDim.x) {
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
                 const long long i = 2 * (R + ((long long)l << S)) + h;
                 return i < n0 ? (long long)j * xss + i * xrs : -1LL;
             },
-            x, first != 0, &st[j], sR, red);
+            x, first != 0, &st[j], sR, red, amx);
     }
 }
 

@@ -279,6 +279,7 @@ __global__ __launch_bounds__(NT) void ustage_kernel(BRowSets sets,
 {
     __shared__ LdsRec<NT * BR> sR;
     __shared__ unsigned long long red[2][NT / 64];
+    __shared__ int amx[NT / 64];
     if (stop[m] == 0) return;   // every system has stopped
     // logical blocks b, b + grid, ... (grid a multiple of 8: same XCD)
     for (int b = blockIdx.x; b < njobs; b += gridDim.x) {
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(NT) void ustage_kernel(BRowSets sets,
                 const long long i = 2 * (J.R + ((long long)l << J.S)) + h;
                 return i < S.n ? S.off + i : -1LL;
             },
-            x, first != 0, &st[J.sys], sR, red);
+            x, first != 0, &st[J.sys], sR, red, amx);
     }
 }
 
