@@ -41,7 +41,8 @@ constexpr size_t kBRowBytes = 3 * sizeof(float4) + sizeof(float2);   // 56
 struct SysState {
     unsigned long long dmax, xmax;   // bit patterns of max |dx|, max |x| this sweep
     double dprev;                    // max |dx| of the previous sweep
-    int sweep, pad;
+    int sweep;                       // index of the sweep being decided
+    int last;                        // index of the last sweep run (set on stopping)
 };
 
 __device__ __forceinline__ BRow identity_brow()
@@ -353,6 +354,7 @@ __global__ void decide_kernel(SysState* __restrict__ st, int* __restrict__ stop,
     const bool done = !bad && (small || contracting);
     s.dprev = dmax;
     if (done || bad || s.sweep >= max_sweeps) {
+        s.last = s.sweep;
         if (bad) s.sweep = max_sweeps + 1;
         else if (!done) ++s.sweep;          // flag: tolerance not met
         stop[j] = 1;

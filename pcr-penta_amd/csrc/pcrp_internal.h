@@ -111,6 +111,18 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
 // FP32 inner solve + FP64 refinement over an uneven batch (pcrp_uneven_f32r.hip).
 size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m);
 
+// Windowed FP32 refinement for diagonally dominant long / uneven batches
+// (pcrp_window_f32r.hip). fell_back: the batch was not solved (a system's
+// couplings outlast a window's halo, or the stream is capturing) and y is
+// untouched; the caller then runs the pass/stage path.
+size_t window_f32r_workspace(long long elems, int m);
+int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
+                        bool& fell_back);
+int window_f32r_uneven(const double* a, const double* b, const double* c, const double* d,
+                       const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
+                       int max_sweeps, double tol, int* sweeps_out, void* ws, size_t ws_bytes,
+                       hipStream_t st, bool& fell_back);
+
 inline int pcr_steps(long long n)
 {
     int s = 0;

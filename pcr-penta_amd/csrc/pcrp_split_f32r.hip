@@ -35,6 +35,8 @@
 #include "pcrp_block_f32.h"
 #include "pcrp_internal.h"
 
+#include <algorithm>
+
 namespace pcrp {
 
 namespace {
@@ -199,13 +201,18 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
 
 size_t split_f32r_workspace(int n, int m)
 {
-    return plan(n, m).bytes;
+    return std::max(plan(n, m).bytes, window_f32r_workspace((long long)n * m, m));
 }
 
 int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_bytes,
                      hipStream_t st)
 {
     (void)layout;
+    // diagonally dominant batches: one windowed pass per sweep
+    bool fell_back = true;
+    const int wrc = window_f32r_uniform(A, ws, ws_bytes, st, fell_back);
+    if (wrc != PCRP_OK) return wrc == PCRP_ERR_WORKSPACE ? -3 : -1;
+    if (!fell_back) return 0;
     const int n0 = A.n, m0 = A.m;
     const Layout P = plan(n0, m0);
     const int S = P.S;

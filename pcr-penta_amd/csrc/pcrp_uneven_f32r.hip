@@ -308,7 +308,7 @@ size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m)
 {
     Plan P;
     if (plan_uneven(h_off, h_n, m, P) != PCRP_OK) return 0;
-    return P.bytes;
+    return std::max(P.bytes, window_f32r_workspace(P.rows, m));
 }
 
 int uneven_f32r_solve(const double* a, const double* b, const double* c, const double* d,
@@ -319,7 +319,13 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
     Plan P;
     int rc = plan_uneven(h_off, h_n, m, P);
     if (rc != PCRP_OK) return rc;
-    if (!ws || ws_bytes < P.bytes) return PCRP_ERR_WORKSPACE;
+    if (!ws || ws_bytes < std::max(P.bytes, window_f32r_workspace(P.rows, m)))
+        return PCRP_ERR_WORKSPACE;
+    // diagonally dominant batches: one windowed pass per sweep
+    bool fell_back = true;
+    rc = window_f32r_uneven(a, b, c, d, e, y, h_off, h_n, m, max_sweeps, tol, sweeps_out, ws,
+                            ws_bytes, st, fell_back);
+    if (rc != PCRP_OK || !fell_back) return rc;
     char* w = static_cast<char*>(ws);
     // host tables -> the front of the workspace in one copy
     std::vector<char> host(P.off_y0, 0);

@@ -1,0 +1,389 @@
+// FP32 inner solve + FP64 refinement by overlapping windows, for long and
+// uneven batches of diagonally dominant systems: one kernel per refinement
+// sweep, each reading A, y (and x) once and writing x once.
+//
+// No counterpart in the reference, which is FP64-only (kernel.cu:32-154,
+// 158-791). The inner solve is the 2x2-block PCR of pcrp_block_f32.h,
+// truncated: on a diagonally dominant matrix the multipliers of block cyclic
+// reduction decay quadratically from level to level (Heller, SIAM J. Numer.
+// Anal. 13 (1976) 484-496), and once every multiplier of a level is below
+// kTruncF32 the couplings left are negligible, so X = D^-1 Y right after it.
+// Stopping after level s, block row K depends only on block rows within
+// 2^(s+1) - 1 of it, so the solve is local (the truncated-SPIKE observation,
+// Polizzi & Sameh, Parallel Computing 32 (2006) 177-194):
+//   * a system of nb <= kWinNB block rows is one window, every level it needs
+//     (exact, like the one-workgroup kernel);
+//   * a longer system is cut into windows of kWinT interior block rows with
+//     kWinH = 2^kWinL - 1 halo block rows on each side; a window runs at most
+//     kWinL levels with neighbours outside the window treated as absent, so
+//     the values at its interior rows equal those of the whole-system PCR
+//     (the halo rows absorb the edge). A window whose interior multipliers are
+//     still above kTruncF32 after kWinL levels sets `nonlocal`, and the batch
+//     is solved by the pass/stage path instead (host read-back after the
+//     first sweep; results of the window path are discarded, y is untouched).
+// One sweep, per window: r = y0 - A x (FP64, rows of the window from global
+// memory), FP32 block rows, factor and right-hand side together level by
+// level in LDS, dx = D^-1 Y at the interior rows, x' = x + dx (FP64) into the
+// other x buffer (windows read x at their halo rows, so x is double
+// buffered: sweep it writes y when it is odd, the workspace copy when even),
+// max |dx| and max |x'| per system by atomic max; then the stopping rule of
+// the other refinement paths (decide_kernel). Systems whose last sweep wrote
+// the workspace copy are copied to y at the end.
+// Traffic per unknown: first sweep 48 B read + 16 B written (x and the copy
+// of y), later sweeps 56 B read + 8 B written, the final copy 16 B.
+#include "pcr_penta.h"
+#include "pcrp_block_f32.h"
+#include "pcrp_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace pcrp {
+
+namespace {
+
+constexpr int kWinNT = 256, kWinBR = 4;
+constexpr int kWinNB = kWinNT * kWinBR;    // block rows per window (LDS records)
+constexpr int kWinL = 5;                   // levels a window of a longer system may run
+constexpr int kWinH = (1 << kWinL) - 1;    // halo block rows per side: reach of kWinL levels
+constexpr int kWinT = kWinNB - 2 * kWinH;  // interior block rows
+
+struct WSys {
+    long long off;   // element of row 0 (row i at off + i * rs)
+    long long w0;    // first window of the system (uneven batches)
+    int n, nb, nw, pad;
+};
+
+struct WArgs {
+    const double *a, *b, *c, *d, *e;
+    const double* rhs;   // first sweep: y; later sweeps: y0
+    double* y0;          // first sweep: receives y at the interior rows; later: null
+    const double* xin;   // x of the previous sweep (null in the first)
+    double* xout;        // x of this sweep
+    long long rs;
+    const WSys* sys;
+    SysState* st;
+    const int* stop;
+    int* nonlocal;
+    int m, nwin, uniform;
+};
+
+__host__ __device__ inline int windows_for(int nb)
+{
+    return nb <= kWinNB ? 1 : (nb + kWinT - 1) / kWinT;
+}
+
+// Window w -> (system j, window index wi). Uniform batches are window-major
+// (the m systems' windows at one position are consecutive: in the interleaved
+// layout their rows share cache lines) with runs of 16 on one XCD; uneven
+// batches are system-major (a binary search on the first-window table).
+__device__ __forceinline__ void window_of(const WArgs& A, int w, int& j, int& wi)
+{
+    if (A.uniform) {
+        const int v = xcd_remap_c<16>(w, A.nwin);
+        j = v % A.m;
+        wi = v / A.m;
+        return;
+    }
+    int lo = 0, hi = A.m - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (A.sys[mid].w0 <= w) lo = mid; else hi = mid - 1;
+    }
+    j = lo;
+    wi = (int)(w - A.sys[lo].w0);
+}
+
+template <int NT, int BR>
+__global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
+{
+    constexpr int NB = NT * BR;
+    static_assert(NB == kWinNB, "one window per workgroup");
+    __shared__ LdsRec<NB> sR;
+    __shared__ unsigned long long red[2][NT / 64];
+    __shared__ int flg[NT / 64];
+    if (A.stop[A.m] == 0) return;   // every system has stopped
+    const int t = threadIdx.x;
+    // windows w, w + grid, ... (grid a multiple of 8: same XCD)
+    for (int w = blockIdx.x; w < A.nwin; w += gridDim.x) {
+        int j, wi;
+        window_of(A, w, j, wi);
+        if (A.stop[j]) continue;   // uniform over the workgroup
+        const WSys S = A.sys[j];
+        const bool full = S.nw == 1;
+        const int kb = full ? 0 : wi * kWinT - kWinH;   // block row of window slot 0
+        const int k0 = full ? 0 : wi * kWinT;           // interior block rows [k0, k1)
+        const int k1 = full ? S.nb : min(S.nb, k0 + kWinT);
+        int lmax = 0;
+        if (full) { while ((1 << lmax) < S.nb) ++lmax; } else { lmax = kWinL; }
+
+        BRow r[BR];
+        double xv[BR][2];   // x of the previous sweep (first sweep: y) at the rows
+#pragma unroll
+        for (int q = 0; q < BR; ++q) {
+            const int K = kb + t + NT * q;
+            xv[q][0] = xv[q][1] = 0.0;
+            if (K < 0 || K >= S.nb) {
+                r[q] = identity_brow();
+                continue;
+            }
+            double rw[2][6];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const long long i = 2LL * K + h;
+                if (i < S.n) {
+                    const long long g = S.off + i * A.rs;
+                    load_row(A.a, A.b, A.c, A.d, A.e, A.rhs, g, i, S.n, rw[h]);
+                    if (A.xin) {
+                        xv[q][h] = A.xin[g];
+                        rw[h][5] = residual_row(A.a, A.b, A.c, A.d, A.e, A.rhs, A.xin, g, i, S.n,
+                                                A.rs);
+                    } else {
+                        xv[q][h] = rw[h][5];
+                    }
+                } else {   // identity row past the end of an odd n
+                    rw[h][0] = rw[h][1] = rw[h][3] = rw[h][4] = rw[h][5] = 0.0;
+                    rw[h][2] = 1.0;
+                }
+            }
+            r[q] = brow_from_rows(rw);
+        }
+
+        // levels: factor and right-hand side together, until the interior
+        // multipliers are all below kTruncF32 (uniform over the workgroup)
+        int s = 0;
+        for (; s < lmax; ++s) {
+            const int delta = 1 << s;
+            // rows outside the system stay identity rows and are nobody's
+            // neighbour: skipped (whole waves of a short system's window)
+#pragma unroll
+            for (int q = 0; q < BR; ++q) {
+                const int K = kb + t + NT * q;
+                if (K >= 0 && K < S.nb) put_rec(sR, t + NT * q, minv(r[q].D), r[q]);
+            }
+            __syncthreads();
+            float am = 0.f;
+#pragma unroll
+            for (int q = 0; q < BR; ++q) {
+                const int l = t + NT * q, K = kb + l;
+                if (K < 0 || K >= S.nb) continue;
+                // neighbours outside the system or the window are absent
+                const bool hl = l - delta >= 0 && K - delta >= 0;
+                const bool hr = l + delta < NB && K + delta < S.nb;
+                float ak = 0.f;
+                r[q] = level(r[q], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l, ak);
+                if (K >= k0 && K < k1) am = fmaxf(am, ak);
+            }
+            const int big = __any(am > kTruncF32);   // one flag per wave
+            if ((t & 63) == 0) flg[t >> 6] = big;
+            __syncthreads();
+            int any = 0;
+#pragma unroll
+            for (int v = 0; v < NT / 64; ++v) any |= flg[v];
+            if (!any) break;
+        }
+        // couplings above the threshold past the halo's reach: not local
+        if (s == lmax && !full && t == 0) *A.nonlocal = 1;
+
+        unsigned long long dmax = 0ull, xmax = 0ull;
+#pragma unroll
+        for (int q = 0; q < BR; ++q) {
+            const int K = kb + t + NT * q;
+            if (K >= k0 && K < k1) {
+                const M2 Di = minv(r[q].D);
+                const float dd[2] = {fmaf(Di.p, r[q].y0, Di.q * r[q].y1),
+                                     fmaf(Di.r, r[q].y0, Di.s * r[q].y1)};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const long long i = 2LL * K + h;
+                    if (i < S.n) {
+                        const long long g = S.off + i * A.rs;
+                        const double dx = (double)dd[h];
+                        double xn;
+                        if (A.xin) {
+                            xn = xv[q][h] + dx;
+                        } else {
+                            xn = dx;
+                            A.y0[g] = xv[q][h];
+                        }
+                        A.xout[g] = xn;
+                        const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
+                        dmax = bd > dmax ? bd : dmax;
+                        xmax = bx > xmax ? bx : xmax;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long od = __shfl_xor(dmax, off, 64);
+            const unsigned long long ox = __shfl_xor(xmax, off, 64);
+            dmax = od > dmax ? od : dmax;
+            xmax = ox > xmax ? ox : xmax;
+        }
+        if ((t & 63) == 0) {
+            red[0][t >> 6] = dmax;
+            red[1][t >> 6] = xmax;
+        }
+        __syncthreads();
+        if (t == 0) {
+#pragma unroll
+            for (int v = 1; v < NT / 64; ++v) {
+                dmax = red[0][v] > dmax ? red[0][v] : dmax;
+                xmax = red[1][v] > xmax ? red[1][v] : xmax;
+            }
+            atomicMax(&A.st[j].dmax, dmax);
+            atomicMax(&A.st[j].xmax, xmax);
+        }
+        // (sR, flg and red are next written behind the next window's barriers)
+    }
+}
+
+// Systems whose last sweep wrote the workspace copy (even sweep index): copy
+// their interior rows to y. xin: the workspace copy, xout: y.
+__global__ __launch_bounds__(256) void wcopy_kernel(WArgs A)
+{
+    for (int w = blockIdx.x; w < A.nwin; w += gridDim.x) {
+        int j, wi;
+        window_of(A, w, j, wi);
+        if (A.st[j].last & 1) continue;
+        const WSys S = A.sys[j];
+        const bool full = S.nw == 1;
+        const long long i0 = full ? 0 : 2LL * wi * kWinT;
+        const long long ie = 2LL * (wi + 1) * kWinT;
+        const long long i1 = full || ie > S.n ? S.n : ie;
+        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            const long long g = S.off + i * A.rs;
+            A.xout[g] = A.xin[g];
+        }
+    }
+}
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct WLayout {
+    size_t state, stop, flag, sys, x, y0, bytes;
+};
+
+WLayout wlayout(long long elems, int m)
+{
+    WLayout L{};
+    size_t o = 0;
+    L.state = o; o = align_up(o + (size_t)m * sizeof(SysState));
+    L.stop = o;  o = align_up(o + (size_t)(m + 1) * sizeof(int));
+    L.flag = o;  o = align_up(o + sizeof(int));
+    L.sys = o;   o = align_up(o + (size_t)m * sizeof(WSys));
+    L.x = o;     o = align_up(o + (size_t)elems * sizeof(double));
+    L.y0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
+    L.bytes = o;
+    return L;
+}
+
+bool capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone;
+}
+
+// Runs the window path on the systems of `sys` (w0 and nw filled in here).
+// fell_back: a window was not local; nothing was written to y.
+int window_run(const double* a, const double* b, const double* c, const double* d,
+               const double* e, double* y, std::vector<WSys>& sys, bool uniform, long long rs,
+               long long elems, int max_sweeps, double tol, int* sweeps_out, void* ws,
+               size_t ws_bytes, hipStream_t st, bool& fell_back)
+{
+    fell_back = true;
+    const int m = (int)sys.size();
+    long long nwin = 0;
+    for (auto& s : sys) {
+        s.nw = windows_for(s.nb);
+        s.w0 = nwin;
+        nwin += s.nw;
+    }
+    if (nwin > 0x7fffffffLL) return PCRP_OK;   // leave it to the pass/stage path
+    const WLayout L = wlayout(elems, m);
+    if (!ws || ws_bytes < L.bytes) return PCRP_ERR_WORKSPACE;
+    char* w = static_cast<char*>(ws);
+    SysState* state = reinterpret_cast<SysState*>(w + L.state);
+    int* stop = reinterpret_cast<int*>(w + L.stop);
+    int* flag = reinterpret_cast<int*>(w + L.flag);
+    WSys* dsys = reinterpret_cast<WSys*>(w + L.sys);
+    double* xw = reinterpret_cast<double*>(w + L.x);
+    double* y0 = reinterpret_cast<double*>(w + L.y0);
+    if (hipMemcpyAsync(dsys, sys.data(), sys.size() * sizeof(WSys), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess)
+        return PCRP_ERR_HIP;
+    static const int gWin = resident_grid(wsweep_kernel<kWinNT, kWinBR>, kWinNT);
+    static const int gCopy = resident_grid(wcopy_kernel, 256);
+    const unsigned sblocks = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
+    WArgs A{a, b, c, d, e, y, y0, nullptr, xw, rs, dsys, state, stop, flag, m, (int)nwin,
+            uniform ? 1 : 0};
+    for (int it = 0; it <= max_sweeps; ++it) {
+        // sweep it writes y when odd, the workspace copy when even
+        A.rhs = it == 0 ? y : y0;
+        A.y0 = it == 0 ? y0 : nullptr;
+        A.xin = it == 0 ? nullptr : ((it - 1) & 1 ? y : xw);
+        A.xout = (it & 1) ? y : xw;
+        hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR>),
+                           dim3(walk_grid(nwin, it < kFullGridSweeps ? 0 : gWin)), dim3(kWinNT),
+                           0, st, A);
+        hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
+                           max_sweeps, tol, sweeps_out);
+        if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+        if (it == 0) {
+            int h = 0;
+            if (hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return PCRP_ERR_HIP;
+            if (h) return PCRP_OK;   // fell_back stays true
+        }
+    }
+    A.xin = xw;
+    A.xout = y;
+    hipLaunchKernelGGL(wcopy_kernel, dim3(walk_grid(nwin, gCopy)), dim3(256), 0, st, A);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    fell_back = false;
+    return PCRP_OK;
+}
+
+}  // namespace
+
+size_t window_f32r_workspace(long long elems, int m)
+{
+    return wlayout(elems, m).bytes;
+}
+
+int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
+                        bool& fell_back)
+{
+    fell_back = true;
+    if (capturing(st)) return PCRP_OK;   // the host read-back cannot be captured
+    std::vector<WSys> sys(A.m);
+    const int nb = (A.n + 1) / 2;
+    for (int j = 0; j < A.m; ++j) sys[j] = WSys{(long long)j * A.ss, 0, A.n, nb, 0, 0};
+    return window_run(A.a, A.b, A.c, A.d, A.e, A.y, sys, true, A.rs, (long long)A.n * A.m,
+                      A.max_sweeps, A.tol, A.sweeps_out, ws, ws_bytes, st, fell_back);
+}
+
+int window_f32r_uneven(const double* a, const double* b, const double* c, const double* d,
+                       const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
+                       int max_sweeps, double tol, int* sweeps_out, void* ws, size_t ws_bytes,
+                       hipStream_t st, bool& fell_back)
+{
+    fell_back = true;
+    if (capturing(st)) return PCRP_OK;
+    std::vector<WSys> sys(m);
+    long long elems = 0;
+    for (int j = 0; j < m; ++j) {
+        if (h_n[j] < 1 || h_off[j] < 0) return PCRP_ERR_INVALID_SIZE;
+        sys[j] = WSys{h_off[j], 0, h_n[j], (h_n[j] + 1) / 2, 0, 0};
+        elems = std::max(elems, (long long)h_off[j] + h_n[j]);
+    }
+    return window_run(a, b, c, d, e, y, sys, false, 1, elems, max_sweeps, tol, sweeps_out, ws,
+                      ws_bytes, st, fell_back);
+}
+
+}  // namespace pcrp

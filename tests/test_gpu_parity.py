@@ -483,6 +483,76 @@ def test_uneven_refined_config4_sample(gpu, oracle_lib):
     assert errs.max() <= 1e-12 and sw.max() <= 3
 
 
+def _slow_decay_rows(n, seed):
+    """Rows of c = 2.05, b = d = -1, a = e = 0 (a shifted 1-D Laplacian,
+    condition ~80): its block multipliers fall below 2^-20 only after 6
+    reduction levels, past the reach of a long system's window, so a batch
+    holding it is solved by the pass/stage path."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros(n); e = np.zeros(n)
+    b = -np.ones(n); d = -np.ones(n); c = np.full(n, 2.05)
+    b[0] = 0.0; d[-1] = 0.0
+    return [a, b, c, d, e, rng.uniform(-1, 1, n)]
+
+
+@pytest.mark.parametrize("layout", [I, C])
+def test_refined_split_falls_back_on_slow_decay(gpu, oracle_lib, layout):
+    """A uniform batch with one system whose couplings outlast the window halo
+    (the others from the dominant generator) still meets 1e-12 vs dgbsv."""
+    import torch
+    n, m = 6000, 3
+    _say("refined-split-fallback", n, m, layout)
+    dev = gpu.generate(0, 41, n, m, layout)
+    slow = _slow_decay_rows(n, 5)
+    for t, v in zip(dev, slow):
+        (t.view(n, m)[:, 1] if layout == I else t.view(m, n)[1, :]).copy_(torch.from_numpy(v))
+    ins = [_np(t).copy() for t in dev]
+    x, sw = _refined(gpu, dev, n, m, layout)
+    ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+    err = oracle_lib.normwise_relerr(x, ref, n, m, layout)
+    assert err <= 1e-12, (err, sw)
+    assert sw.max() <= 8, sw
+
+
+def test_uneven_refined_falls_back_on_slow_decay(gpu, oracle_lib):
+    """Uneven batch with one long slow-decay system among dominant ones."""
+    import torch
+    sizes = [300, 5000, 2100, 9000]
+    off, n, total = gpu.uneven_layout(sizes)
+    _say("uneven-refined-fallback", sizes)
+    dev = gpu.generate_uneven(0, 8, off, n)
+    slow = _slow_decay_rows(9000, 6)
+    sl = slice(int(off[3]), int(off[3]) + 9000)
+    for t, v in zip(dev, slow):
+        t[sl].copy_(torch.from_numpy(v))
+    ins = [_np(t).copy() for t in dev]
+    sweeps = torch.full((len(n),), -1, dtype=torch.int32, device="cuda")
+    gpu.solve_uneven_refined(*dev, off, n, max_sweeps=8, tol=1e-12, sweeps=sweeps)
+    torch.cuda.synchronize()
+    x = _np(dev[5])
+    for o, nj in zip(off, n):
+        s = slice(int(o), int(o) + int(nj))
+        ref = oracle_lib.gbsv(*[t[s] for t in ins])
+        assert np.abs(x[s] - ref).max() / np.abs(ref).max() <= 1e-12, (int(nj), _np(sweeps))
+
+
+@pytest.mark.parametrize("n", [2049, 3848, 3849, 3850, 5773, 5775, 7697])
+def test_refined_window_edges(gpu, oracle_lib, n):
+    """Long systems cut into windows (962 interior block rows, 31 halo block
+    rows per side): sizes with one block row past a window edge, exact
+    multiples, odd n; both generators and layouts, m not a multiple of 16."""
+    m = 5
+    for kind, layout in ((0, I), (1, C)):
+        _say("refined-window", n, m, kind, layout)
+        dev = gpu.generate(kind, 900 + n, n, m, layout)
+        ins = [_np(t).copy() for t in dev]
+        x, sw = _refined(gpu, dev, n, m, layout)
+        ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+        err = oracle_lib.normwise_relerr(x, ref, n, m, layout)
+        assert err <= 1e-12, (n, kind, layout, err, sw)
+        assert sw.min() >= 1 and sw.max() <= 3, sw
+
+
 # ---------------- Heston ADI driver (hipGraph time steps) ----------------
 
 def _heston_gpu(gpu, precision, use_graph, **kw):
