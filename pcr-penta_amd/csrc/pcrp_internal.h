@@ -115,7 +115,8 @@ size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m);
 // (pcrp_window_f32r.hip). fell_back: the batch was not solved (a system's
 // couplings outlast a window's halo, or the stream is capturing) and y is
 // untouched; the caller then runs the pass/stage path.
-size_t window_f32r_workspace(long long elems, int m);
+size_t window_f32r_workspace(long long elems, int m, bool transposed);
+size_t window_f32r_workspace_uniform(int n, int m, int layout);
 int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
                         bool& fell_back);
 int window_f32r_uneven(const double* a, const double* b, const double* c, const double* d,

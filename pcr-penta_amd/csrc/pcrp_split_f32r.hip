@@ -201,7 +201,9 @@ __global__ __launch_bounds__(NT) void bstage_kernel(
 
 size_t split_f32r_workspace(int n, int m)
 {
-    return std::max(plan(n, m).bytes, window_f32r_workspace((long long)n * m, m));
+    return std::max(plan(n, m).bytes,
+                    std::max(window_f32r_workspace_uniform(n, m, PCRP_LAYOUT_INTERLEAVED),
+                             window_f32r_workspace_uniform(n, m, PCRP_LAYOUT_CONTIGUOUS)));
 }
 
 int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_bytes,

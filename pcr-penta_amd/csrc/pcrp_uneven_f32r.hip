@@ -308,7 +308,7 @@ size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m)
 {
     Plan P;
     if (plan_uneven(h_off, h_n, m, P) != PCRP_OK) return 0;
-    return std::max(P.bytes, window_f32r_workspace(P.rows, m));
+    return std::max(P.bytes, window_f32r_workspace(P.rows, m, false));
 }
 
 int uneven_f32r_solve(const double* a, const double* b, const double* c, const double* d,
@@ -319,7 +319,7 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
     Plan P;
     int rc = plan_uneven(h_off, h_n, m, P);
     if (rc != PCRP_OK) return rc;
-    if (!ws || ws_bytes < std::max(P.bytes, window_f32r_workspace(P.rows, m)))
+    if (!ws || ws_bytes < std::max(P.bytes, window_f32r_workspace(P.rows, m, false)))
         return PCRP_ERR_WORKSPACE;
     // diagonally dominant batches: one windowed pass per sweep
     bool fell_back = true;

@@ -42,7 +42,7 @@ namespace pcrp {
 
 namespace {
 
-constexpr int kWinNT = 256, kWinBR = 4;
+constexpr int kWinNT = 256, kWinBR = 2;
 constexpr int kWinNB = kWinNT * kWinBR;    // block rows per window (LDS records)
 constexpr int kWinL = 5;                   // levels a window of a longer system may run
 constexpr int kWinH = (1 << kWinL) - 1;    // halo block rows per side: reach of kWinL levels
@@ -67,6 +67,14 @@ struct WArgs {
     int* nonlocal;
     int m, nwin, uniform;
 };
+
+// Interleaved uniform batches (more than one system) are transposed first:
+// in place, a window's loads would touch one element per cache line.
+inline bool window_transposed(int n, int m, long long rs)
+{
+    (void)n;
+    return m > 1 && rs == m;
+}
 
 __host__ __device__ inline int windows_for(int nb)
 {
@@ -204,7 +212,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                             xn = xv[q][h] + dx;
                         } else {
                             xn = dx;
-                            A.y0[g] = xv[q][h];
+                            if (A.y0) A.y0[g] = xv[q][h];
                         }
                         A.xout[g] = xn;
                         const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
@@ -259,14 +267,70 @@ __global__ __launch_bounds__(256) void wcopy_kernel(WArgs A)
     }
 }
 
+// Interleaved -> system-contiguous copies of the six arrays (row i of system
+// j from i*m + j to j*n + i): tiles of 64 rows x 32 systems through LDS, so
+// both sides are coalesced. blockIdx.z: the array.
+struct Six {
+    const double* p[6];
+};
+constexpr int kTrRows = 64, kTrSys = 32;
+
+__global__ __launch_bounds__(256) void wtranspose_in_kernel(Six src, double* __restrict__ dst,
+                                                            int n, int m)
+{
+    __shared__ double tile[kTrRows][kTrSys + 1];
+    const int f = blockIdx.z;
+    const long long i0 = (long long)blockIdx.x * kTrRows;
+    const int j0 = blockIdx.y * kTrSys;
+    const double* __restrict__ in = src.p[f];
+    const int tj = threadIdx.x % kTrSys, ti = threadIdx.x / kTrSys;   // 32 x 8
+    for (int r = ti; r < kTrRows; r += 256 / kTrSys) {
+        const long long i = i0 + r;
+        if (i < n && j0 + tj < m) tile[r][tj] = in[i * m + j0 + tj];
+    }
+    __syncthreads();
+    const int oi = threadIdx.x % kTrRows, oj = threadIdx.x / kTrRows;   // 64 x 4
+    double* __restrict__ out = dst + (long long)f * n * m;
+    for (int c = oj; c < kTrSys; c += 256 / kTrRows) {
+        const long long i = i0 + oi;
+        if (i < n && j0 + c < m) out[(long long)(j0 + c) * n + i] = tile[oi][c];
+    }
+}
+
+// The result back to the interleaved y: system j's x from the buffer its
+// last sweep wrote (odd: x1, even: x0), system-contiguous.
+__global__ __launch_bounds__(256) void wtranspose_out_kernel(const double* __restrict__ x0,
+                                                             const double* __restrict__ x1,
+                                                             const SysState* __restrict__ st,
+                                                             double* __restrict__ y, int n, int m)
+{
+    __shared__ double tile[kTrSys][kTrRows + 1];
+    const long long i0 = (long long)blockIdx.x * kTrRows;
+    const int j0 = blockIdx.y * kTrSys;
+    const int oi = threadIdx.x % kTrRows, oj = threadIdx.x / kTrRows;   // 64 x 4
+    for (int c = oj; c < kTrSys; c += 256 / kTrRows) {
+        const long long i = i0 + oi;
+        const int j = j0 + c;
+        if (i < n && j < m) tile[c][oi] = ((st[j].last & 1) ? x1 : x0)[(long long)j * n + i];
+    }
+    __syncthreads();
+    const int tj = threadIdx.x % kTrSys, ti = threadIdx.x / kTrSys;   // 32 x 8
+    for (int r = ti; r < kTrRows; r += 256 / kTrSys) {
+        const long long i = i0 + r;
+        if (i < n && j0 + tj < m) y[i * m + j0 + tj] = tile[tj][r];
+    }
+}
+
 constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
+// Direct: x buffer and copy of y (16 B per element). Transposed (interleaved
+// uniform batches): six system-contiguous arrays and two x buffers (64 B).
 struct WLayout {
-    size_t state, stop, flag, sys, x, y0, bytes;
+    size_t state, stop, flag, sys, x, y0, six, bytes;
 };
 
-WLayout wlayout(long long elems, int m)
+WLayout wlayout(long long elems, int m, bool transposed)
 {
     WLayout L{};
     size_t o = 0;
@@ -276,6 +340,7 @@ WLayout wlayout(long long elems, int m)
     L.sys = o;   o = align_up(o + (size_t)m * sizeof(WSys));
     L.x = o;     o = align_up(o + (size_t)elems * sizeof(double));
     L.y0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
+    L.six = o;   if (transposed) o = align_up(o + 6 * (size_t)elems * sizeof(double));
     L.bytes = o;
     return L;
 }
@@ -287,11 +352,13 @@ bool capturing(hipStream_t st)
 }
 
 // Runs the window path on the systems of `sys` (w0 and nw filled in here).
-// fell_back: a window was not local; nothing was written to y.
+// transposed: uniform interleaved batch of n rows (rs = m); the windows then
+// work on system-contiguous copies. fell_back: a window was not local;
+// nothing was written to y.
 int window_run(const double* a, const double* b, const double* c, const double* d,
                const double* e, double* y, std::vector<WSys>& sys, bool uniform, long long rs,
-               long long elems, int max_sweeps, double tol, int* sweeps_out, void* ws,
-               size_t ws_bytes, hipStream_t st, bool& fell_back)
+               long long elems, bool transposed, int n, int max_sweeps, double tol,
+               int* sweeps_out, void* ws, size_t ws_bytes, hipStream_t st, bool& fell_back)
 {
     fell_back = true;
     const int m = (int)sys.size();
@@ -302,7 +369,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         nwin += s.nw;
     }
     if (nwin > 0x7fffffffLL) return PCRP_OK;   // leave it to the pass/stage path
-    const WLayout L = wlayout(elems, m);
+    const WLayout L = wlayout(elems, m, transposed);
     if (!ws || ws_bytes < L.bytes) return PCRP_ERR_WORKSPACE;
     char* w = static_cast<char*>(ws);
     SysState* state = reinterpret_cast<SysState*>(w + L.state);
@@ -311,6 +378,11 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     WSys* dsys = reinterpret_cast<WSys*>(w + L.sys);
     double* xw = reinterpret_cast<double*>(w + L.x);
     double* y0 = reinterpret_cast<double*>(w + L.y0);
+    double* six = reinterpret_cast<double*>(w + L.six);
+    if (transposed) {
+        rs = 1;
+        for (int j = 0; j < m; ++j) sys[j].off = (long long)j * n;
+    }
     if (hipMemcpyAsync(dsys, sys.data(), sys.size() * sizeof(WSys), hipMemcpyHostToDevice, st) !=
             hipSuccess ||
         hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess)
@@ -319,14 +391,31 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     static const int gCopy = resident_grid(wcopy_kernel, 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
-    WArgs A{a, b, c, d, e, y, y0, nullptr, xw, rs, dsys, state, stop, flag, m, (int)nwin,
-            uniform ? 1 : 0};
+    const dim3 tgrid((unsigned)((n + kTrRows - 1) / kTrRows), (unsigned)((m + kTrSys - 1) / kTrSys));
+    if (transposed) {
+        if (tgrid.x > 0x7fffffffu || tgrid.y > 65535u) return PCRP_ERR_INVALID_SIZE;
+        hipLaunchKernelGGL(wtranspose_in_kernel, dim3(tgrid.x, tgrid.y, 6), dim3(256), 0, st,
+                           Six{{a, b, c, d, e, y}}, six, n, m);
+    }
+    const long long N = elems;
+    // direct: sweep it writes y when odd, the workspace x when even, and the
+    // first sweep copies y to y0; transposed: x buffers xw / y0, A and y from
+    // the copies
+    const double* ca = transposed ? six : a;
+    WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
+            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
+            nullptr, rs, dsys, state, stop, flag, m, (int)nwin, uniform ? 1 : 0};
+    double* const xbuf[2] = {xw, transposed ? y0 : y};
     for (int it = 0; it <= max_sweeps; ++it) {
-        // sweep it writes y when odd, the workspace copy when even
-        A.rhs = it == 0 ? y : y0;
-        A.y0 = it == 0 ? y0 : nullptr;
-        A.xin = it == 0 ? nullptr : ((it - 1) & 1 ? y : xw);
-        A.xout = (it & 1) ? y : xw;
+        if (transposed) {
+            A.rhs = six + 5 * N;
+            A.y0 = nullptr;
+        } else {
+            A.rhs = it == 0 ? y : y0;
+            A.y0 = it == 0 ? y0 : nullptr;
+        }
+        A.xin = it == 0 ? nullptr : xbuf[(it - 1) & 1];
+        A.xout = xbuf[it & 1];
         hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR>),
                            dim3(walk_grid(nwin, it < kFullGridSweeps ? 0 : gWin)), dim3(kWinNT),
                            0, st, A);
@@ -341,9 +430,14 @@ int window_run(const double* a, const double* b, const double* c, const double* 
             if (h) return PCRP_OK;   // fell_back stays true
         }
     }
-    A.xin = xw;
-    A.xout = y;
-    hipLaunchKernelGGL(wcopy_kernel, dim3(walk_grid(nwin, gCopy)), dim3(256), 0, st, A);
+    if (transposed) {
+        hipLaunchKernelGGL(wtranspose_out_kernel, dim3(tgrid.x, tgrid.y), dim3(256), 0, st, xbuf[0],
+                           xbuf[1], state, y, n, m);
+    } else {
+        A.xin = xw;
+        A.xout = y;
+        hipLaunchKernelGGL(wcopy_kernel, dim3(walk_grid(nwin, gCopy)), dim3(256), 0, st, A);
+    }
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     fell_back = false;
     return PCRP_OK;
@@ -351,9 +445,14 @@ int window_run(const double* a, const double* b, const double* c, const double* 
 
 }  // namespace
 
-size_t window_f32r_workspace(long long elems, int m)
+size_t window_f32r_workspace(long long elems, int m, bool transposed)
 {
-    return wlayout(elems, m).bytes;
+    return wlayout(elems, m, transposed).bytes;
+}
+
+size_t window_f32r_workspace_uniform(int n, int m, int layout)
+{
+    return wlayout((long long)n * m, m, layout == PCRP_LAYOUT_INTERLEAVED && m > 1).bytes;
 }
 
 int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
@@ -365,7 +464,8 @@ int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipSt
     const int nb = (A.n + 1) / 2;
     for (int j = 0; j < A.m; ++j) sys[j] = WSys{(long long)j * A.ss, 0, A.n, nb, 0, 0};
     return window_run(A.a, A.b, A.c, A.d, A.e, A.y, sys, true, A.rs, (long long)A.n * A.m,
-                      A.max_sweeps, A.tol, A.sweeps_out, ws, ws_bytes, st, fell_back);
+                      window_transposed(A.n, A.m, A.rs), A.n, A.max_sweeps, A.tol, A.sweeps_out,
+                      ws, ws_bytes, st, fell_back);
 }
 
 int window_f32r_uneven(const double* a, const double* b, const double* c, const double* d,
@@ -382,8 +482,8 @@ int window_f32r_uneven(const double* a, const double* b, const double* c, const 
         sys[j] = WSys{h_off[j], 0, h_n[j], (h_n[j] + 1) / 2, 0, 0};
         elems = std::max(elems, (long long)h_off[j] + h_n[j]);
     }
-    return window_run(a, b, c, d, e, y, sys, false, 1, elems, max_sweeps, tol, sweeps_out, ws,
-                      ws_bytes, st, fell_back);
+    return window_run(a, b, c, d, e, y, sys, false, 1, elems, false, 0, max_sweeps, tol,
+                      sweeps_out, ws, ws_bytes, st, fell_back);
 }
 
 }  // namespace pcrp

@@ -536,11 +536,12 @@ def test_uneven_refined_falls_back_on_slow_decay(gpu, oracle_lib):
         assert np.abs(x[s] - ref).max() / np.abs(ref).max() <= 1e-12, (int(nj), _np(sweeps))
 
 
-@pytest.mark.parametrize("n", [2049, 3848, 3849, 3850, 5773, 5775, 7697])
+@pytest.mark.parametrize("n", [2049, 2700, 2701, 2702, 3601, 3603, 4500])
 def test_refined_window_edges(gpu, oracle_lib, n):
-    """Long systems cut into windows (962 interior block rows, 31 halo block
+    """Long systems cut into windows (450 interior block rows, 31 halo block
     rows per side): sizes with one block row past a window edge, exact
-    multiples, odd n; both generators and layouts, m not a multiple of 16."""
+    multiples, odd n; both generators and layouts (the interleaved batch is
+    transposed first), m not a multiple of 32."""
     m = 5
     for kind, layout in ((0, I), (1, C)):
         _say("refined-window", n, m, kind, layout)
