@@ -9,6 +9,8 @@
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
+#include <algorithm>
+
 #include <stdio.h>
 
 using namespace pcrp;
@@ -154,6 +156,9 @@ size_t pcrp_workspace_size(int n, int m, int precision)
     if (check_shape(n, m) != PCRP_OK) return 0;
     if (precision == PCRP_PREC_F64) return split_f64_workspace(n, m);
     if (precision == PCRP_PREC_F32_REFINE && n > lds_f32r_max_n()) return split_f32r_workspace(n, m);
+    if (precision == PCRP_PREC_F32_REFINE && window_preferred(n, m))
+        return std::max(window_f32r_workspace_uniform(n, m, PCRP_LAYOUT_INTERLEAVED),
+                        window_f32r_workspace_uniform(n, m, PCRP_LAYOUT_CONTIGUOUS));
     return 0;
 }
 
@@ -187,7 +192,17 @@ int pcrp_solve_f32r_f64(const double* a, const double* b, const double* c, const
     SolveArgsF32R A{a, b, c, d, e, y, n, m, 0, 0, pcr_steps(n), max_sweeps, tol, sweeps_out};
     if (!layout_strides(n, m, layout, &A.rs, &A.ss)) return PCRP_ERR_LAYOUT;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (n <= lds_f32r_max_n()) return lds_f32r_solve(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    if (n <= lds_f32r_max_n()) {
+        // large batches of n > 1024: the window path when a workspace is given
+        // (the one-workgroup kernel if a system's couplings do not decay)
+        if (window_preferred(n, m) && ws && ws_bytes >= window_f32r_workspace_uniform(n, m, layout)) {
+            bool fell_back = true;
+            const int rc = window_f32r_uniform(A, ws, ws_bytes, s, fell_back);
+            if (rc != PCRP_OK) return rc;
+            if (!fell_back) return PCRP_OK;
+        }
+        return lds_f32r_solve(A, s) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    }
     if (!ws || ws_bytes < split_f32r_workspace(n, m)) return PCRP_ERR_WORKSPACE;
     const int rc = split_f32r_solve(A, layout, ws, ws_bytes, s);
     return rc == 0 ? PCRP_OK : (rc == -3 ? PCRP_ERR_WORKSPACE : PCRP_ERR_HIP);

@@ -217,6 +217,22 @@ def test_refined_config1_shape(gpu, oracle_lib):
     print("sweeps histogram", np.bincount(sw), flush=True)
 
 
+@pytest.mark.parametrize("layout", [I, C])
+def test_refined_large_batch_window_route(gpu, oracle_lib, layout):
+    """n in (1024, 2048] with at least 2^23 unknowns runs the window path
+    (the one-workgroup kernel below that): 1e-12 vs dgbsv, 2 corrections."""
+    n = 1100
+    m = (1 << 23) // n + 1
+    _say("refined-window-route", n, m, layout)
+    dev = gpu.generate(0, 77, n, m, layout)
+    ins = [_np(t).copy() for t in dev]
+    x, sw = _refined(gpu, dev, n, m, layout)
+    ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+    err = oracle_lib.normwise_relerr(x, ref, n, m, layout)
+    assert err <= 1e-12, (err, np.bincount(sw))
+    assert sw.min() >= 1 and sw.max() <= 3, np.bincount(sw)
+
+
 @pytest.mark.parametrize("n", [64, 1000, 3000])
 def test_refined_flags_nan_system(gpu, oracle_lib, n):
     """A system with a NaN coefficient is flagged (sweeps = max_sweeps + 1)
