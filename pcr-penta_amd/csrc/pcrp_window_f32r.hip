@@ -66,6 +66,7 @@ struct WArgs {
     const int* stop;
     int* nonlocal;
     int m, nwin, uniform;
+    int vec;   // rs == 1 and every array 16-byte aligned
 };
 
 // Interleaved uniform batches (more than one system) are transposed first:
@@ -102,6 +103,46 @@ __device__ __forceinline__ void window_of(const WArgs& A, int w, int& j, int& wi
     wi = (int)(w - A.sys[lo].w0);
 }
 
+__device__ __forceinline__ double2 ld2(const double* p)
+{
+    return *reinterpret_cast<const double2*>(p);
+}
+
+// Rows i0 = 2K and i0 + 1 of a contiguous system, 2 <= i0 and i0 + 3 < n
+// (every coupling in the band, x at i0-2 .. i0+3 in the system), element g0
+// 16-byte aligned: one 16-byte load per array for both rows, three for x.
+// The residual keeps residual_row's operation order.
+__device__ __forceinline__ void load_pair_vec(const WArgs& A, long long g0, double (&rw)[2][6],
+                                              double (&xv)[2])
+{
+    const double2 av = ld2(A.a + g0), bv = ld2(A.b + g0), cv = ld2(A.c + g0);
+    const double2 dv = ld2(A.d + g0), ev = ld2(A.e + g0), yv = ld2(A.rhs + g0);
+    rw[0][0] = av.x; rw[0][1] = bv.x; rw[0][2] = cv.x; rw[0][3] = dv.x; rw[0][4] = ev.x;
+    rw[1][0] = av.y; rw[1][1] = bv.y; rw[1][2] = cv.y; rw[1][3] = dv.y; rw[1][4] = ev.y;
+    if (A.xin) {
+        const double2 xm = ld2(A.xin + g0 - 2), xc = ld2(A.xin + g0), xp = ld2(A.xin + g0 + 2);
+        xv[0] = xc.x;
+        xv[1] = xc.y;
+        double ax = cv.x * xc.x;
+        ax = __builtin_fma(av.x, xm.x, ax);
+        ax = __builtin_fma(bv.x, xm.y, ax);
+        ax = __builtin_fma(dv.x, xc.y, ax);
+        ax = __builtin_fma(ev.x, xp.x, ax);
+        rw[0][5] = yv.x - ax;
+        ax = cv.y * xc.y;
+        ax = __builtin_fma(av.y, xm.y, ax);
+        ax = __builtin_fma(bv.y, xc.x, ax);
+        ax = __builtin_fma(dv.y, xp.x, ax);
+        ax = __builtin_fma(ev.y, xp.y, ax);
+        rw[1][5] = yv.y - ax;
+    } else {
+        xv[0] = yv.x;
+        xv[1] = yv.y;
+        rw[0][5] = yv.x;
+        rw[1][5] = yv.y;
+    }
+}
+
 template <int NT, int BR>
 __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
 {
@@ -136,6 +177,12 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                 continue;
             }
             double rw[2][6];
+            const long long i0 = 2LL * K;
+            if (A.vec && ((S.off & 1) == 0) && i0 >= 2 && i0 + 3 < S.n) {
+                load_pair_vec(A, S.off + i0, rw, xv[q]);
+                r[q] = brow_from_rows(rw);
+                continue;
+            }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const long long i = 2LL * K + h;
@@ -404,7 +451,12 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     const double* ca = transposed ? six : a;
     WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
             transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
-            nullptr, rs, dsys, state, stop, flag, m, (int)nwin, uniform ? 1 : 0};
+            nullptr, rs, dsys, state, stop, flag, m, (int)nwin, uniform ? 1 : 0, 0};
+    {
+        const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+        A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
+                al16(y) && al16(xw) && al16(y0) && al16(six) ? 1 : 0;
+    }
     double* const xbuf[2] = {xw, transposed ? y0 : y};
     for (int it = 0; it <= max_sweeps; ++it) {
         if (transposed) {
