@@ -142,10 +142,10 @@ __device__ __forceinline__ float maxabs2(const M2& m)
 // before, and the next level's multipliers about its square (quadratic
 // decay of the off-diagonal blocks of block cyclic reduction on diagonally
 // dominant matrices, Heller, SIAM J. Numer. Anal. 13 (1976) 484-496): the
-// remaining levels would change X by ~2^-40 relative, far below the FP32
+// remaining levels would change X by ~2^-32 relative, far below the FP32
 // solve's own ~3e-7, so X = D^-1 Y right after that level. Systems whose
 // couplings do not decay run every level.
-constexpr float kTruncF32 = 0x1p-20f;
+constexpr float kTruncF32 = 0x1p-16f;
 
 __device__ __forceinline__ float4 f4(const M2& m) { return make_float4(m.p, m.q, m.r, m.s); }
 __device__ __forceinline__ M2 m2(const float4& v) { return M2{v.x, v.y, v.z, v.w}; }
