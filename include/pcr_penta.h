@@ -84,20 +84,32 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
  * (k >= 2), is <= tol ||x|| (and the contraction is <= 0.1). Norms are
  * max-norms. Inputs and output are FP64. sweeps_out: optional DEVICE array of
  * m ints receiving each system's number of refinement sweeps (max_sweeps + 1
- * flags a system that did not stop, or whose correction or iterate is NaN).
+ * flags a system that did not stop, or whose correction or iterate is NaN
+ * or infinite).
  * solve32 is PCR on the 2x2-block tridiagonal form of the matrix (rows 2k,
  * 2k+1 as one block row, diagonal-block pivots), which keeps FP32 error near
- * 3e-7 where PCR-Penta's own off-diagonal pivots lose up to 1e-2.
+ * 3e-7 where PCR-Penta's own off-diagonal pivots lose up to 1e-2. Its
+ * levels stop early once every multiplier of a level is below 2^-16 (the
+ * couplings left are then ~2^-32 relative: diagonally dominant systems need
+ * 4-5 levels).
  * Sizes n <= 2048 run in one LDS-resident kernel per system (the FP32
  * factorisation is computed once and its multipliers kept in registers;
  * interleaved batches with more systems than CUs may take two neighbouring
- * systems per workgroup, with the same results).
- * Larger n need a workspace (pcrp_workspace_size with PCRP_PREC_F32_REFINE):
- * every sweep re-runs the FP32 block solve split like pcrp_solve_f64 (HBM
- * passes, then an LDS stage per residue sub-system) on the FP32 residual
- * (from the second sweep on the passes reuse the matrix records the first
- * one left in the workspace), with per-system stop flags so converged
- * systems drop out of later sweeps.
+ * systems per workgroup, with the same results), except batches of n > 1024
+ * with at least 2^23 unknowns given a workspace of pcrp_workspace_size
+ * bytes, which take the window path below.
+ * Larger n need a workspace (pcrp_workspace_size with PCRP_PREC_F32_REFINE).
+ * Window path (diagonally dominant systems): each sweep is one pass over
+ * windows of 450 block rows with 31-block-row halos, computing the FP64
+ * residual, the truncated FP32 block solve and x += dx; interleaved batches
+ * are first copied to a system-contiguous layout in the workspace. If a
+ * window's couplings outlast its halo, the call reads that flag back to the
+ * host after the first sweep (one stream synchronisation) and solves the
+ * batch with the split path instead: every sweep re-runs the FP32 block
+ * solve split like pcrp_solve_f64 (HBM passes, then an LDS stage per residue
+ * sub-system) on the FP32 residual. Under stream capture the split path runs
+ * directly (no read-back), so a captured call stays asynchronous.
+ * Per-system stop flags let converged systems drop out of later sweeps.
  * A block pivot with determinant below 2^-100 gets a zero inverse (the
  * refinement corrects the difference, or flags the system).
  */
@@ -131,8 +143,12 @@ int pcrp_solve_uneven_f64(const double *a, const double *b, const double *c,
  * pcrp_solve_uneven_f64 (pcrp_uneven_workspace_size with
  * PCRP_PREC_F32_REFINE); refinement arguments and stopping rule as
  * pcrp_solve_f32r_f64, sweeps_out a DEVICE array of m ints. The inner solve
- * is PCR on the 2x2-block form: systems up to 2048 rows in one workgroup,
- * larger ones after HBM pass rounds shared by all large systems.
+ * is PCR on the 2x2-block form. Diagonally dominant batches take the window
+ * path of pcrp_solve_f32r_f64 (systems of up to 1024 rows as one window,
+ * larger ones cut into windows; one stream synchronisation after the first
+ * sweep); otherwise, and under stream capture: systems up to 2048 rows in
+ * one workgroup, larger ones after HBM pass rounds shared by all large
+ * systems.
  */
 int pcrp_solve_uneven_f32r_f64(const double *a, const double *b, const double *c,
                                const double *d, const double *e, double *y,
