@@ -552,6 +552,38 @@ def test_uneven_refined_falls_back_on_slow_decay(gpu, oracle_lib):
         assert np.abs(x[s] - ref).max() / np.abs(ref).max() <= 1e-12, (int(nj), _np(sweeps))
 
 
+def test_refined_split_under_graph_capture(gpu, oracle_lib):
+    """A long-system FP32 refinement captured in a graph takes the pass/stage
+    path (the window path's host read-back cannot be captured): three
+    replays, with eager copies and synchronisations between them, each within
+    1e-12 of dgbsv and of the eager (window path) result."""
+    import torch
+    n, m = 4096, 3
+    _say("refined-graph", n, m)
+    dev = gpu.generate(0, 61, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+    y0 = dev[5].clone()
+    ws, nb = gpu._workspace(n, m, gpu.PREC_F32_REFINE, dev[5].device)
+    x_eager, _ = _refined(gpu, dev, n, m, I)
+    assert oracle_lib.normwise_relerr(x_eager, ref, n, m, I) <= 1e-12
+    y = y0.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        gpu.solve_refined(*dev[:5], y, n, m, I, max_sweeps=8, tol=1e-12, stream=s, ws=ws)
+    torch.cuda.synchronize()
+    for k in range(3):
+        y.copy_(y0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        x = _np(y)
+        assert oracle_lib.normwise_relerr(x, ref, n, m, I) <= 1e-12, k
+        assert oracle_lib.normwise_relerr(x, x_eager, n, m, I) <= 1e-12, k
+
+
 @pytest.mark.parametrize("n", [2049, 2700, 2701, 2702, 3601, 3603, 4500])
 def test_refined_window_edges(gpu, oracle_lib, n):
     """Long systems cut into windows (450 interior block rows, 31 halo block
