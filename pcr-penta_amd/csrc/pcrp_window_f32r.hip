@@ -65,6 +65,7 @@ struct WArgs {
     SysState* st;
     const int* stop;
     int* nonlocal;
+    unsigned char* lev;   // levels each window runs (found by the first sweep)
     int m, nwin, uniform;
     int vec;   // rs == 1 and every array 16-byte aligned
 };
@@ -143,7 +144,10 @@ __device__ __forceinline__ void load_pair_vec(const WArgs& A, long long g0, doub
     }
 }
 
-template <int NT, int BR>
+// FIND (first sweep): run levels until the interior multipliers are all
+// below kTruncF32 and store the count per window; later sweeps (same matrix,
+// same multipliers) run exactly that many levels without the test.
+template <int NT, int BR, bool FIND>
 __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
 {
     constexpr int NB = NT * BR;
@@ -206,6 +210,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
 
         // levels: factor and right-hand side together, until the interior
         // multipliers are all below kTruncF32 (uniform over the workgroup)
+        if (!FIND) lmax = A.lev[w];
         int s = 0;
         for (; s < lmax; ++s) {
             const int delta = 1 << s;
@@ -227,18 +232,25 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                 const bool hr = l + delta < NB && K + delta < S.nb;
                 float ak = 0.f;
                 r[q] = level(r[q], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l, ak);
-                if (K >= k0 && K < k1) am = fmaxf(am, ak);
+                if (FIND && K >= k0 && K < k1) am = fmaxf(am, ak);
             }
-            const int big = __any(am > kTruncF32);   // one flag per wave
-            if ((t & 63) == 0) flg[t >> 6] = big;
+            if (FIND) {
+                const int big = __any(am > kTruncF32);   // one flag per wave
+                if ((t & 63) == 0) flg[t >> 6] = big;
+            }
             __syncthreads();
-            int any = 0;
+            if (FIND) {
+                int any = 0;
 #pragma unroll
-            for (int v = 0; v < NT / 64; ++v) any |= flg[v];
-            if (!any) break;
+                for (int v = 0; v < NT / 64; ++v) any |= flg[v];
+                if (!any) break;
+            }
         }
-        // couplings above the threshold past the halo's reach: not local
-        if (s == lmax && !full && t == 0) *A.nonlocal = 1;
+        if (FIND && t == 0) {
+            A.lev[w] = (unsigned char)(s < lmax ? s + 1 : lmax);
+            // couplings above the threshold past the halo's reach: not local
+            if (s == lmax && !full) *A.nonlocal = 1;
+        }
 
         unsigned long long dmax = 0ull, xmax = 0ull;
 #pragma unroll
@@ -371,10 +383,17 @@ __global__ __launch_bounds__(256) void wtranspose_out_kernel(const double* __res
 constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
+// Bound on the windows of `elems` rows in m systems: windows_for(nb) <=
+// nb / kWinT + 1 and nb <= n / 2 + 1 per system.
+size_t lev_capacity(long long elems, int m)
+{
+    return (size_t)elems / (2 * kWinT) + 2 * (size_t)m + 64;
+}
+
 // Direct: x buffer and copy of y (16 B per element). Transposed (interleaved
 // uniform batches): six system-contiguous arrays and two x buffers (64 B).
 struct WLayout {
-    size_t state, stop, flag, sys, x, y0, six, bytes;
+    size_t state, stop, flag, sys, lev, x, y0, six, bytes;
 };
 
 WLayout wlayout(long long elems, int m, bool transposed)
@@ -385,6 +404,7 @@ WLayout wlayout(long long elems, int m, bool transposed)
     L.stop = o;  o = align_up(o + (size_t)(m + 1) * sizeof(int));
     L.flag = o;  o = align_up(o + sizeof(int));
     L.sys = o;   o = align_up(o + (size_t)m * sizeof(WSys));
+    L.lev = o;   o = align_up(o + lev_capacity(elems, m));
     L.x = o;     o = align_up(o + (size_t)elems * sizeof(double));
     L.y0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
     L.six = o;   if (transposed) o = align_up(o + 6 * (size_t)elems * sizeof(double));
@@ -415,8 +435,9 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         s.w0 = nwin;
         nwin += s.nw;
     }
-    if (nwin > 0x7fffffffLL) return PCRP_OK;   // leave it to the pass/stage path
     const WLayout L = wlayout(elems, m, transposed);
+    // leave it to the pass/stage path
+    if (nwin > 0x7fffffffLL || (size_t)nwin > lev_capacity(elems, m)) return PCRP_OK;
     if (!ws || ws_bytes < L.bytes) return PCRP_ERR_WORKSPACE;
     char* w = static_cast<char*>(ws);
     SysState* state = reinterpret_cast<SysState*>(w + L.state);
@@ -426,6 +447,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     double* xw = reinterpret_cast<double*>(w + L.x);
     double* y0 = reinterpret_cast<double*>(w + L.y0);
     double* six = reinterpret_cast<double*>(w + L.six);
+    unsigned char* lev = reinterpret_cast<unsigned char*>(w + L.lev);
     if (transposed) {
         rs = 1;
         for (int j = 0; j < m; ++j) sys[j].off = (long long)j * n;
@@ -434,7 +456,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
             hipSuccess ||
         hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess)
         return PCRP_ERR_HIP;
-    static const int gWin = resident_grid(wsweep_kernel<kWinNT, kWinBR>, kWinNT);
+    static const int gWin = resident_grid(wsweep_kernel<kWinNT, kWinBR, false>, kWinNT);
     static const int gCopy = resident_grid(wcopy_kernel, 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
@@ -451,7 +473,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     const double* ca = transposed ? six : a;
     WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
             transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
-            nullptr, rs, dsys, state, stop, flag, m, (int)nwin, uniform ? 1 : 0, 0};
+            nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0, 0};
     {
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
@@ -468,9 +490,13 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         }
         A.xin = it == 0 ? nullptr : xbuf[(it - 1) & 1];
         A.xout = xbuf[it & 1];
-        hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR>),
-                           dim3(walk_grid(nwin, it < kFullGridSweeps ? 0 : gWin)), dim3(kWinNT),
-                           0, st, A);
+        if (it == 0)
+            hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR, true>), dim3((unsigned)nwin),
+                               dim3(kWinNT), 0, st, A);
+        else
+            hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR, false>),
+                               dim3(walk_grid(nwin, it < kFullGridSweeps ? 0 : gWin)),
+                               dim3(kWinNT), 0, st, A);
         hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
                            max_sweeps, tol, sweeps_out);
         if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
