@@ -21,16 +21,23 @@
 //     still above kTruncF32 after kWinL levels sets `nonlocal`, and the batch
 //     is solved by the pass/stage path instead (host read-back after the
 //     first sweep; results of the window path are discarded, y is untouched).
+//     A stream under capture cannot read back: the callers then use the
+//     pass/stage path directly.
 // One sweep, per window: r = y0 - A x (FP64, rows of the window from global
 // memory), FP32 block rows, factor and right-hand side together level by
-// level in LDS, dx = D^-1 Y at the interior rows, x' = x + dx (FP64) into the
-// other x buffer (windows read x at their halo rows, so x is double
-// buffered: sweep it writes y when it is odd, the workspace copy when even),
-// max |dx| and max |x'| per system by atomic max; then the stopping rule of
-// the other refinement paths (decide_kernel). Systems whose last sweep wrote
-// the workspace copy are copied to y at the end.
-// Traffic per unknown: first sweep 48 B read + 16 B written (x and the copy
-// of y), later sweeps 56 B read + 8 B written, the final copy 16 B.
+// level in LDS (the first sweep finds each window's level count, later
+// sweeps, same matrix, run that count), dx = D^-1 Y at the interior rows,
+// x' = x + dx (FP64) into the other x buffer (windows read x at their halo
+// rows, so x is double buffered), max |dx| and max |x'| per system by atomic
+// max; then the stopping rule of the other refinement paths (decide_kernel).
+// Direct form (uneven and contiguous batches): sweep it writes y when it is
+// odd and the workspace copy when even, the first sweep also copies y to
+// y0, and systems whose last sweep was even are copied to y at the end:
+// per unknown, first sweep 48 B read + 16 B written, later sweeps 56 B read
+// + 8 B written, the final copy 16 B. Interleaved uniform batches are first
+// copied to system-contiguous arrays (96 B per unknown, LDS-tiled transpose),
+// swept there with both x buffers in the workspace, and x is transposed back
+// (16 B).
 #include "pcr_penta.h"
 #include "pcrp_block_f32.h"
 #include "pcrp_internal.h"
@@ -72,9 +79,8 @@ struct WArgs {
 
 // Interleaved uniform batches (more than one system) are transposed first:
 // in place, a window's loads would touch one element per cache line.
-inline bool window_transposed(int n, int m, long long rs)
+inline bool window_transposed(int m, long long rs)
 {
-    (void)n;
     return m > 1 && rs == m;
 }
 
@@ -542,7 +548,7 @@ int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipSt
     const int nb = (A.n + 1) / 2;
     for (int j = 0; j < A.m; ++j) sys[j] = WSys{(long long)j * A.ss, 0, A.n, nb, 0, 0};
     return window_run(A.a, A.b, A.c, A.d, A.e, A.y, sys, true, A.rs, (long long)A.n * A.m,
-                      window_transposed(A.n, A.m, A.rs), A.n, A.max_sweeps, A.tol, A.sweeps_out,
+                      window_transposed(A.m, A.rs), A.n, A.max_sweeps, A.tol, A.sweeps_out,
                       ws, ws_bytes, st, fell_back);
 }
 
