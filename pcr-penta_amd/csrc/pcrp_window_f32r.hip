@@ -55,10 +55,15 @@ constexpr int kWinL = 5;                   // levels a window of a longer system
 constexpr int kWinH = (1 << kWinL) - 1;    // halo block rows per side: reach of kWinL levels
 constexpr int kWinT = kWinNB - 2 * kWinH;  // interior block rows
 
+// Block row K holds rows 2K - shift and 2K - shift + 1 (rows outside 0..n-1
+// are identity rows): a contiguous system at an odd element offset takes
+// shift = 1, so that every block row's first element is 16-byte aligned
+// (the leading identity row is decoupled: row 0's couplings to rows -2, -1
+// are outside the band).
 struct WSys {
     long long off;   // element of row 0 (row i at off + i * rs)
     long long w0;    // first window of the system (uneven batches)
-    int n, nb, nw, pad;
+    int n, nb, nw, shift;
 };
 
 struct WArgs {
@@ -115,7 +120,7 @@ __device__ __forceinline__ double2 ld2(const double* p)
     return *reinterpret_cast<const double2*>(p);
 }
 
-// Rows i0 = 2K and i0 + 1 of a contiguous system, 2 <= i0 and i0 + 3 < n
+// Rows i0 and i0 + 1 of a contiguous system, 2 <= i0 and i0 + 3 < n
 // (every coupling in the band, x at i0-2 .. i0+3 in the system), element g0
 // 16-byte aligned: one 16-byte load per array for both rows, three for x.
 // The residual keeps residual_row's operation order.
@@ -187,16 +192,16 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                 continue;
             }
             double rw[2][6];
-            const long long i0 = 2LL * K;
-            if (A.vec && ((S.off & 1) == 0) && i0 >= 2 && i0 + 3 < S.n) {
+            const long long i0 = 2LL * K - S.shift;
+            if (A.vec && ((S.off + i0) & 1) == 0 && i0 >= 2 && i0 + 3 < S.n) {
                 load_pair_vec(A, S.off + i0, rw, xv[q]);
                 r[q] = brow_from_rows(rw);
                 continue;
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const long long i = 2LL * K + h;
-                if (i < S.n) {
+                const long long i = i0 + h;
+                if (i >= 0 && i < S.n) {
                     const long long g = S.off + i * A.rs;
                     load_row(A.a, A.b, A.c, A.d, A.e, A.rhs, g, i, S.n, rw[h]);
                     if (A.xin) {
@@ -206,7 +211,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                     } else {
                         xv[q][h] = rw[h][5];
                     }
-                } else {   // identity row past the end of an odd n
+                } else {   // identity row before row 0 or past row n-1
                     rw[h][0] = rw[h][1] = rw[h][3] = rw[h][4] = rw[h][5] = 0.0;
                     rw[h][2] = 1.0;
                 }
@@ -268,8 +273,8 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                                      fmaf(Di.r, r[q].y0, Di.s * r[q].y1)};
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const long long i = 2LL * K + h;
-                    if (i < S.n) {
+                    const long long i = 2LL * K - S.shift + h;
+                    if (i >= 0 && i < S.n) {
                         const long long g = S.off + i * A.rs;
                         const double dx = (double)dd[h];
                         double xn;
@@ -322,9 +327,10 @@ __global__ __launch_bounds__(256) void wcopy_kernel(WArgs A)
         if (A.st[j].last & 1) continue;
         const WSys S = A.sys[j];
         const bool full = S.nw == 1;
-        const long long i0 = full ? 0 : 2LL * wi * kWinT;
-        const long long ie = 2LL * (wi + 1) * kWinT;
-        const long long i1 = full || ie > S.n ? S.n : ie;
+        const int k0 = full ? 0 : wi * kWinT;
+        const int k1 = full ? S.nb : min(S.nb, k0 + kWinT);
+        const long long i0 = max(0LL, 2LL * k0 - S.shift);
+        const long long i1 = min((long long)S.n, 2LL * k1 - S.shift);
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             const long long g = S.off + i * A.rs;
             A.xout[g] = A.xin[g];
@@ -435,8 +441,14 @@ int window_run(const double* a, const double* b, const double* c, const double* 
 {
     fell_back = true;
     const int m = (int)sys.size();
+    if (transposed) {
+        rs = 1;
+        for (int j = 0; j < m; ++j) sys[j].off = (long long)j * n;
+    }
     long long nwin = 0;
     for (auto& s : sys) {
+        s.shift = rs == 1 ? (int)(s.off & 1) : 0;
+        s.nb = (s.n + s.shift + 1) / 2;
         s.nw = windows_for(s.nb);
         s.w0 = nwin;
         nwin += s.nw;
@@ -454,10 +466,6 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     double* y0 = reinterpret_cast<double*>(w + L.y0);
     double* six = reinterpret_cast<double*>(w + L.six);
     unsigned char* lev = reinterpret_cast<unsigned char*>(w + L.lev);
-    if (transposed) {
-        rs = 1;
-        for (int j = 0; j < m; ++j) sys[j].off = (long long)j * n;
-    }
     if (hipMemcpyAsync(dsys, sys.data(), sys.size() * sizeof(WSys), hipMemcpyHostToDevice, st) !=
             hipSuccess ||
         hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess)
