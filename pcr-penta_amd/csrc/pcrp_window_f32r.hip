@@ -23,21 +23,20 @@
 //     first sweep; results of the window path are discarded, y is untouched).
 //     A stream under capture cannot read back: the callers then use the
 //     pass/stage path directly.
-// One sweep, per window: r = y0 - A x (FP64, rows of the window from global
+// One sweep, per window: r = y - A x (FP64, rows of the window from global
 // memory), FP32 block rows, factor and right-hand side together level by
 // level in LDS (the first sweep finds each window's level count, later
 // sweeps, same matrix, run that count), dx = D^-1 Y at the interior rows,
 // x' = x + dx (FP64) into the other x buffer (windows read x at their halo
 // rows, so x is double buffered), max |dx| and max |x'| per system by atomic
 // max; then the stopping rule of the other refinement paths (decide_kernel).
-// Direct form (uneven and contiguous batches): sweep it writes y when it is
-// odd and the workspace copy when even, the first sweep also copies y to
-// y0, and systems whose last sweep was even are copied to y at the end:
-// per unknown, first sweep 48 B read + 16 B written, later sweeps 56 B read
-// + 8 B written, the final copy 16 B. Interleaved uniform batches are first
-// copied to system-contiguous arrays (96 B per unknown, LDS-tiled transpose),
-// swept there with both x buffers in the workspace, and x is transposed back
-// (16 B).
+// Both x buffers live in the workspace (sweep it writes buffer it & 1), so
+// the caller's y stays the right-hand side of every sweep, and each system's
+// x is copied to y at the end from the buffer its last sweep wrote: per
+// unknown, first sweep 48 B read + 8 B written, later sweeps 56 B read + 8 B
+// written, the final copy 16 B. Interleaved uniform batches are first copied
+// to system-contiguous arrays (96 B per unknown, LDS-tiled transpose), swept
+// there, and x is transposed back (16 B).
 #include "pcr_penta.h"
 #include "pcrp_block_f32.h"
 #include "pcrp_internal.h"
@@ -68,8 +67,7 @@ struct WSys {
 
 struct WArgs {
     const double *a, *b, *c, *d, *e;
-    const double* rhs;   // first sweep: y; later sweeps: y0
-    double* y0;          // first sweep: receives y at the interior rows; later: null
+    const double* rhs;   // y (or its system-contiguous copy)
     const double* xin;   // x of the previous sweep (null in the first)
     double* xout;        // x of this sweep
     long long rs;
@@ -148,8 +146,6 @@ __device__ __forceinline__ void load_pair_vec(const WArgs& A, long long g0, doub
         ax = __builtin_fma(ev.y, xp.y, ax);
         rw[1][5] = yv.y - ax;
     } else {
-        xv[0] = yv.x;
-        xv[1] = yv.y;
         rw[0][5] = yv.x;
         rw[1][5] = yv.y;
     }
@@ -182,7 +178,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
         if (full) { while ((1 << lmax) < S.nb) ++lmax; } else { lmax = kWinL; }
 
         BRow r[BR];
-        double xv[BR][2];   // x of the previous sweep (first sweep: y) at the rows
+        double xv[BR][2];   // x of the previous sweep at the rows (first sweep: 0)
 #pragma unroll
         for (int q = 0; q < BR; ++q) {
             const int K = kb + t + NT * q;
@@ -208,8 +204,6 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                         xv[q][h] = A.xin[g];
                         rw[h][5] = residual_row(A.a, A.b, A.c, A.d, A.e, A.rhs, A.xin, g, i, S.n,
                                                 A.rs);
-                    } else {
-                        xv[q][h] = rw[h][5];
                     }
                 } else {   // identity row before row 0 or past row n-1
                     rw[h][0] = rw[h][1] = rw[h][3] = rw[h][4] = rw[h][5] = 0.0;
@@ -277,13 +271,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                     if (i >= 0 && i < S.n) {
                         const long long g = S.off + i * A.rs;
                         const double dx = (double)dd[h];
-                        double xn;
-                        if (A.xin) {
-                            xn = xv[q][h] + dx;
-                        } else {
-                            xn = dx;
-                            if (A.y0) A.y0[g] = xv[q][h];
-                        }
+                        const double xn = A.xin ? xv[q][h] + dx : dx;
                         A.xout[g] = xn;
                         const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
                         dmax = bd > dmax ? bd : dmax;
@@ -317,14 +305,16 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
     }
 }
 
-// Systems whose last sweep wrote the workspace copy (even sweep index): copy
-// their interior rows to y. xin: the workspace copy, xout: y.
-__global__ __launch_bounds__(256) void wcopy_kernel(WArgs A)
+// x of every system to y, from the buffer its last sweep wrote (even sweep
+// index: x0, odd: x1), over the window's interior rows.
+__global__ __launch_bounds__(256) void wcopy_kernel(WArgs A, const double* __restrict__ x0,
+                                                    const double* __restrict__ x1,
+                                                    double* __restrict__ y)
 {
     for (int w = blockIdx.x; w < A.nwin; w += gridDim.x) {
         int j, wi;
         window_of(A, w, j, wi);
-        if (A.st[j].last & 1) continue;
+        const double* __restrict__ x = (A.st[j].last & 1) ? x1 : x0;
         const WSys S = A.sys[j];
         const bool full = S.nw == 1;
         const int k0 = full ? 0 : wi * kWinT;
@@ -333,7 +323,7 @@ __global__ __launch_bounds__(256) void wcopy_kernel(WArgs A)
         const long long i1 = min((long long)S.n, 2LL * k1 - S.shift);
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             const long long g = S.off + i * A.rs;
-            A.xout[g] = A.xin[g];
+            y[g] = x[g];
         }
     }
 }
@@ -405,7 +395,7 @@ size_t lev_capacity(long long elems, int m)
 // Direct: x buffer and copy of y (16 B per element). Transposed (interleaved
 // uniform batches): six system-contiguous arrays and two x buffers (64 B).
 struct WLayout {
-    size_t state, stop, flag, sys, lev, x, y0, six, bytes;
+    size_t state, stop, flag, sys, lev, x0, x1, six, bytes;
 };
 
 WLayout wlayout(long long elems, int m, bool transposed)
@@ -417,8 +407,8 @@ WLayout wlayout(long long elems, int m, bool transposed)
     L.flag = o;  o = align_up(o + sizeof(int));
     L.sys = o;   o = align_up(o + (size_t)m * sizeof(WSys));
     L.lev = o;   o = align_up(o + lev_capacity(elems, m));
-    L.x = o;     o = align_up(o + (size_t)elems * sizeof(double));
-    L.y0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
+    L.x0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
+    L.x1 = o;    o = align_up(o + (size_t)elems * sizeof(double));
     L.six = o;   if (transposed) o = align_up(o + 6 * (size_t)elems * sizeof(double));
     L.bytes = o;
     return L;
@@ -462,8 +452,8 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     int* stop = reinterpret_cast<int*>(w + L.stop);
     int* flag = reinterpret_cast<int*>(w + L.flag);
     WSys* dsys = reinterpret_cast<WSys*>(w + L.sys);
-    double* xw = reinterpret_cast<double*>(w + L.x);
-    double* y0 = reinterpret_cast<double*>(w + L.y0);
+    double* x0 = reinterpret_cast<double*>(w + L.x0);
+    double* x1 = reinterpret_cast<double*>(w + L.x1);
     double* six = reinterpret_cast<double*>(w + L.six);
     unsigned char* lev = reinterpret_cast<unsigned char*>(w + L.lev);
     if (hipMemcpyAsync(dsys, sys.data(), sys.size() * sizeof(WSys), hipMemcpyHostToDevice, st) !=
@@ -481,27 +471,21 @@ int window_run(const double* a, const double* b, const double* c, const double* 
                            Six{{a, b, c, d, e, y}}, six, n, m);
     }
     const long long N = elems;
-    // direct: sweep it writes y when odd, the workspace x when even, and the
-    // first sweep copies y to y0; transposed: x buffers xw / y0, A and y from
-    // the copies
+    // transposed: A and y from the system-contiguous copies
     const double* ca = transposed ? six : a;
     WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
-            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
+            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr,
             nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0, 0};
     {
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
-                al16(y) && al16(xw) && al16(y0) && al16(six) ? 1 : 0;
+                al16(y) && al16(x0) && al16(x1) && al16(six) ? 1 : 0;
     }
-    double* const xbuf[2] = {xw, transposed ? y0 : y};
+    // both x buffers in the workspace: y (or its transposed copy) stays the
+    // right-hand side of every sweep
+    double* const xbuf[2] = {x0, x1};
     for (int it = 0; it <= max_sweeps; ++it) {
-        if (transposed) {
-            A.rhs = six + 5 * N;
-            A.y0 = nullptr;
-        } else {
-            A.rhs = it == 0 ? y : y0;
-            A.y0 = it == 0 ? y0 : nullptr;
-        }
+        A.rhs = transposed ? six + 5 * N : y;
         A.xin = it == 0 ? nullptr : xbuf[(it - 1) & 1];
         A.xout = xbuf[it & 1];
         if (it == 0)
@@ -526,9 +510,8 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         hipLaunchKernelGGL(wtranspose_out_kernel, dim3(tgrid.x, tgrid.y), dim3(256), 0, st, xbuf[0],
                            xbuf[1], state, y, n, m);
     } else {
-        A.xin = xw;
-        A.xout = y;
-        hipLaunchKernelGGL(wcopy_kernel, dim3(walk_grid(nwin, gCopy)), dim3(256), 0, st, A);
+        hipLaunchKernelGGL(wcopy_kernel, dim3(walk_grid(nwin, gCopy)), dim3(256), 0, st, A,
+                           xbuf[0], xbuf[1], y);
     }
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     fell_back = false;
