@@ -230,7 +230,9 @@ def bench_uneven(args, m, prec, kind):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(args.workload),
                      "solve_ms": solve_s * 1e3,
-                     "note": "whole solve (pass rounds + persistent LDS launch) per step"},
+                     "note": ("whole solve per step: three window-path refinement sweeps "
+                              "(one kernel each) and the final copy" if refined else
+                              "whole solve (pass rounds + persistent LDS launch) per step")},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
