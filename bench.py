@@ -28,7 +28,8 @@ BYTES_PER_UNKNOWN = 56     # 7 FP64 words
 
 WORKLOADS = {
     # name: (n, m, precision, generator kind); kind 0 = strictly dominant
-    # (config 0/1 distribution), kind 1 = weakly dominant (config 2)
+    # (off-diagonals U[-1,1], diagonal U[4.5,5.5]), kind 1 = weakly dominant
+    # (off-diagonals U[-0.5,0.5], diagonal U[2.2,2.6])
     "lds_f64_m16384_n1024": (1024, 16384, "f64", 0),
     "lds_f32r_m16384_n2048": (2048, 16384, "f32r", 0),
     "lds_f32r_m16384_n1024": (1024, 16384, "f32r", 0),
@@ -36,10 +37,11 @@ WORKLOADS = {
     "lds_f64_m256_n512": (512, 256, "f64", 0),
     "hbm_f64_m32_n1048576": (1 << 20, 32, "f64", 1),
     "hbm_f32r_m32_n1048576": (1 << 20, 32, "f32r", 1),
-    # uneven: n = 0 marks the config-4 size draw (10000 systems, n log-uniform in [64, 131072])
+    # uneven: n = 0 marks the size draw of 10000 systems, n log-uniform in [64, 131072]
     "uneven_f64_m10000": (0, 10000, "f64", 0),
     "uneven_f32r_m10000": (0, 10000, "f32r", 0),
-    # Heston ADI (config 3): n = -1 marks the ADI driver; one step = one whole 200-step run
+    # Heston ADI (Ns = 1024, Nv = 512): n = -1 marks the ADI driver; one step = one whole
+    # 200-step run
     "heston_adi_f64": (-1, 0, "f64", 0),
     "heston_adi_f32r": (-1, 0, "f32r", 0),
 }
@@ -137,7 +139,8 @@ def measured_traffic(workload):
 
 
 def bench_uneven(args, m, prec, kind):
-    """Config 4: one uneven batch (contiguous systems back to back), seed 1."""
+    """One uneven batch (contiguous systems back to back, n log-uniform in [64, 131072]),
+    seed 1."""
     import numpy as np
     import torch
     import pcrpenta
@@ -145,7 +148,7 @@ def bench_uneven(args, m, prec, kind):
     K, W = args.steps, args.warmup
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    sizes = pcrpenta.config4_sizes(m, seed)
+    sizes = pcrpenta.uneven_log_uniform_sizes(m, seed)
     off, nn, total = pcrpenta.uneven_layout(sizes)
     a, b, c, d, e, y0 = pcrpenta.generate_uneven(kind, seed, off, nn, device=dev)
     nbuf = max(1, min(K + W, int(8e9 // (8 * total))))
@@ -218,7 +221,8 @@ def bench_uneven(args, m, prec, kind):
         "value": total / (wall / K), "unit": "unknowns/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": prec,
-        "data": "synthetic (seeded counter-based generator, config-0 distribution, config-4 sizes)",
+        "data": "synthetic (seeded counter-based generator, strictly dominant distribution, 10000 "
+                "sizes log-uniform in [64, 131072])",
         "config": {"workload": args.workload, "m": m, "unknowns": total, "n_min": int(nn.min()),
                    "n_max": int(nn.max()), "large_systems": int((nn > 1024).sum()),
                    "layout": "contiguous per system", "precision": prec, "seed": seed,
@@ -265,7 +269,7 @@ def bench_uneven(args, m, prec, kind):
 
 
 def bench_heston(args, prec):
-    """Config 3: Heston ADI, Douglas theta = 1/2, Ns = 1024, Nv = 512, 200 steps,
+    """Heston ADI, Douglas theta = 1/2, Ns = 1024, Nv = 512, 200 steps,
     each step replayed from one captured hipGraph. Unknowns per step: Nv S-lines
     of 1024 plus Ns v-lines of 512 = 2 * 524288."""
     import torch
@@ -294,7 +298,7 @@ def bench_heston(args, prec):
         "value": unknowns / (wall / K), "unit": "unknowns/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": prec,
-        "data": "synthetic (Heston European call, config-3 parameters)",
+        "data": "synthetic (Heston European call, Ns = 1024, Nv = 512, 200 steps of 1/200)",
         "config": {"workload": args.workload, "ns": p.ns, "nv": p.nv, "time_steps": p.steps,
                    "dt": p.dt, "scheme": "Douglas theta=0.5", "graph": "one hipGraph per time "
                    "step, replayed", "unknowns_per_run": unknowns,
@@ -408,7 +412,7 @@ def main():
         "vs_baseline": None,
         "dtype": prec,
         "data": "synthetic (seeded counter-based generator, "
-                + ("config-0 distribution)" if kind == 0 else "config-2 weakly dominant distribution)"),
+                + ("strictly dominant distribution)" if kind == 0 else "weakly dominant distribution)"),
         "config": {"workload": args.workload, "n": n, "m": m, "layout": "interleaved",
                    "precision": prec, "seed": seed,
                    "generator": "dominant" if kind == 0 else "weak",

@@ -357,8 +357,8 @@ def generate_uneven(kind: int, seed: int, offsets, sizes, device="cuda", stream=
     return arrs
 
 
-def config4_sizes(m: int = 10000, seed: int = 1, lo: int = 64, hi: int = 131072):
-    """Config-4 sizes: n_j = floor(exp(U[ln lo, ln hi))), U drawn by the counter
+def uneven_log_uniform_sizes(m: int = 10000, seed: int = 1, lo: int = 64, hi: int = 131072):
+    """Sizes of the uneven benchmark batch: n_j = floor(exp(U[ln lo, ln hi))), U drawn by the counter
     generator keyed (seed, 7, j, 0)."""
     import numpy as np
     M64 = (1 << 64) - 1
@@ -390,13 +390,13 @@ class HestonParams(ctypes.Structure):
                 ("theta_scheme", ctypes.c_double)]
 
 
-# config 3: Ns = 1024, Nv = 512 unknowns, 200 steps of 1/200
-CONFIG3 = dict(ns=1024, nv=512, smax=800.0, vmax=5.0, kappa=3.0, theta_v=0.04, sigma=0.15,
+# Heston ADI problem: Ns = 1024, Nv = 512 unknowns, 200 steps of 1/200
+HESTON_GRID = dict(ns=1024, nv=512, smax=800.0, vmax=5.0, kappa=3.0, theta_v=0.04, sigma=0.15,
                rho=-0.7, r=0.03, strike=100.0, dt=1.0 / 200, steps=200, theta_scheme=0.5)
 
 
 def heston_params(**kw) -> HestonParams:
-    d = dict(CONFIG3)
+    d = dict(HESTON_GRID)
     d.update(kw)
     return HestonParams(**d)
 

@@ -73,7 +73,7 @@ def lapack_dgbsv(a, b, c, d, e, y):
 
 def main():
     cases = []
-    # config 0 (m=256, n=512, seed 0, dominant), first 4 systems
+    # the m=256, n=512 dominant batch (seed 0), first 4 systems
     for j in range(4):
         cases.append(("c0", 0, 0, j, 512))
     # a spread of sizes, both generators, incl. non-powers of two (2x2 final branch)

@@ -38,10 +38,10 @@ def test_generator_matches_oracle(gpu, oracle_lib):
 
 
 @pytest.mark.parametrize("layout", [I, C])
-def test_config0_bit_exact(gpu, oracle_lib, layout):
-    """Config 0: m=256 systems of n=512, seed 0, strictly dominant."""
+def test_dominant_m256_n512_bit_exact(gpu, oracle_lib, layout):
+    """m=256 systems of n=512, seed 0, strictly dominant."""
     n, m = 512, 256
-    _say("config0", layout)
+    _say("m256-n512", layout)
     dev = gpu.generate(0, 0, n, m, layout)
     ins = [_np(t).copy() for t in dev]
     x = _run_f64(gpu, dev, n, m, layout)
@@ -205,10 +205,10 @@ def test_refined_reports_nonconvergence(gpu, oracle_lib):
     assert 1e-9 < err < 1e-1, err   # a plain FP32 PCR solve
 
 
-def test_refined_config1_shape(gpu, oracle_lib):
-    """Config-1 FP32+refine shape (n = 2048) on a slice of the batch."""
+def test_refined_n2048_slice(gpu, oracle_lib):
+    """FP32+refine at n = 2048 on a slice (m = 256) of the m = 16384 batch."""
     n, m = 2048, 256
-    _say("refined-c1", n, m)
+    _say("refined-n2048", n, m)
     dev = gpu.generate(0, 0, n, m, I)
     ins = [_np(t).copy() for t in dev]
     x, sw = _refined(gpu, dev, n, m, I)
@@ -314,17 +314,17 @@ def test_refined_split_boundary_sizes(gpu, oracle_lib, n):
         assert sw.min() >= 1 and sw.max() <= 3, sw
 
 
-def test_refined_split_config2_sample(gpu, oracle_lib):
-    """Config-2 size (n = 2^20, weakly dominant), 2 systems, FP32 + refinement:
+def test_refined_split_n2pow20_weak_sample(gpu, oracle_lib):
+    """n = 2^20, weakly dominant, 2 systems, FP32 + refinement:
     within 1e-12 of dgbsv."""
     n, m = 1 << 20, 2
-    _say("refined-split-c2", n, m)
+    _say("refined-split-n2^20", n, m)
     dev = gpu.generate(1, 0, n, m, I)
     ins = [_np(t).copy() for t in dev]
     x, sw = _refined(gpu, dev, n, m, I)
     xg = oracle_lib.gbsv_batch(*ins, n, m, I)
     err = oracle_lib.normwise_relerr(x, xg, n, m, I)
-    print("c2 refined vs dgbsv", err, "sweeps", sw, flush=True)
+    print("n=2^20 refined vs dgbsv", err, "sweeps", sw, flush=True)
     assert err <= 1e-12 and sw.max() <= 3
 
 
@@ -373,18 +373,18 @@ def test_split_tunables_bit_exact(gpu, oracle_lib):
         gpu.split_tune_set(*saved[:3])
 
 
-def test_split_config2_shape_sample(gpu, oracle_lib):
-    """Config-2 system size (n = 2^20, weakly dominant generator), 2 systems:
+def test_split_n2pow20_weak_sample(gpu, oracle_lib):
+    """n = 2^20 systems (weakly dominant generator), 2 systems:
     bit-identical to the oracle and within FP64 PCR noise of dgbsv."""
     n, m = 1 << 20, 2
-    _say("split-c2", n, m)
+    _say("split-n2^20", n, m)
     dev = gpu.generate(1, 0, n, m, I)
     ins = [_np(t).copy() for t in dev]
     x = _run_f64(gpu, dev, n, m, I)
     ref = oracle_lib.pcr_batch(*ins, n, m, I)
     assert np.array_equal(x, ref)
     xg = oracle_lib.gbsv_batch(*ins, n, m, I)
-    print("c2 pcr64 vs dgbsv", oracle_lib.normwise_relerr(x, xg, n, m, I), flush=True)
+    print("n=2^20 pcr64 vs dgbsv", oracle_lib.normwise_relerr(x, xg, n, m, I), flush=True)
 
 
 # ---------------- uneven batches (persistent work queue) ----------------
@@ -441,9 +441,9 @@ def test_uneven_deep_splits_bit_exact(gpu, oracle_lib):
         gpu.split_tune_set(*saved[:3])
 
 
-def test_uneven_config4_sample(gpu, oracle_lib):
-    """First 400 systems of the config-4 size draw (n log-uniform in [64, 131072], seed 1)."""
-    sizes = gpu.config4_sizes()[:400]
+def test_uneven_log_uniform_sample(gpu, oracle_lib):
+    """First 400 systems of the uneven size draw (n log-uniform in [64, 131072], seed 1)."""
+    sizes = gpu.uneven_log_uniform_sizes()[:400]
     _say("uneven-c4", len(sizes), int(sizes.sum()))
     x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 0, 1)
     assert np.array_equal(x, ref)
@@ -490,12 +490,12 @@ def test_uneven_refined_flags_nonconvergence(gpu, oracle_lib):
     assert errs.max() < 1e-2 and errs.max() > 1e-9, errs
 
 
-def test_uneven_refined_config4_sample(gpu, oracle_lib):
-    """First 300 systems of the config-4 size draw, FP32 + refinement."""
-    sizes = gpu.config4_sizes()[:300]
+def test_uneven_refined_log_uniform_sample(gpu, oracle_lib):
+    """First 300 systems of the uneven size draw, FP32 + refinement."""
+    sizes = gpu.uneven_log_uniform_sizes()[:300]
     _say("uneven-refined-c4", len(sizes), int(sizes.sum()))
     errs, sw = _uneven_refined_case(gpu, oracle_lib, sizes, 0, 1)
-    print("c4 sample refined: max err", errs.max(), "sweeps", np.bincount(sw), flush=True)
+    print("uneven sample refined: max err", errs.max(), "sweeps", np.bincount(sw), flush=True)
     assert errs.max() <= 1e-12 and sw.max() <= 3
 
 
@@ -622,7 +622,7 @@ def test_heston_matches_cpu_restatement(gpu, oracle_lib):
     kw = dict(ns=64, nv=32, steps=20)
     _say("heston-small")
     U, p = _heston_gpu(gpu, gpu.PREC_F64, False, **kw)
-    d = dict(gpu.CONFIG3)
+    d = dict(gpu.HESTON_GRID)
     d.update(kw)
     ref = H.adi(d, oracle_lib)
     err = np.abs(U - ref).max() / np.abs(ref).max()
@@ -638,13 +638,13 @@ def test_heston_graph_replay_equals_eager(gpu):
         assert np.array_equal(Ue, Ug), prec
 
 
-def test_heston_config3_price_and_refine(gpu):
-    """Config 3 (Ns=1024, Nv=512, 200 steps): the FP64 price at S=100, v=0.04
+def test_heston_price_and_refine(gpu):
+    """Ns=1024, Nv=512, 200 steps: the FP64 price at S=100, v=0.04
     is within 2e-3 relative of the semi-analytic Heston price, and FP32+refine
     agrees with FP64 to 1e-10 normwise."""
     import heston_ref as H
     from scipy.interpolate import RectBivariateSpline
-    _say("heston-c3")
+    _say("heston-1024x512")
     U64, p = _heston_gpu(gpu, gpu.PREC_F64, True)
     U32, _ = _heston_gpu(gpu, gpu.PREC_F32_REFINE, True)
     hS, hv = p.smax / (p.ns + 1), p.vmax / p.nv
