@@ -481,6 +481,28 @@ def test_uneven_refined_mixed_sizes(gpu, oracle_lib):
         assert sw.min() >= 1 and sw.max() <= 3, sw
 
 
+def test_uneven_refined_flags_nan_system(gpu, oracle_lib):
+    """Uneven batch (window path, odd and even offsets): a system with a NaN
+    coefficient reports max_sweeps + 1, the others meet 1e-12 vs dgbsv."""
+    import torch
+    sizes = [301, 5000, 2100, 777]
+    off, n, total = gpu.uneven_layout(sizes)
+    _say("uneven-refined-nan", sizes)
+    dev = gpu.generate_uneven(0, 23, off, n)
+    ins = [_np(t).copy() for t in dev]
+    dev[2][int(off[1]) + 2500] = float("nan")          # c of system 1
+    sweeps = torch.full((len(n),), -1, dtype=torch.int32, device="cuda")
+    gpu.solve_uneven_refined(*dev, off, n, max_sweeps=4, tol=1e-12, sweeps=sweeps)
+    torch.cuda.synchronize()
+    sw, x = _np(sweeps), _np(dev[5])
+    assert sw[1] == 5, sw
+    for j in (0, 2, 3):
+        assert 1 <= sw[j] <= 3, sw
+        s = slice(int(off[j]), int(off[j]) + int(n[j]))
+        ref = oracle_lib.gbsv(*[t[s] for t in ins])
+        assert np.abs(x[s] - ref).max() / np.abs(ref).max() <= 1e-12, j
+
+
 def test_uneven_refined_flags_nonconvergence(gpu, oracle_lib):
     """max_sweeps = 0 returns the bare FP32 solve, flagged (sweeps = 1)."""
     sizes = [10, 3000, 5000]
