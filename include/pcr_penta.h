@@ -128,7 +128,9 @@ int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
  * persistent work-queue LDS launch. Results equal solving each system alone
  * with pcrp_solve_f64 (bit for bit). No counterpart in the reference, whose
  * driver runs one (m, n) shape per launch (main.cu:245-260).
- * The plan tables are copied to the workspace on `stream` each call.
+ * The plan tables are copied to the workspace on `stream` each call, from a
+ * host buffer that lives only during the call (so the call cannot be
+ * captured in a graph).
  */
 size_t pcrp_uneven_workspace_size(const int64_t *h_offsets, const int *h_n, int m,
                                   int precision);
