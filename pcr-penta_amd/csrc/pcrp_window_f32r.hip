@@ -63,6 +63,7 @@ struct WSys {
     long long off;   // element of row 0 (row i at off + i * rs)
     long long w0;    // first window of the system (uneven batches)
     int n, nb, nw, shift;
+    int T, pad;      // interior block rows per window (balanced: ceil(nb / nw) <= kWinT)
 };
 
 struct WArgs {
@@ -171,9 +172,13 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
         if (A.stop[j]) continue;   // uniform over the workgroup
         const WSys S = A.sys[j];
         const bool full = S.nw == 1;
-        const int kb = full ? 0 : wi * kWinT - kWinH;   // block row of window slot 0
-        const int k0 = full ? 0 : wi * kWinT;           // interior block rows [k0, k1)
-        const int k1 = full ? S.nb : min(S.nb, k0 + kWinT);
+        const int kb = full ? 0 : wi * S.T - kWinH;     // block row of window slot 0
+        const int k0 = full ? 0 : wi * S.T;             // interior block rows [k0, k1)
+        const int k1 = full ? S.nb : min(S.nb, k0 + S.T);
+        // block rows the window holds: the interior and its halos, inside the
+        // system (the slots past them are idle)
+        const int klo = full ? 0 : max(0, k0 - kWinH);
+        const int khi = full ? S.nb : min(S.nb, k1 + kWinH);
         int lmax = 0;
         if (full) { while ((1 << lmax) < S.nb) ++lmax; } else { lmax = kWinL; }
 
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
         for (int q = 0; q < BR; ++q) {
             const int K = kb + t + NT * q;
             xv[q][0] = xv[q][1] = 0.0;
-            if (K < 0 || K >= S.nb) {
+            if (K < klo || K >= khi) {
                 r[q] = identity_brow();
                 continue;
             }
@@ -219,22 +224,23 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
         int s = 0;
         for (; s < lmax; ++s) {
             const int delta = 1 << s;
-            // rows outside the system stay identity rows and are nobody's
+            // slots outside [klo, khi) stay identity rows and are nobody's
             // neighbour: skipped (whole waves of a short system's window)
 #pragma unroll
             for (int q = 0; q < BR; ++q) {
                 const int K = kb + t + NT * q;
-                if (K >= 0 && K < S.nb) put_rec(sR, t + NT * q, minv(r[q].D), r[q]);
+                if (K >= klo && K < khi) put_rec(sR, t + NT * q, minv(r[q].D), r[q]);
             }
             __syncthreads();
             float am = 0.f;
 #pragma unroll
             for (int q = 0; q < BR; ++q) {
                 const int l = t + NT * q, K = kb + l;
-                if (K < 0 || K >= S.nb) continue;
+                if (K < klo || K >= khi) continue;
                 // neighbours outside the system or the window are absent
-                const bool hl = l - delta >= 0 && K - delta >= 0;
-                const bool hr = l + delta < NB && K + delta < S.nb;
+                // (klo >= kb and khi <= kb + NB: the slots exist)
+                const bool hl = K - delta >= klo;
+                const bool hr = K + delta < khi;
                 float ak = 0.f;
                 r[q] = level(r[q], sR, hl, hl ? l - delta : l, hr, hr ? l + delta : l, ak);
                 if (FIND && K >= k0 && K < k1) am = fmaxf(am, ak);
@@ -317,8 +323,8 @@ __global__ __launch_bounds__(256) void wcopy_kernel(WArgs A, const double* __res
         const double* __restrict__ x = (A.st[j].last & 1) ? x1 : x0;
         const WSys S = A.sys[j];
         const bool full = S.nw == 1;
-        const int k0 = full ? 0 : wi * kWinT;
-        const int k1 = full ? S.nb : min(S.nb, k0 + kWinT);
+        const int k0 = full ? 0 : wi * S.T;
+        const int k1 = full ? S.nb : min(S.nb, k0 + S.T);
         const long long i0 = max(0LL, 2LL * k0 - S.shift);
         const long long i1 = min((long long)S.n, 2LL * k1 - S.shift);
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -440,6 +446,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         s.shift = rs == 1 ? (int)(s.off & 1) : 0;
         s.nb = (s.n + s.shift + 1) / 2;
         s.nw = windows_for(s.nb);
+        s.T = (s.nb + s.nw - 1) / s.nw;
         s.w0 = nwin;
         nwin += s.nw;
     }
