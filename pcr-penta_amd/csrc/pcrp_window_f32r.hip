@@ -13,8 +13,9 @@
 // Polizzi & Sameh, Parallel Computing 32 (2006) 177-194):
 //   * a system of nb <= kWinNB block rows is one window, every level it needs
 //     (exact, like the one-workgroup kernel);
-//   * a longer system is cut into windows of kWinT interior block rows with
-//     kWinH = 2^kWinL - 1 halo block rows on each side; a window runs at most
+//   * a longer system is cut into windows of at most kWinT interior block
+//     rows (balanced: ceil(nb / windows)) with kWinH = 2^kWinL - 1 halo block
+//     rows on each side; a window runs at most
 //     kWinL levels with neighbours outside the window treated as absent, so
 //     the values at its interior rows equal those of the whole-system PCR
 //     (the halo rows absorb the edge). A window whose interior multipliers are
