@@ -95,9 +95,11 @@ __host__ __device__ inline int windows_for(int nb)
 }
 
 // Window w -> (system j, window index wi). Uniform batches are window-major
-// (the m systems' windows at one position are consecutive: in the interleaved
-// layout their rows share cache lines) with runs of 16 on one XCD; uneven
-// batches are system-major (a binary search on the first-window table).
+// (the m systems' windows at one position are consecutive) with runs of 16
+// on one XCD: on the n = 2^20, m = 32 batch 7% faster than system-major
+// order with runs of 8 consecutive windows of a system per XCD (2.5% slower
+// at n = 2048, m = 16384). Uneven batches are system-major (a binary search
+// on the first-window table).
 __device__ __forceinline__ void window_of(const WArgs& A, int w, int& j, int& wi)
 {
     if (A.uniform) {
