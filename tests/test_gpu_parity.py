@@ -608,10 +608,11 @@ def test_refined_split_under_graph_capture(gpu, oracle_lib):
 
 @pytest.mark.parametrize("n", [2049, 2700, 2701, 2702, 3601, 3603, 4500])
 def test_refined_window_edges(gpu, oracle_lib, n):
-    """Long systems cut into windows (450 interior block rows, 31 halo block
-    rows per side): sizes with one block row past a window edge, exact
-    multiples, odd n; both generators and layouts (the interleaved batch is
-    transposed first), m not a multiple of 32."""
+    """Long systems cut into windows (at most 450 interior block rows,
+    balanced over the system's windows; 31 halo block rows per side): sizes
+    with one block row past a window edge, exact multiples, odd n; both
+    generators and layouts (the interleaved batch is transposed first), m not
+    a multiple of 32."""
     m = 5
     for kind, layout in ((0, I), (1, C)):
         _say("refined-window", n, m, kind, layout)
