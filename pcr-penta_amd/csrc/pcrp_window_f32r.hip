@@ -79,6 +79,7 @@ struct WArgs {
     int* nonlocal;
     unsigned char* lev;   // levels each window runs (found by the first sweep)
     int m, nwin, uniform;
+    int nwu;   // uniform batches: windows per system
     int vec;   // rs == 1 and every array 16-byte aligned
 };
 
@@ -94,18 +95,26 @@ __host__ __device__ inline int windows_for(int nb)
     return nb <= kWinNB ? 1 : (nb + kWinT - 1) / kWinT;
 }
 
-// Window w -> (system j, window index wi). Uniform batches are window-major
-// (the m systems' windows at one position are consecutive) with runs of 16
-// on one XCD: on the n = 2^20, m = 32 batch 7% faster than system-major
-// order with runs of 8 consecutive windows of a system per XCD (2.5% slower
-// at n = 2048, m = 16384). Uneven batches are system-major (a binary search
-// on the first-window table).
+// Window w -> (system j, window index wi). Uniform batches of systems with
+// many windows are window-major (the m systems' windows at one position are
+// consecutive) with runs of 16 on one XCD: on the n = 2^20, m = 32 batch 7%
+// faster than system-major order; systems of at most 8 windows go
+// system-major with runs of 8 on one XCD (neighbouring windows share halo
+// rows in that XCD's L2: 2.5% faster at n = 2048, m = 16384). Uneven batches
+// are system-major (a binary search on the first-window table).
 __device__ __forceinline__ void window_of(const WArgs& A, int w, int& j, int& wi)
 {
     if (A.uniform) {
-        const int v = xcd_remap_c<16>(w, A.nwin);
-        j = v % A.m;
-        wi = v / A.m;
+        const int nw = A.nwu;
+        if (nw <= 8) {
+            const int v = xcd_remap_c<8>(w, A.nwin);
+            j = v / nw;
+            wi = v - j * nw;
+        } else {
+            const int v = xcd_remap_c<16>(w, A.nwin);
+            j = v % A.m;
+            wi = v / A.m;
+        }
         return;
     }
     int lo = 0, hi = A.m - 1;
@@ -485,7 +494,8 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     const double* ca = transposed ? six : a;
     WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
             transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr,
-            nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0, 0};
+            nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0,
+            uniform ? sys[0].nw : 0, 0};
     {
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
