@@ -31,13 +31,15 @@
 // x' = x + dx (FP64) into the other x buffer (windows read x at their halo
 // rows, so x is double buffered), max |dx| and max |x'| per system by atomic
 // max; then the stopping rule of the other refinement paths (decide_kernel).
-// Both x buffers live in the workspace (sweep it writes buffer it & 1), so
-// the caller's y stays the right-hand side of every sweep, and each system's
-// x is copied to y at the end from the buffer its last sweep wrote: per
-// unknown, first sweep 48 B read + 8 B written, later sweeps 56 B read + 8 B
-// written, the final copy 16 B. Interleaved uniform batches are first copied
-// to system-contiguous arrays (96 B per unknown, LDS-tiled transpose), swept
-// there, and x is transposed back (16 B).
+// Direct form (uneven and contiguous batches): sweep it writes x0, x1 or y
+// for it % 3 = 0, 1, 2; the first sweep reads y and copies it to y0, later
+// sweeps read the right-hand side from y0, so the usual two corrections end
+// in y and other systems are copied there from the buffer their last sweep
+// wrote: per unknown, first sweep 48 B read + 16 B written, later sweeps
+// 56 B read + 8 B written. Interleaved uniform batches are first copied to
+// system-contiguous arrays (96 B per unknown, LDS-tiled transpose), swept
+// there with x alternating between x0 and x1, and x is transposed back
+// (16 B).
 #include "pcr_penta.h"
 #include "pcrp_block_f32.h"
 #include "pcrp_internal.h"
@@ -69,7 +71,8 @@ struct WSys {
 
 struct WArgs {
     const double *a, *b, *c, *d, *e;
-    const double* rhs;   // y (or its system-contiguous copy)
+    const double* rhs;   // right-hand side: y, its copy y0, or its system-contiguous copy
+    double* y0;          // first sweep of the direct form: receives y at the interior rows
     const double* xin;   // x of the previous sweep (null in the first)
     double* xout;        // x of this sweep
     long long rs;
@@ -159,6 +162,8 @@ __device__ __forceinline__ void load_pair_vec(const WArgs& A, long long g0, doub
         ax = __builtin_fma(ev.y, xp.y, ax);
         rw[1][5] = yv.y - ax;
     } else {
+        xv[0] = yv.x;   // the first sweep copies y to y0
+        xv[1] = yv.y;
         rw[0][5] = yv.x;
         rw[1][5] = yv.y;
     }
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
         if (full) { while ((1 << lmax) < S.nb) ++lmax; } else { lmax = kWinL; }
 
         BRow r[BR];
-        double xv[BR][2];   // x of the previous sweep at the rows (first sweep: 0)
+        double xv[BR][2];   // x of the previous sweep at the rows (first sweep: y)
 #pragma unroll
         for (int q = 0; q < BR; ++q) {
             const int K = kb + t + NT * q;
@@ -221,6 +226,8 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                         xv[q][h] = A.xin[g];
                         rw[h][5] = residual_row(A.a, A.b, A.c, A.d, A.e, A.rhs, A.xin, g, i, S.n,
                                                 A.rs);
+                    } else {
+                        xv[q][h] = rw[h][5];   // the first sweep copies y to y0
                     }
                 } else {   // identity row before row 0 or past row n-1
                     rw[h][0] = rw[h][1] = rw[h][3] = rw[h][4] = rw[h][5] = 0.0;
@@ -290,6 +297,7 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
                         const long long g = S.off + i * A.rs;
                         const double dx = (double)dd[h];
                         const double xn = A.xin ? xv[q][h] + dx : dx;
+                        if (!A.xin && A.y0) A.y0[g] = xv[q][h];
                         A.xout[g] = xn;
                         const unsigned long long bd = abs_bits(dx), bx = abs_bits(xn);
                         dmax = bd > dmax ? bd : dmax;
@@ -323,8 +331,8 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
     }
 }
 
-// x of every system to y, from the buffer its last sweep wrote (even sweep
-// index: x0, odd: x1), over the window's interior rows.
+// x of every system to y from the buffer its last sweep wrote (sweep it
+// writes x0, x1, y for it % 3 = 0, 1, 2), over the window's interior rows.
 __global__ __launch_bounds__(256) void wcopy_kernel(WArgs A, const double* __restrict__ x0,
                                                     const double* __restrict__ x1,
                                                     double* __restrict__ y)
@@ -332,7 +340,9 @@ __global__ __launch_bounds__(256) void wcopy_kernel(WArgs A, const double* __res
     for (int w = blockIdx.x; w < A.nwin; w += gridDim.x) {
         int j, wi;
         window_of(A, w, j, wi);
-        const double* __restrict__ x = (A.st[j].last & 1) ? x1 : x0;
+        const int b = A.st[j].last % 3;
+        if (b == 2) continue;   // already in y
+        const double* __restrict__ x = b == 1 ? x1 : x0;
         const WSys S = A.sys[j];
         const bool full = S.nw == 1;
         const int k0 = full ? 0 : wi * S.T;
@@ -413,7 +423,7 @@ size_t lev_capacity(long long elems, int m)
 // Direct: x buffer and copy of y (16 B per element). Transposed (interleaved
 // uniform batches): six system-contiguous arrays and two x buffers (64 B).
 struct WLayout {
-    size_t state, stop, flag, sys, lev, x0, x1, six, bytes;
+    size_t state, stop, flag, sys, lev, x0, x1, y0, six, bytes;
 };
 
 WLayout wlayout(long long elems, int m, bool transposed)
@@ -427,6 +437,7 @@ WLayout wlayout(long long elems, int m, bool transposed)
     L.lev = o;   o = align_up(o + lev_capacity(elems, m));
     L.x0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
     L.x1 = o;    o = align_up(o + (size_t)elems * sizeof(double));
+    L.y0 = o;    if (!transposed) o = align_up(o + (size_t)elems * sizeof(double));
     L.six = o;   if (transposed) o = align_up(o + 6 * (size_t)elems * sizeof(double));
     L.bytes = o;
     return L;
@@ -473,6 +484,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     WSys* dsys = reinterpret_cast<WSys*>(w + L.sys);
     double* x0 = reinterpret_cast<double*>(w + L.x0);
     double* x1 = reinterpret_cast<double*>(w + L.x1);
+    double* y0 = reinterpret_cast<double*>(w + L.y0);
     double* six = reinterpret_cast<double*>(w + L.six);
     unsigned char* lev = reinterpret_cast<unsigned char*>(w + L.lev);
     if (hipMemcpyAsync(dsys, sys.data(), sys.size() * sizeof(WSys), hipMemcpyHostToDevice, st) !=
@@ -493,21 +505,25 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     // transposed: A and y from the system-contiguous copies
     const double* ca = transposed ? six : a;
     WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
-            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr,
+            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
             nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0,
             uniform ? sys[0].nw : 0, 0};
     {
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
-                al16(y) && al16(x0) && al16(x1) && al16(six) ? 1 : 0;
+                al16(y) && al16(x0) && al16(x1) && al16(y0) && al16(six) ? 1 : 0;
     }
-    // both x buffers in the workspace: y (or its transposed copy) stays the
-    // right-hand side of every sweep
-    double* const xbuf[2] = {x0, x1};
+    // Direct form: sweep it writes x0, x1, y for it % 3 = 0, 1, 2; the first
+    // sweep reads y and copies it to y0, later sweeps read y0, so the usual
+    // two corrections end in y with no final copy. Transposed form: x0, x1
+    // alternate and the right-hand side is the transposed copy of y.
+    double* const xbuf[3] = {x0, x1, transposed ? x0 : y};
+    const int cyc = transposed ? 2 : 3;
     for (int it = 0; it <= max_sweeps; ++it) {
-        A.rhs = transposed ? six + 5 * N : y;
-        A.xin = it == 0 ? nullptr : xbuf[(it - 1) & 1];
-        A.xout = xbuf[it & 1];
+        A.rhs = transposed ? six + 5 * N : (it == 0 ? y : y0);
+        A.y0 = transposed || it > 0 ? nullptr : y0;
+        A.xin = it == 0 ? nullptr : xbuf[(it - 1) % cyc];
+        A.xout = xbuf[it % cyc];
         if (it == 0)
             hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR, true>), dim3((unsigned)nwin),
                                dim3(kWinNT), 0, st, A);
