@@ -21,7 +21,8 @@
 //     (the halo rows absorb the edge). A window whose interior multipliers are
 //     still above kTruncF32 after kWinL levels sets `nonlocal`, and the batch
 //     is solved by the pass/stage path instead (host read-back after the
-//     first sweep; results of the window path are discarded, y is untouched).
+//     first sweep, which writes only workspace buffers: y is still the
+//     right-hand side).
 //     A stream under capture cannot read back: the callers then use the
 //     pass/stage path directly.
 // One sweep, per window: r = y - A x (FP64, rows of the window from global
