@@ -574,6 +574,21 @@ def test_uneven_refined_falls_back_on_slow_decay(gpu, oracle_lib):
         assert np.abs(x[s] - ref).max() / np.abs(ref).max() <= 1e-12, (int(nj), _np(sweeps))
 
 
+@pytest.mark.parametrize("kind,n,m", [(1, 1 << 20, 32), (0, 2048, 16384)])
+def test_refined_full_benchmark_shapes(gpu, oracle_lib, kind, n, m):
+    """FP32 refinement at the full benchmark shapes (n = 2^20, m = 32, weakly
+    dominant; n = 2048, m = 16384, strictly dominant; interleaved, window
+    path): every system within 1e-12 of dgbsv in two or three corrections."""
+    _say("refined-full", kind, n, m)
+    dev = gpu.generate(kind, 0, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    x, sw = _refined(gpu, dev, n, m, I)
+    ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+    err = oracle_lib.normwise_relerr(x, ref, n, m, I)
+    print("full-shape refined vs dgbsv", err, np.bincount(sw), flush=True)
+    assert err <= 1e-12 and sw.min() >= 1 and sw.max() <= 3
+
+
 def test_refined_split_under_graph_capture(gpu, oracle_lib):
     """A long-system FP32 refinement captured in a graph takes the pass/stage
     path (the window path's host read-back cannot be captured): three
