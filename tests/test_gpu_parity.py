@@ -412,6 +412,27 @@ def test_uneven_generator_matches_batch_generator(gpu, oracle_lib):
             assert np.array_equal(_np(u)[o:o + nj], v)
 
 
+@pytest.mark.parametrize("kind,n,m", [(1, 1 << 20, 32), (0, 2048, 16384)])
+def test_f64_full_benchmark_shapes_bit_exact(gpu, oracle_lib, kind, n, m):
+    """Plain FP64 at the full benchmark shapes (n = 2^20, m = 32 weakly
+    dominant; n = 2048, m = 16384; interleaved, split path): every system
+    equal to the oracle's PCR-Penta bit for bit."""
+    _say("f64-full", kind, n, m)
+    dev = gpu.generate(kind, 0, n, m, I)
+    ins = [_np(t).copy() for t in dev]
+    x = _run_f64(gpu, dev, n, m, I)
+    assert np.array_equal(x, oracle_lib.pcr_batch(*ins, n, m, I))
+
+
+def test_uneven_f64_full_batch_bit_exact(gpu, oracle_lib):
+    """The whole uneven benchmark batch (10000 systems, n log-uniform in
+    [64, 131072], 1.67e8 unknowns) equals the oracle bit for bit."""
+    sizes = gpu.uneven_log_uniform_sizes()
+    _say("uneven-f64-full", len(sizes), int(sizes.sum()))
+    x, ref, ins, dev = _uneven_case(gpu, oracle_lib, sizes, 0, 1)
+    assert np.array_equal(x, ref)
+
+
 def test_uneven_bit_exact_mixed_sizes(gpu, oracle_lib):
     rng = np.random.default_rng(11)
     sizes = [1, 2, 3, 5, 64, 100, 1000, 1024, 1025, 2048, 3000, 5000, 16385]
