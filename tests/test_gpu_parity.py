@@ -542,6 +542,16 @@ def test_uneven_refined_log_uniform_sample(gpu, oracle_lib):
     assert errs.max() <= 1e-12 and sw.max() <= 3
 
 
+def test_uneven_refined_full_batch(gpu, oracle_lib):
+    """The whole uneven benchmark batch (10000 systems, n log-uniform in
+    [64, 131072]), FP32 + refinement: every system within 1e-12 of dgbsv."""
+    sizes = gpu.uneven_log_uniform_sizes()
+    _say("uneven-refined-full", len(sizes), int(sizes.sum()))
+    errs, sw = _uneven_refined_case(gpu, oracle_lib, sizes, 0, 1)
+    print("full uneven refined: max err", errs.max(), "sweeps", np.bincount(sw), flush=True)
+    assert errs.max() <= 1e-12 and sw.min() >= 1 and sw.max() <= 3
+
+
 def _slow_decay_rows(n, seed):
     """Rows of c = 2.05, b = d = -1, a = e = 0 (a shifted 1-D Laplacian,
     condition ~80): its block multipliers fall below 2^-20 only after 6
