@@ -180,6 +180,11 @@ int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t *d_offsets, cons
  * replays it (needs a non-NULL stream; the call then synchronises `stream`
  * before releasing the graph); use_graph = 0 enqueues every step.
  */
+/* Refinement settings of the FP32 line solves inside pcrp_heston_solve:
+ * normwise tolerance and maximum corrections per line solve. */
+#define PCRP_HESTON_REFINE_TOL 1e-12
+#define PCRP_HESTON_MAX_SWEEPS 8
+
 typedef struct {
     int ns, nv;                 /* unknowns in S and in v */
     double smax, vmax;          /* grid extents */

@@ -320,6 +320,10 @@ __device__ __forceinline__ void bstage_solve(const BRows& in, int nb, InAt in_at
         atomicMax(&st->dmax, dmax);
         atomicMax(&st->xmax, xmax);
     }
+    // red is read by thread 0 above; a caller's next sub-system of one block
+    // row runs no level, so no barrier would separate its red writes from
+    // that read
+    __syncthreads();
 }
 
 // stop[0 .. m-1]: per-system stop flags; stop[m]: systems still running, so

@@ -293,8 +293,8 @@ int solve_lines(int precision, const double* a, const double* b, const double* c
 {
     if (precision == PCRP_PREC_F64)
         return pcrp_solve_f64(a, b, c, d, e, y, n, m, layout, h.solver_ws, h.solver_bytes, st);
-    return pcrp_solve_f32r_f64(a, b, c, d, e, y, n, m, layout, 8, 1e-13, nullptr, h.solver_ws,
-                               h.solver_bytes, st);
+    return pcrp_solve_f32r_f64(a, b, c, d, e, y, n, m, layout, PCRP_HESTON_MAX_SWEEPS,
+                               PCRP_HESTON_REFINE_TOL, nullptr, h.solver_ws, h.solver_bytes, st);
 }
 
 // one Douglas step: four launches, fixed pointers

@@ -328,7 +328,11 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
             atomicMax(&A.st[j].dmax, dmax);
             atomicMax(&A.st[j].xmax, xmax);
         }
-        // (sR, flg and red are next written behind the next window's barriers)
+        // Thread 0 reads red above while the other waves may already be at
+        // the next window; a window of one block row runs no level (no
+        // barrier before its red writes), so red, sR and flg are released
+        // here, once per window.
+        __syncthreads();
     }
 }
 

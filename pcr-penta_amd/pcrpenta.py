@@ -151,15 +151,26 @@ def _workspace(n, m, precision, device):
     return ws, nb
 
 
+def _keep_until_done(ws, stream):
+    """A workspace allocated here on torch's current stream but used on another
+    `stream` stays reserved by the caching allocator until that stream's work
+    has completed (the solve is asynchronous; ws is dropped on return)."""
+    if stream is not None:
+        ws.record_stream(stream)
+
+
 def solve(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, stream=None, ws=None):
     """Plain FP64 PCR-Penta: y <- A^{-1} y for m systems of size n (in place)."""
     _validate([a, b, c, d, e, y], n, m)
-    if ws is None:
+    own = ws is None
+    if own:
         ws, nb = _workspace(n, m, PREC_F64, y.device)
     else:
         nb = ws.numel()
     st = load().pcrp_solve_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], n, m, layout,
                                _ptr(ws), nb, _stream(stream))
+    if own:
+        _keep_until_done(ws, stream)
     _check(st, "pcrp_solve_f64")
     return y
 
@@ -169,13 +180,16 @@ def solve_refined(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, m
     """FP32 PCR-Penta + FP64 iterative refinement (in place on y).
     ``sweeps``: optional int32 device tensor of m entries for per-system sweep counts."""
     _validate([a, b, c, d, e, y], n, m)
-    if ws is None:
+    own = ws is None
+    if own:
         ws, nb = _workspace(n, m, PREC_F32_REFINE, y.device)
     else:
         nb = ws.numel()
     sp = _ptr(sweeps) if sweeps is not None else None
     st = load().pcrp_solve_f32r_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], n, m, layout,
                                     max_sweeps, tol, sp, _ptr(ws), nb, _stream(stream))
+    if own:
+        _keep_until_done(ws, stream)
     _check(st, "pcrp_solve_f32r_f64")
     return y
 
@@ -311,11 +325,14 @@ def solve_uneven(a, b, c, d, e, y, offsets, sizes, stream=None, ws=None):
     for t in (a, b, c, d, e, y):
         if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
             raise TypeError("pcrpenta: arrays must be contiguous float64 device tensors")
-    if ws is None:
+    own = ws is None
+    if own:
         nb = uneven_workspace_size(off, n)
         ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y.device)
     st = load().pcrp_solve_uneven_f64(*[_ptr(t) for t in (a, b, c, d, e, y)], off.ctypes.data,
                                       n.ctypes.data, len(n), _ptr(ws), ws.numel(), _stream(stream))
+    if own:
+        _keep_until_done(ws, stream)
     _check(st, "pcrp_solve_uneven_f64")
     return y
 
@@ -331,13 +348,16 @@ def solve_uneven_refined(a, b, c, d, e, y, offsets, sizes, max_sweeps: int = 8,
     for t in (a, b, c, d, e, y):
         if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
             raise TypeError("pcrpenta: arrays must be contiguous float64 device tensors")
-    if ws is None:
+    own = ws is None
+    if own:
         nb = uneven_workspace_size(off, n, PREC_F32_REFINE)
         ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y.device)
     sp = _ptr(sweeps) if sweeps is not None else None
     st = load().pcrp_solve_uneven_f32r_f64(*[_ptr(t) for t in (a, b, c, d, e, y)],
                                            off.ctypes.data, n.ctypes.data, len(n), max_sweeps,
                                            tol, sp, _ptr(ws), ws.numel(), _stream(stream))
+    if own:
+        _keep_until_done(ws, stream)
     _check(st, "pcrp_solve_uneven_f32r_f64")
     return y
 
@@ -390,6 +410,11 @@ class HestonParams(ctypes.Structure):
                 ("theta_scheme", ctypes.c_double)]
 
 
+# refinement settings of the FP32 line solves in the Heston driver
+# (PCRP_HESTON_REFINE_TOL, PCRP_HESTON_MAX_SWEEPS in include/pcr_penta.h)
+HESTON_REFINE_TOL = 1e-12
+HESTON_MAX_SWEEPS = 8
+
 # Heston ADI problem: Ns = 1024, Nv = 512 unknowns, 200 steps of 1/200
 HESTON_GRID = dict(ns=1024, nv=512, smax=800.0, vmax=5.0, kappa=3.0, theta_v=0.04, sigma=0.15,
                rho=-0.7, r=0.03, strike=100.0, dt=1.0 / 200, steps=200, theta_scheme=0.5)
@@ -409,7 +434,8 @@ def heston(params: HestonParams, precision: int = PREC_F64, use_graph: bool = Tr
     nb = int(lib.pcrp_heston_workspace_size(ctypes.byref(params), precision))
     if nb == 0:
         raise PCRPError(1, "pcrp_heston_workspace_size")
-    if ws is None:
+    own = ws is None
+    if own:
         ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
     if u is None:
         u = torch.empty(params.ns * params.nv, dtype=torch.float64, device="cuda")
@@ -417,5 +443,7 @@ def heston(params: HestonParams, precision: int = PREC_F64, use_graph: bool = Tr
         raise ValueError("graph capture needs a non-default stream")
     st = lib.pcrp_heston_solve(ctypes.byref(params), precision, int(use_graph), _ptr(u), _ptr(ws),
                                ws.numel(), _stream(stream))
+    if own:
+        _keep_until_done(ws, stream)
     _check(st, "pcrp_heston_solve")
     return u

@@ -65,12 +65,14 @@ def padded(p, U, tau, interior=True):
     return G
 
 
-def adi(p, oracle, precision_solver=None):
-    """Run the Douglas scheme; returns U (nv x ns)."""
+def line_systems(p):
+    """Diagonals of the S-line systems (I - th dt A1) and of the v-line systems
+    (I - th dt A2), each as five (nv, ns) arrays: row j of the S arrays is S-line
+    j (contiguous, m = nv systems of n = ns), column i of the v arrays is v-line i
+    (interleaved, m = ns systems of n = nv)."""
     ns, nv, dt, th = p["ns"], p["nv"], p["dt"], p["theta_scheme"]
     hS, hv, S, v, w1, w2 = coefs(p)
     f = -th * dt
-    # line systems
     sa = np.where(np.arange(1, ns + 1)[None, :] - 2 >= 1, f * w1[0], 0.0)
     sb = np.where(np.arange(1, ns + 1)[None, :] - 1 >= 1, f * w1[1], 0.0)
     sc = 1.0 + f * w1[2]
@@ -82,30 +84,52 @@ def adi(p, oracle, precision_solver=None):
     vc = 1.0 + f * w2[2][:, None] * np.ones((1, ns))
     vd = np.where(jv + 1 <= nv - 1, f * w2[3][:, None], 0.0)
     ve = np.where(jv + 2 <= nv - 1, f * w2[4][:, None], 0.0)
-    U = np.maximum(S[None, :] - p["strike"], 0.0) * np.ones((nv, 1))
+    return (sa, sb, sc, sd, se), (va, vb, vc, vd, ve)
+
+
+def payoff(p):
+    hS, hv, S, v, w1, w2 = coefs(p)
+    return np.maximum(S[None, :] - p["strike"], 0.0) * np.ones((p["nv"], 1))
+
+
+def explicit(p, U, n):
+    """Explicit stage of Douglas step n: (R1, W), the right-hand side of the
+    S-line systems and the v-line correction (X = Y1 + W is the v-line
+    right-hand side)."""
+    ns, nv, dt, th = p["ns"], p["nv"], p["dt"], p["theta_scheme"]
+    hS, hv, S, v, w1, w2 = coefs(p)
+    t0, t1 = n * dt, (n + 1) * dt
+    G0 = padded(p, U, t0)
+    B1 = padded(p, None, t1, interior=False)   # boundary nodes only, at t1
+
+    def shiftS(G, o):
+        return G[2:nv + 2, 2 + o:ns + 2 + o]
+
+    def shiftV(G, o):
+        return G[2 + o:nv + 2 + o, 2:ns + 2]
+    l1 = sum(w1[k] * shiftS(G0, k - 2) for k in range(5))
+    l2 = sum(w2[k][:, None] * shiftV(G0, k - 2) for k in range(5))
+    # mixed term (zero at v = 0)
+    c0 = p["rho"] * p["sigma"] * v[:, None] * S[None, :] / (4 * hS * hv)
+    l0 = c0 * (G0[3:nv + 3, 3:ns + 3] - G0[1:nv + 1, 3:ns + 3]
+               - G0[3:nv + 3, 1:ns + 1] + G0[1:nv + 1, 1:ns + 1])
+    l0[0] = 0.0
+    y0 = U + dt * (l0 + l1 + l2)
+    # boundary parts at t1: S = 0 contributes 0, S = Smax and v = vmax nodes
+    b1 = sum(w1[k] * shiftS(B1, k - 2) for k in range(5))
+    b2 = sum(w2[k][:, None] * shiftV(B1, k - 2) for k in range(3, 5))
+    R1 = y0 - th * dt * l1 + th * dt * b1
+    W = -th * dt * l2 + th * dt * b2
+    return R1, W
+
+
+def adi(p, oracle, precision_solver=None):
+    """Run the Douglas scheme; returns U (nv x ns)."""
+    ns, nv = p["ns"], p["nv"]
+    (sa, sb, sc, sd, se), (va, vb, vc, vd, ve) = line_systems(p)
+    U = payoff(p)
     for n in range(p["steps"]):
-        t0, t1 = n * dt, (n + 1) * dt
-        G0 = padded(p, U, t0)
-        B1 = padded(p, None, t1, interior=False)   # boundary nodes only, at t1
-
-        def shiftS(G, o):
-            return G[2:nv + 2, 2 + o:ns + 2 + o]
-
-        def shiftV(G, o):
-            return G[2 + o:nv + 2 + o, 2:ns + 2]
-        l1 = sum(w1[k] * shiftS(G0, k - 2) for k in range(5))
-        l2 = sum(w2[k][:, None] * shiftV(G0, k - 2) for k in range(5))
-        # mixed term (zero at v = 0)
-        c0 = p["rho"] * p["sigma"] * v[:, None] * S[None, :] / (4 * hS * hv)
-        l0 = c0 * (G0[3:nv + 3, 3:ns + 3] - G0[1:nv + 1, 3:ns + 3]
-                   - G0[3:nv + 3, 1:ns + 1] + G0[1:nv + 1, 1:ns + 1])
-        l0[0] = 0.0
-        y0 = U + dt * (l0 + l1 + l2)
-        # boundary parts at t1: S = 0 contributes 0, S = Smax and v = vmax nodes
-        b1 = sum(w1[k] * shiftS(B1, k - 2) for k in range(5))
-        b2 = sum(w2[k][:, None] * shiftV(B1, k - 2) for k in range(3, 5))
-        R1 = y0 - th * dt * l1 + th * dt * b1
-        W = -th * dt * l2 + th * dt * b2
+        R1, W = explicit(p, U, n)
         Y1 = oracle.pcr_batch(sa.ravel(), sb.ravel(), sc.ravel(), sd.ravel(), se.ravel(),
                               R1.ravel(), ns, nv, oracle.CONTIGUOUS).reshape(nv, ns)
         X = Y1 + W

@@ -412,11 +412,12 @@ def test_uneven_generator_matches_batch_generator(gpu, oracle_lib):
             assert np.array_equal(_np(u)[o:o + nj], v)
 
 
-@pytest.mark.parametrize("kind,n,m", [(1, 1 << 20, 32), (0, 2048, 16384)])
+@pytest.mark.parametrize("kind,n,m", [(1, 1 << 20, 32), (0, 2048, 16384), (0, 1024, 16384)])
 def test_f64_full_benchmark_shapes_bit_exact(gpu, oracle_lib, kind, n, m):
     """Plain FP64 at the full benchmark shapes (n = 2^20, m = 32 weakly
-    dominant; n = 2048, m = 16384; interleaved, split path): every system
-    equal to the oracle's PCR-Penta bit for bit."""
+    dominant; n = 2048, m = 16384, split path; n = 1024, m = 16384, the LDS
+    kernel's own benchmark batch; interleaved): every system equal to the
+    oracle's PCR-Penta bit for bit."""
     _say("f64-full", kind, n, m)
     dev = gpu.generate(kind, 0, n, m, I)
     ins = [_np(t).copy() for t in dev]
@@ -522,6 +523,53 @@ def test_uneven_refined_flags_nan_system(gpu, oracle_lib):
         s = slice(int(off[j]), int(off[j]) + int(n[j]))
         ref = oracle_lib.gbsv(*[t[s] for t in ins])
         assert np.abs(x[s] - ref).max() / np.abs(ref).max() <= 1e-12, j
+
+
+def test_uneven_refined_tiny_windows_on_resident_grid(gpu, oracle_lib):
+    """Thousands of 1- and 2-row systems among long ones, with a tolerance so
+    strict that sweeps 3+ run on the resident grid (a workgroup walks several
+    windows, and a window of one block row runs no level): every system within
+    1e-12 of dgbsv, and results and sweep counts equal those of solving the
+    system alone (the per-window max reduction must not mix windows)."""
+    import torch
+    sizes = []
+    for k in range(3000):
+        sizes.append(1 + (k % 2))
+        if k % 300 == 0:
+            sizes.append(5000 + k)
+    off, n, total = gpu.uneven_layout(sizes)
+    _say("uneven-refined-tiny", len(sizes), total)
+    dev = gpu.generate_uneven(0, 77, off, n)
+    ins = [_np(t).copy() for t in dev]
+    # tol below what two corrections reach (~1e-13 relative corrections): systems
+    # stop at sweep 3 or later, or run out
+    tol, ms = 1e-20, 6
+    sweeps = torch.full((len(n),), -1, dtype=torch.int32, device="cuda")
+    gpu.solve_uneven_refined(*dev, off, n, max_sweeps=ms, tol=tol, sweeps=sweeps)
+    torch.cuda.synchronize()
+    sw, x = _np(sweeps), _np(dev[5])
+    print("tiny-window batch sweeps", np.bincount(sw), flush=True)
+    assert sw.max() >= 3, sw   # sweeps from the third on run on the resident grid
+    for j in range(len(n)):
+        sl = slice(int(off[j]), int(off[j]) + int(n[j]))
+        ref = oracle_lib.gbsv(*[t[sl] for t in ins])
+        assert np.abs(x[sl] - ref).max() / np.abs(ref).max() <= 1e-12, j
+    long_ids = [j for j in range(len(n)) if n[j] > 2]
+    check = long_ids + list(range(0, len(n), 97))
+    for j in check:
+        sl = slice(int(off[j]), int(off[j]) + int(n[j]))
+        # same element-offset parity as in the batch (odd offsets pair rows from row -1)
+        par = int(off[j]) & 1
+        one = []
+        for t in ins:
+            buf = np.zeros(par + int(n[j]))
+            buf[par:] = t[sl]
+            one.append(torch.from_numpy(buf).cuda())
+        s1 = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+        gpu.solve_uneven_refined(*one, [par], [int(n[j])], max_sweeps=ms, tol=tol, sweeps=s1)
+        torch.cuda.synchronize()
+        assert int(s1.item()) == int(sw[j]), (j, int(n[j]), int(s1.item()), int(sw[j]))
+        assert np.array_equal(_np(one[5])[par:], x[sl]), (j, int(n[j]))
 
 
 def test_uneven_refined_flags_nonconvergence(gpu, oracle_lib):
@@ -696,6 +744,65 @@ def test_heston_matches_cpu_restatement(gpu, oracle_lib):
     ref = H.adi(d, oracle_lib)
     err = np.abs(U - ref).max() / np.abs(ref).max()
     assert err <= 1e-12, err
+
+
+def test_heston_full_grid_f64_matches_cpu_restatement(gpu, oracle_lib):
+    """The Heston grid itself (Ns = 1024, Nv = 512), FP64, first 4 Douglas
+    steps: the GPU driver equals the numpy restatement with oracle line
+    solves to 1e-12 normwise (the stencil sums round in another order)."""
+    import heston_ref as H
+    kw = dict(steps=4)
+    _say("heston-full-grid-f64", kw)
+    U, p = _heston_gpu(gpu, gpu.PREC_F64, False, **kw)
+    d = dict(gpu.HESTON_GRID)
+    d.update(kw)
+    ref = H.adi(d, oracle_lib)
+    err = np.abs(U - ref).max() / np.abs(ref).max()
+    print("heston full grid FP64 vs restatement", err, flush=True)
+    assert err <= 1e-12, err
+
+
+def _heston_line_rhs(oracle_lib, d):
+    """Line systems of the Heston grid and the right-hand sides of a Douglas
+    step a few steps in (S-lines: R1; v-lines: X = Y1 + W, with Y1 from the
+    FP64 oracle), built on the CPU."""
+    import heston_ref as H
+    S, V = H.line_systems(d)
+    ns, nv = d["ns"], d["nv"]
+    U = H.adi(dict(d, steps=3), oracle_lib)
+    R1, W = H.explicit(d, U, 3)
+    Y1 = oracle_lib.pcr_batch(*[t.ravel() for t in S], R1.ravel(), ns, nv,
+                              oracle_lib.CONTIGUOUS).reshape(nv, ns)
+    return S, V, R1, Y1 + W
+
+
+@pytest.mark.parametrize("lines", ["S", "v"])
+def test_heston_line_solves_refined_vs_dgbsv(gpu, oracle_lib, lines):
+    """Every Heston line system (S-lines: 512 contiguous systems of n = 1024,
+    v-lines: 1024 interleaved systems of n = 512) solved by FP32 + FP64
+    refinement at the driver's settings is within 1e-12 of dgbsv, normwise,
+    system by system."""
+    import torch
+    d = dict(gpu.HESTON_GRID)
+    S, V, R1, X = _heston_line_rhs(oracle_lib, d)
+    ns, nv = d["ns"], d["nv"]
+    if lines == "S":
+        diags, rhs, n, m, layout = S, R1, ns, nv, C
+    else:
+        diags, rhs, n, m, layout = V, X, nv, ns, I
+    ins = [np.ascontiguousarray(t).ravel() for t in diags] + [np.ascontiguousarray(rhs).ravel()]
+    _say("heston-lines-refined", lines, n, m, layout)
+    dev = [torch.from_numpy(t.copy()).cuda() for t in ins]
+    x, sw = _refined(gpu, dev, n, m, layout, max_sweeps=gpu.HESTON_MAX_SWEEPS,
+                     tol=gpu.HESTON_REFINE_TOL)
+    ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+    rs, ss = oracle_lib.strides(n, m, layout)
+    errs = np.array([np.abs(x[j * ss + np.arange(n) * rs] - ref[j * ss + np.arange(n) * rs]).max()
+                     / np.abs(ref[j * ss + np.arange(n) * rs]).max() for j in range(m)])
+    print(f"heston {lines}-lines: max normwise error {errs.max():.3e} (median "
+          f"{np.median(errs):.3e}), sweeps {np.bincount(sw).tolist()}", flush=True)
+    assert errs.max() <= 1e-12, (lines, errs.max(), int(errs.argmax()))
+    assert sw.min() >= 1 and sw.max() <= gpu.HESTON_MAX_SWEEPS, sw
 
 
 def test_heston_graph_replay_equals_eager(gpu):
