@@ -1,23 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the batched PCR-Penta solve on one MI355X.
 
-Workload (default): m = 16384 systems of n = 1024, interleaved layout, plain
-FP64, seeded strictly dominant generator (off-diagonals U[-1,1], main
-U[4.5,5.5], RHS U[-1,1], seed 0). One "step" = one solve of the whole batch
-(y <- A^{-1} y in place). Every step gets its own pristine right-hand side,
-generated into HBM before the timed region, so nothing but solves is timed.
+Default workload: the uneven batch, 10000 systems with n drawn log-uniform in
+[64, 131072] (seed 1, about 1.7e8 unknowns), strictly dominant generator
+(off-diagonals U[-1,1], main U[4.5,5.5], RHS U[-1,1]), systems stored back to
+back. One "step" = one solve of the whole batch (y <- A^{-1} y in place).
+`value` is plain FP64 PCR-Penta (the reference's precision); the same line
+carries the FP32 PCR + FP64 refinement form on the same batch (its unknowns/s,
+its ratio to FP64, its sweep counts), each form's error against the FP64
+banded-LU (dgbsv) oracle over every system, each form's HBM roofline (56
+algorithmic bytes per unknown: a, b, c, d, e, y read + x written, all FP64),
+and the CPU baseline (the oracle's FP64 PCR-Penta, OpenMP over systems) on the
+whole batch on this host. Every timed step gets its own pristine right-hand
+side, generated into HBM before the timed region, so nothing but solves is
+timed. `also` adds the other workloads (Heston ADI, the n = 1024 / 2048
+batches, n = 2^20) timed the same way in both forms.
 
-Prints ONE JSON line: unknowns/s (value), ms per step, the dominant kernel's
-roofline (56 algorithmic bytes per unknown: a,b,c,d,e,y read + x written, all
-FP64) and the CPU baseline (the oracle's FP64 PCR-Penta, OpenMP over systems)
-timed on a sample of the same workload on this host.
+--workload NAME runs one form of one workload (the profiles use these).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -25,6 +33,8 @@ sys.path.insert(0, os.path.join(ROOT, "pcr-penta_amd"))
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_UNKNOWN = 56     # 7 FP64 words
+METRIC = "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12"
+HEADLINE = "uneven_m10000"
 
 WORKLOADS = {
     # name: (n, m, precision, generator kind); kind 0 = strictly dominant
@@ -46,6 +56,7 @@ WORKLOADS = {
     "heston_adi_f32r": (-1, 0, "f32r", 0),
 }
 MAX_SWEEPS, TOL = 8, 1e-12
+RHS_BYTES_CAP = 64e9       # pristine right-hand sides kept in HBM for the timed steps
 
 
 def parse():
@@ -53,10 +64,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="lds_f64_m16384_n1024", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default=HEADLINE, choices=sorted(WORKLOADS) + [HEADLINE])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and checks")
+    ap.add_argument("--no-also", action="store_true", help="headline: skip the other workloads")
     ap.add_argument("--cpu-systems", type=int, default=2048,
-                    help="systems in the CPU-baseline sample")
+                    help="uniform workloads: systems in the CPU-baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="CPU baseline: timed runs (median)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner; use the built-in default variant")
     ap.add_argument("--tune-save", default=None, help="write the tuning table to this file")
@@ -70,61 +83,37 @@ def parse():
         kf, sh = (int(v) for v in args.split.split(","))
         args.split = (kf, sh)
         args.no_tune = True
+    if args.tune_load:
+        args.no_tune = True
     return args
 
 
-def apply_split(args):
-    if args.split:
-        import pcrpenta
-        lds_max = pcrpenta.split_tune_get()[0]
-        pcrpenta.split_tune_set(lds_max, *args.split)
+def log(obj):
+    print(json.dumps(obj), file=sys.stderr, flush=True)
 
 
-CPU_SECONDS = 10.0   # CPU-baseline sample: repeat the host solve for about this long
+# ---------------------------------------------------------------- timing
 
-
-def timed_repeats(fn, seconds=CPU_SECONDS, min_runs=3):
-    """Run fn() after one warm-up until `seconds` have passed (at least
-    min_runs times); returns (runs, elapsed seconds, last result)."""
-    out = fn()
-    runs, t0 = 0, time.perf_counter()
-    while runs < min_runs or time.perf_counter() - t0 < seconds:
-        out = fn()
-        runs += 1
-    return runs, time.perf_counter() - t0, out
-
-
-def cpu_baseline(n, m, layout, kind, seed, gpu_x_sample, nsys, prec):
-    """The oracle's FP64 PCR-Penta on the host (all threads, OpenMP over
-    systems): the first `nsys` systems of the same workload, solved over and
-    over for about CPU_SECONDS; also checks the GPU's result on those systems
-    (bit-identical for FP64, error against the FP64 banded-LU oracle for
-    FP32+refine)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import oracle
-    oracle.load()
-    arrs = oracle.gen_batch(kind, seed, n, nsys, oracle.INTERLEAVED)
-    threads = oracle.max_threads()
-    runs, t, x = timed_repeats(
-        lambda: oracle.pcr_batch(*arrs, n, nsys, oracle.INTERLEAVED, threads))
-    agree = err = None
-    if gpu_x_sample is not None:
-        agree = bool(np.array_equal(x.reshape(n, nsys), gpu_x_sample))
-        xg = oracle.gbsv_batch(*arrs, n, nsys, oracle.INTERLEAVED)
-        err = oracle.normwise_relerr(gpu_x_sample.reshape(-1), xg, n, nsys, oracle.INTERLEAVED)
-    return {
-        "value": n * nsys * runs / t,
-        "unit": "unknowns/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"first {nsys} of the {m} systems (n={n}, same generator) solved {runs} times "
-                  f"in {t:.1f} s, FP64 PCR-Penta restated from kernel.cu:32-154, gcc -O2 + "
-                  f"OpenMP over systems",
-        "seconds": t,
-        "bit_identical_to_gpu": agree,
-        "gpu_relerr_vs_dgbsv": err,
-    }
+def time_steps(step, K, W):
+    """W untimed steps, then K timed steps between two device synchronisations,
+    each bracketed by HIP events on the solve stream. Returns (wall seconds of the
+    K steps, mean event-timed seconds per step)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for w in range(W):
+        step(w)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(K)]
+    t0 = time.perf_counter()
+    for k in range(K):
+        evs[k][0].record(stream)
+        step(W + k)
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = sum(s.elapsed_time(f) for s, f in evs) / K / 1e3
+    return wall, ev
 
 
 def measured_traffic(workload):
@@ -138,59 +127,134 @@ def measured_traffic(workload):
         return json.load(f).get(workload, {}).get("bytes_per_step")
 
 
-def bench_uneven(args, m, prec, kind):
-    """One uneven batch (contiguous systems back to back, n log-uniform in [64, 131072]),
-    seed 1."""
-    import numpy as np
+def roofline(unknowns, solve_s, workload, note):
+    achieved = BYTES_PER_UNKNOWN * unknowns / solve_s
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": measured_traffic(workload),
+            "solve_ms": solve_s * 1e3, "note": note}
+
+
+def apply_split(args):
+    if args.split:
+        import pcrpenta
+        lds_max = pcrpenta.split_tune_get()[0]
+        pcrpenta.split_tune_set(lds_max, *args.split)
+
+
+# ---------------------------------------------------------------- CPU side
+
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+_NATIVE = {}
+
+
+def oracle_native():
+    """The oracle built for this host (gcc -O3 -march=native + OpenMP, threads
+    pinned one per core), loaded once; it is the CPU baseline and the checker."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if "path" not in _NATIVE:
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        os.environ.setdefault("OMP_PLACES", "cores")
+        _NATIVE["dir"] = tempfile.mkdtemp(prefix="pcrp_oracle_")
+        _NATIVE["path"] = oracle.load_native(_NATIVE["dir"])
+    return oracle
+
+
+def median_runs(fn, runs, min_seconds=10.0):
+    """One warm-up call, then at least `runs` timed calls and at least
+    min_seconds in total; returns (median seconds, number of runs, last result)."""
+    out = fn()
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < runs or time.perf_counter() - t_start < min_seconds:
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), len(times), out
+
+
+def cpu_fields(oracle, runs, tmed):
+    return {"cores": oracle.max_threads(), "kind": "port", "cpu_model": cpu_info(),
+            "pinning": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
+                       f"OMP_PLACES={os.environ.get('OMP_PLACES')}",
+            "build": "gcc -O3 -march=native -ffp-contract=off -fopenmp, built on this host",
+            "runs": runs, "median_s": tmed}
+
+
+# ---------------------------------------------------------------- uneven batch
+
+def uneven_batch(kind=0, seed=1, m=10000):
     import torch
     import pcrpenta
-    seed = 1
-    K, W = args.steps, args.warmup
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     sizes = pcrpenta.uneven_log_uniform_sizes(m, seed)
     off, nn, total = pcrpenta.uneven_layout(sizes)
-    a, b, c, d, e, y0 = pcrpenta.generate_uneven(kind, seed, off, nn, device=dev)
-    nbuf = max(1, min(K + W, int(8e9 // (8 * total))))
-    ys = [y0.clone() for _ in range(nbuf)]
-    y_check = y0.clone()
+    dev = torch.device("cuda:0")
+    arrs = pcrpenta.generate_uneven(kind, seed, off, nn, device=dev)
+    return {"sizes": sizes, "off": off, "nn": nn, "total": total, "arrs": arrs, "m": m,
+            "seed": seed, "kind": kind}
+
+
+def tune_uneven_split(B):
+    """Autotune the split path's steps fused per pass and tile shape on this batch."""
+    import torch
+    import pcrpenta
+    a, b, c, d, e, y0 = B["arrs"]
+    off, nn = B["off"], B["nn"]
     stream = torch.cuda.current_stream()
-    torch.cuda.synchronize()
-    tuned = None
+    lds_max, _, _, n_shapes = pcrpenta.split_tune_get()
+    best = None
+    ys_t = y0.clone()
+    for kf in (2, 3, 4, 5):
+        for sh in range(n_shapes):
+            try:
+                pcrpenta.split_tune_set(lds_max, kf, sh)
+            except pcrpenta.PCRPError:
+                continue
+            nb = pcrpenta.uneven_workspace_size(off, nn)
+            ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y0.device)
+            ys_t.copy_(y0)
+            pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ys_t.copy_(y0)
+            e0.record(stream)
+            pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1)
+            if best is None or t < best[0]:
+                best = (t, kf, sh)
+            del ws
+    pcrpenta.split_tune_set(lds_max, best[1], best[2])
+    return {"k_fuse": best[1], "shape": best[2], "ms": best[0]}
+
+
+def run_uneven(B, prec, K, W):
+    """K timed solves of the uneven batch in one precision form; returns the
+    timings, the sweep histogram and x of one more solve (device tensor)."""
+    import torch
+    import pcrpenta
+    a, b, c, d, e, y0 = B["arrs"]
+    off, nn, total, m = B["off"], B["nn"], B["total"], B["m"]
     refined = prec != "f64"
-    sweeps = torch.zeros(m, dtype=torch.int32, device=dev) if refined else None
-    if not args.no_tune and not refined:
-        # autotune steps fused per pass and the later rounds' tile shape on this batch
-        lds_max, _, _, n_shapes = pcrpenta.split_tune_get()
-        best = None
-        ys_t = y0.clone()
-        for kf in (2, 3, 4, 5):
-            for sh in range(n_shapes):
-                try:
-                    pcrpenta.split_tune_set(lds_max, kf, sh)
-                except pcrpenta.PCRPError:
-                    continue
-                nb = pcrpenta.uneven_workspace_size(off, nn)
-                ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
-                ys_t.copy_(y0)
-                pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                ys_t.copy_(y0)
-                e0.record(stream)
-                pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
-                e1.record(stream)
-                torch.cuda.synchronize()
-                t = e0.elapsed_time(e1)
-                if best is None or t < best[0]:
-                    best = (t, kf, sh)
-                del ws
-        pcrpenta.split_tune_set(lds_max, best[1], best[2])
-        tuned = {"k_fuse": best[1], "shape": best[2], "ms": best[0]}
-        print(json.dumps({"split_autotune": tuned}), file=sys.stderr, flush=True)
-        del ys_t
+    nbuf = max(1, min(K + W, int(RHS_BYTES_CAP // (8 * total))))
+    ys = [y0.clone() for _ in range(nbuf)]
+    sweeps = torch.zeros(m, dtype=torch.int32, device=y0.device) if refined else None
     nb = pcrpenta.uneven_workspace_size(
         off, nn, pcrpenta.PREC_F32_REFINE if refined else pcrpenta.PREC_F64)
-    ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=dev)
+    ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y0.device)
+    stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
 
     def step(k, y=None):
@@ -200,83 +264,141 @@ def bench_uneven(args, m, prec, kind):
                                                  tol=TOL, sweeps=sweeps, stream=stream, ws=ws)
         return pcrpenta.solve_uneven(a, b, c, d, e, y, off, nn, stream=stream, ws=ws)
 
-    for w in range(W):
-        step(w)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(K)]
-    t0 = time.perf_counter()
-    for k in range(K):
-        evs[k][0].record(stream)
-        step(W + k)
-        evs[k][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    solve_s = sum(s_.elapsed_time(f) for s_, f in evs) / K / 1e3
+    wall, solve_s = time_steps(step, K, W)
+    y_check = y0.clone()
     step(0, y_check)
     torch.cuda.synchronize()
-    achieved = BYTES_PER_UNKNOWN * total / solve_s
+    del ys
+    out = {"value": total / (wall / K), "ms_per_step": wall * 1e3 / K, "solve_s": solve_s,
+           "x": y_check}
+    if refined:
+        out["sweeps_hist"] = torch.bincount(sweeps.long()).tolist()
+    return out
+
+
+def bench_headline(args):
+    """Uneven batch, both forms in one line (value: plain FP64)."""
+    import numpy as np
+    import torch
+    import pcrpenta
+    K, W = args.steps, args.warmup
+    B = uneven_batch()
+    off, nn, total, m = B["off"], B["nn"], B["total"], B["m"]
+    tuned = None
+    if not args.no_tune:
+        tuned = tune_uneven_split(B)
+        log({"split_autotune": tuned})
+    f64 = run_uneven(B, "f64", K, W)
+    f32 = run_uneven(B, "f32r", K, W)
+    ratio = f32["value"] / f64["value"]
     out = {
-        "metric": "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12",
-        "value": total / (wall / K), "unit": "unknowns/s", "n_gpus": 1, "steps": K, "warmup": W,
-        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": prec,
-        "data": "synthetic (seeded counter-based generator, strictly dominant distribution, 10000 "
-                "sizes log-uniform in [64, 131072])",
-        "config": {"workload": args.workload, "m": m, "unknowns": total, "n_min": int(nn.min()),
+        "metric": METRIC, "value": f64["value"], "unit": "unknowns/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": f64["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded counter-based generator, strictly dominant distribution, "
+                "10000 sizes log-uniform in [64, 131072], seed 1)",
+        "config": {"workload": HEADLINE, "m": m, "unknowns": total, "n_min": int(nn.min()),
                    "n_max": int(nn.max()), "large_systems": int((nn > 1024).sum()),
-                   "layout": "contiguous per system", "precision": prec, "seed": seed,
-                   "split": None if refined else list(pcrpenta.split_tune_get()[:3]),
-                   "refine": ({"max_sweeps": MAX_SWEEPS, "tol": TOL,
-                               "sweeps_hist": torch.bincount(sweeps.long()).tolist()}
-                              if refined else None)},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(args.workload),
-                     "solve_ms": solve_s * 1e3,
-                     "note": ("whole solve per step: three window-path refinement sweeps "
-                              "(one kernel each) and the final copy" if refined else
-                              "whole solve (pass rounds + persistent LDS launch) per step")},
+                   "layout": "contiguous per system (back to back)", "seed": 1,
+                   "value_precision": "f64 (plain PCR-Penta)",
+                   "split": list(pcrpenta.split_tune_get()[:3]), "split_autotune": tuned},
+        "roofline": roofline(total, f64["solve_s"], "uneven_f64_m10000",
+                             "plain FP64, whole solve per step (HBM pass rounds + one persistent "
+                             "work-queue LDS launch), HIP events on the solve stream"),
+        "f32r": {
+            "value": f32["value"], "unit": "unknowns/s", "ms_per_step": f32["ms_per_step"],
+            "ratio_vs_f64": ratio, "dtype": "f32 inner solve, f64 residual and update",
+            "refine": {"max_sweeps": MAX_SWEEPS, "tol": TOL, "sweeps_hist": f32["sweeps_hist"]},
+            "roofline": roofline(total, f32["solve_s"], "uneven_f32r_m10000",
+                                 "FP32+refine, whole solve per step (window-path sweeps, one "
+                                 "kernel each, and the final copy)"),
+        },
     }
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        oracle.load()
-        # sample: the first systems of the draw up to ~4e6 unknowns
-        cnt = int(np.searchsorted(np.cumsum(nn.astype(np.int64)), 4_000_000)) + 1
-        cnt = min(cnt, m)
-        sub_total = int(off[cnt - 1] + nn[cnt - 1])
-        ins = [t[:sub_total].cpu().numpy() for t in (a, b, c, d, e, y0)]
-        runs, tc, x = timed_repeats(lambda: oracle.pcr_uneven(*ins, off[:cnt], nn[:cnt]))
-        xg = y_check[:sub_total].cpu().numpy()
-        check = {"bit_identical_to_gpu": bool(np.array_equal(x, xg))}
-        if refined:
-            errs = []
-            for o, nj in zip(off[:cnt], nn[:cnt]):
-                sl = slice(int(o), int(o) + int(nj))
-                ref = oracle.gbsv(*[t[sl] for t in ins])
-                errs.append(float(np.abs(xg[sl] - ref).max() / np.abs(ref).max()))
-            check = {"gpu_max_relerr_vs_dgbsv": max(errs)}
-        out["cpu_baseline"] = {
-            "value": sub_total * runs / tc, "unit": "unknowns/s", "cores": oracle.max_threads(),
-            "kind": "port",
-            "sample": f"first {cnt} systems of the draw ({sub_total} unknowns) solved {runs} "
-                      f"times in {tc:.1f} s, FP64 PCR-Penta restated from kernel.cu:32-154, "
-                      f"OpenMP dynamic schedule over systems",
-            "seconds": tc, **check,
+        oracle = oracle_native()
+        ins = [t.cpu().numpy() for t in B["arrs"]]
+        x64 = f64["x"].cpu().numpy()
+        x32 = f32["x"].cpu().numpy()
+        ref = oracle.gbsv_uneven(*ins, off, nn)
+        e64 = oracle.uneven_relerr(x64, ref, off, nn)
+        e32 = oracle.uneven_relerr(x32, ref, off, nn)
+        out["accuracy"] = {
+            "systems": f"all {m} systems, normwise max-norm relative error vs dgbsv "
+                       "(kl = ku = 2, partial pivoting)",
+            "f64_max_relerr": float(e64.max()), "f64_systems_above_1e-12": int((e64 > 1e-12).sum()),
+            "f32r_max_relerr": float(e32.max()),
+            "f32r_systems_above_1e-12": int((e32 > 1e-12).sum()),
         }
+        out["f32r"]["max_relerr_vs_dgbsv"] = float(e32.max())
+        tmed, runs, xc = median_runs(lambda: oracle.pcr_uneven(*ins, off, nn), args.cpu_runs)
+        out["cpu_baseline"] = {
+            "value": total / tmed, "unit": "unknowns/s",
+            "sample": f"the whole batch ({m} systems, {total} unknowns), FP64 PCR-Penta "
+                      f"restated from kernel.cu:32-154, OpenMP dynamic schedule over systems, "
+                      f"median of {runs} runs",
+            **cpu_fields(oracle, runs, tmed),
+            "bit_identical_to_gpu_f64": bool(np.array_equal(xc, x64)),
+        }
+        del ins
+    del f64["x"], f32["x"]
+    if not args.no_also:
+        out["also"] = also_workloads(args)
     print(json.dumps(out), flush=True)
 
 
-def bench_heston(args, prec):
-    """Heston ADI, Douglas theta = 1/2, Ns = 1024, Nv = 512, 200 steps,
-    each step replayed from one captured hipGraph. Unknowns per step: Nv S-lines
-    of 1024 plus Ns v-lines of 512 = 2 * 524288."""
+# ---------------------------------------------------------------- uniform batches
+
+def run_uniform(n, m, prec, kind, K, W, tune=True, layout=None):
+    """K timed solves of a uniform batch; returns timings (and x of one more
+    solve on a pristine right-hand side, as a device tensor)."""
+    import torch
+    import pcrpenta
+    layout = pcrpenta.INTERLEAVED if layout is None else layout
+    dev = torch.device("cuda:0")
+    a, b, c, d, e, y0 = pcrpenta.generate(kind, 0, n, m, layout, device=dev)
+    pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
+    ws, nb = pcrpenta._workspace(n, m, pid, dev)
+    tuned = None
+    if tune and prec == "f64" and n > 1024:
+        kf, sh, ms = pcrpenta.split_autotune(a, b, c, d, e, y0, n, m, layout, reps=2, ws=ws)
+        tuned = {"split": [kf, sh], "ms": ms}
+    elif tune and prec == "f64":
+        best, best_ms, times = pcrpenta.autotune(a, b, c, d, e, y0, n, m, layout, reps=3)
+        vs = pcrpenta.variants()
+        tuned = {"variant": vs[best], "ms": best_ms,
+                 "all_ms": {f"{v['kind']}-{v['nt']}x{v['rpt']}": round(t, 4)
+                            for v, t in zip(vs, times) if t >= 0}}
+    nbuf = max(1, min(K + W, int(RHS_BYTES_CAP // (8 * n * m))))
+    ys = [y0.clone() for _ in range(nbuf)]
+    sweeps = torch.zeros(m, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+
+    def step(k, y=None):
+        y = ys[k % nbuf] if y is None else y
+        if prec == "f64":
+            return pcrpenta.solve(a, b, c, d, e, y, n, m, layout, stream=stream, ws=ws)
+        return pcrpenta.solve_refined(a, b, c, d, e, y, n, m, layout, max_sweeps=MAX_SWEEPS,
+                                      tol=TOL, sweeps=sweeps, stream=stream, ws=ws)
+
+    wall, solve_s = time_steps(step, K, W)
+    del ys
+    y_check = y0.clone()
+    step(0, y_check)
+    torch.cuda.synchronize()
+    out = {"value": n * m / (wall / K), "ms_per_step": wall * 1e3 / K, "solve_s": solve_s,
+           "tuned": tuned, "x": y_check, "layout": layout}
+    if prec != "f64":
+        out["sweeps_hist"] = torch.bincount(sweeps.long()).tolist()
+    return out
+
+
+def run_heston(prec, K, W):
+    """K timed Heston ADI runs (200 Douglas steps each, one hipGraph per step)."""
     import torch
     import pcrpenta
     pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
     p = pcrpenta.heston_params()
-    K, W = args.steps, args.warmup
     s = torch.cuda.Stream()
     u = torch.empty(p.ns * p.nv, dtype=torch.float64, device="cuda")
     ws = torch.empty(int(pcrpenta.load().pcrp_heston_workspace_size(
@@ -291,24 +413,127 @@ def bench_heston(args, prec):
         s.synchronize()
         wall = time.perf_counter() - t0
     unknowns = 2 * p.ns * p.nv * p.steps
+    return {"value": unknowns / (wall / K), "ms_per_run": wall * 1e3 / K, "u": u, "p": p,
+            "unknowns": unknowns}
+
+
+def also_workloads(args):
+    """The other workloads, both forms each, fewer steps (timing only: the GPU
+    tests hold their parity checks)."""
+    import numpy as np
+    K, W = 10, 3
+    out = {}
+    h64 = run_heston("f64", 5, 2)
+    h32 = run_heston("f32r", 5, 2)
+    p = h64["p"]
     hS, hv = p.smax / (p.ns + 1), p.vmax / p.nv
-    i0, j0 = int(round(100.0 / hS)), int(round(0.04 / hv))
-    out = {
-        "metric": "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12",
-        "value": unknowns / (wall / K), "unit": "unknowns/s", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": prec,
-        "data": "synthetic (Heston European call, Ns = 1024, Nv = 512, 200 steps of 1/200)",
-        "config": {"workload": args.workload, "ns": p.ns, "nv": p.nv, "time_steps": p.steps,
-                   "dt": p.dt, "scheme": "Douglas theta=0.5", "graph": "one hipGraph per time "
-                   "step, replayed", "unknowns_per_run": unknowns,
-                   "price_near_S100_v0.04": float(u.view(p.nv, p.ns)[j0, i0 - 1].item())},
-        "roofline": {"bound": "hbm", "achieved": 56.0 * unknowns / (wall / K) / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": 56.0 * unknowns / (wall / K) / HBM_PEAK, "traffic": None,
-                     "note": "one solve is 29 MB (fits the 256 MB Infinity Cache); whole ADI "
-                             "run incl. explicit stencil kernels"},
+    U64 = h64["u"].view(p.nv, p.ns).cpu().numpy()
+    U32 = h32["u"].view(p.nv, p.ns).cpu().numpy()
+    out["heston_adi"] = {
+        "grid": [p.ns, p.nv], "time_steps": p.steps, "graph": "one hipGraph per time step",
+        "f64_ms_per_run": h64["ms_per_run"], "f32r_ms_per_run": h32["ms_per_run"],
+        "f64_unknowns_per_s": h64["value"], "f32r_unknowns_per_s": h32["value"],
+        "ratio_f32r_vs_f64": h64["ms_per_run"] / h32["ms_per_run"],
+        "f32r_vs_f64_grid_relerr": float(np.abs(U32 - U64).max() / np.abs(U64).max()),
+        "price_at_S100_v0.04_f64": float(_interp_price(U64, p, hS, hv)),
+        "refine": {"tol": 1e-12, "max_sweeps": 8},
     }
+    for label, n, m, kind in (("lds_n1024_m16384", 1024, 16384, 0),
+                              ("n2048_m16384", 2048, 16384, 0),
+                              ("hbm_n1048576_m32", 1 << 20, 32, 1)):
+        r64 = run_uniform(n, m, "f64", kind, K, W)
+        r32 = run_uniform(n, m, "f32r", kind, K, W)
+        out[label] = {
+            "n": n, "m": m, "generator": "dominant" if kind == 0 else "weak",
+            "layout": "interleaved",
+            "f64_unknowns_per_s": r64["value"], "f64_ms": r64["ms_per_step"],
+            "f64_roofline_frac": BYTES_PER_UNKNOWN * n * m / r64["solve_s"] / HBM_PEAK,
+            "f32r_unknowns_per_s": r32["value"], "f32r_ms": r32["ms_per_step"],
+            "f32r_roofline_frac": BYTES_PER_UNKNOWN * n * m / r32["solve_s"] / HBM_PEAK,
+            "ratio_f32r_vs_f64": r32["value"] / r64["value"],
+            "f32r_sweeps_hist": r32["sweeps_hist"],
+            "f64_tuned": (r64["tuned"] or {}).get("variant", (r64["tuned"] or {}).get("split")),
+        }
+        del r64, r32
+    return out
+
+
+def _interp_price(U, p, hS, hv):
+    """Bilinear price at S = 100, v = 0.04 from the grid nodes."""
+    fi = 100.0 / hS - 1.0     # fractional column index (node i at column i - 1)
+    fj = 0.04 / hv
+    i0, j0 = int(fi), int(fj)
+    ti, tj = fi - i0, fj - j0
+    return ((1 - tj) * ((1 - ti) * U[j0, i0] + ti * U[j0, i0 + 1])
+            + tj * ((1 - ti) * U[j0 + 1, i0] + ti * U[j0 + 1, i0 + 1]))
+
+
+# ---------------------------------------------------------------- single-form workloads
+
+def bench_single(args):
+    """One form of one workload (what the rocprof runs use)."""
+    import numpy as np
+    import torch
+    import pcrpenta
+    n, m, prec, kind = WORKLOADS[args.workload]
+    K, W = args.steps, args.warmup
+    torch.cuda.set_device(0)
+    if args.tune_load:
+        pcrpenta.tune_load(args.tune_load)
+    if n == -1:
+        r = run_heston(prec, K, W)
+        p = r["p"]
+        out = {"metric": METRIC, "value": r["value"], "unit": "unknowns/s", "n_gpus": 1,
+               "steps": K, "warmup": W, "ms_per_step": r["ms_per_run"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": prec,
+               "data": "synthetic (Heston European call, Ns = 1024, Nv = 512, 200 steps of 1/200)",
+               "config": {"workload": args.workload, "ns": p.ns, "nv": p.nv,
+                          "time_steps": p.steps, "graph": "one hipGraph per time step",
+                          "unknowns_per_run": r["unknowns"]}}
+        print(json.dumps(out), flush=True)
+        return
+    if n == 0:
+        B = uneven_batch(kind)
+        if not args.no_tune and prec == "f64":
+            log({"split_autotune": tune_uneven_split(B)})
+        r = run_uneven(B, prec, K, W)
+        unknowns = B["total"]
+        cfg = {"workload": args.workload, "m": m, "unknowns": unknowns,
+               "layout": "contiguous per system", "precision": prec,
+               "split": list(pcrpenta.split_tune_get()[:3])}
+    else:
+        r = run_uniform(n, m, prec, kind, K, W, tune=not args.no_tune)
+        if r["tuned"]:
+            log({"autotune": r["tuned"]})
+        unknowns = n * m
+        cfg = {"workload": args.workload, "n": n, "m": m, "layout": "interleaved",
+               "precision": prec, "generator": "dominant" if kind == 0 else "weak",
+               "split": list(pcrpenta.split_tune_get()[:3]) if n > 1024 else None}
+    if args.tune_save:
+        pcrpenta.tune_save(args.tune_save)
+    if prec != "f64":
+        cfg["refine"] = {"max_sweeps": MAX_SWEEPS, "tol": TOL, "sweeps_hist": r["sweeps_hist"]}
+    out = {"metric": METRIC, "value": r["value"], "unit": "unknowns/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": prec,
+           "data": "synthetic (seeded counter-based generator)", "config": cfg,
+           "roofline": roofline(unknowns, r["solve_s"], args.workload, "whole solve per step")}
+    if not args.no_cpu and n > 0:
+        oracle = oracle_native()
+        nsys = min(args.cpu_systems, m)
+        arrs = oracle.gen_batch(kind, 0, n, nsys, oracle.INTERLEAVED)
+        tmed, runs, x = median_runs(
+            lambda: oracle.pcr_batch(*arrs, n, nsys, oracle.INTERLEAVED), args.cpu_runs)
+        xs = r["x"].view(n, m)[:, :nsys].cpu().numpy()
+        chk = {"bit_identical_to_gpu": bool(np.array_equal(x.reshape(n, nsys), xs))}
+        if prec != "f64":
+            xg = oracle.gbsv_batch(*arrs, n, nsys, oracle.INTERLEAVED)
+            chk = {"gpu_relerr_vs_dgbsv": oracle.normwise_relerr(xs.reshape(-1), xg, n, nsys,
+                                                                 oracle.INTERLEAVED)}
+        out["cpu_baseline"] = {"value": n * nsys / tmed, "unit": "unknowns/s",
+                               "sample": f"first {nsys} of the {m} systems (n={n}, same "
+                                         f"generator), median of {runs} runs",
+                               **cpu_fields(oracle, runs, tmed), **chk}
     print(json.dumps(out), flush=True)
 
 
@@ -321,115 +546,12 @@ def main():
         sys.exit(2)
     import torch
     import pcrpenta
-
-    n, m, prec, kind = WORKLOADS[args.workload]
+    pcrpenta.load()
+    torch.cuda.set_device(0)
     apply_split(args)
-    if n == 0:
-        return bench_uneven(args, m, prec, kind)
-    if n == -1:
-        return bench_heston(args, prec)
-    layout = pcrpenta.INTERLEAVED
-    seed = 0
-    K, W = args.steps, args.warmup
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-    a, b, c, d, e, y0 = pcrpenta.generate(kind, seed, n, m, layout, device=dev)
-    nbuf = max(1, min(K + W, int(8e9 // (8 * n * m))))
-    ys = [y0.clone() for _ in range(nbuf)]
-    y_check = y0.clone()
-    pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
-    ws, nb = pcrpenta._workspace(n, m, pid, dev)
-    sweeps = torch.zeros(m, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream()
-    torch.cuda.synchronize()
-    tuned = None
-    if args.tune_load:
-        pcrpenta.tune_load(args.tune_load)
-    elif not args.no_tune and prec == "f64" and n > 1024:
-        # split path: the autotuner picks steps fused per pass and the pass tile shape
-        kf, sh, ms = pcrpenta.split_autotune(a, b, c, d, e, y0, n, m, layout, reps=2, ws=ws)
-        tuned = {"split": [kf, sh], "ms": ms}
-        print(json.dumps({"split_autotune": tuned}), file=sys.stderr, flush=True)
-        if args.tune_save:
-            pcrpenta.tune_save(args.tune_save)
-    elif not args.no_tune and prec == "f64" and n <= 1024:
-        # the autotuner times every LDS variant on this very batch (outside the timed region)
-        best, best_ms, times = pcrpenta.autotune(a, b, c, d, e, y0, n, m, layout, reps=3)
-        vs = pcrpenta.variants()
-        tuned = {"variant": vs[best], "ms": best_ms,
-                 "all_ms": {f"{v['kind']}-{v['nt']}x{v['rpt']}": round(t, 4)
-                            for v, t in zip(vs, times) if t >= 0}}
-        print(json.dumps({"autotune": tuned}), file=sys.stderr, flush=True)
-        if args.tune_save:
-            pcrpenta.tune_save(args.tune_save)
-    vid = pcrpenta.tune_get(min(n, 1024)) if prec == "f64" else -1
-    variant = pcrpenta.variants()[vid] if vid >= 0 else None
-
-    def step(k, y=None):
-        y = ys[k % nbuf] if y is None else y
-        if prec == "f64":
-            return pcrpenta.solve(a, b, c, d, e, y, n, m, layout, stream=stream, ws=ws)
-        return pcrpenta.solve_refined(a, b, c, d, e, y, n, m, layout, max_sweeps=MAX_SWEEPS,
-                                      tol=TOL, sweeps=sweeps, stream=stream, ws=ws)
-
-    for w in range(W):
-        step(w)
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(K)]
-    t0 = time.perf_counter()
-    for k in range(K):
-        evs[k][0].record(stream)
-        step(W + k)
-        evs[k][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(f) for s, f in evs]
-    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    ms_per_step = wall * 1e3 / K
-    unknowns = n * m
-    value = unknowns / (wall / K)
-
-    # one more solve on a pristine RHS for the CPU-baseline agreement check
-    step(0, y_check)
-    torch.cuda.synchronize()
-    sweep_hist = None
-    if prec != "f64":
-        sweep_hist = torch.bincount(sweeps.long()).tolist()
-
-    achieved = BYTES_PER_UNKNOWN * unknowns / avg_kernel_s
-    traffic = measured_traffic(args.workload)
-    out = {
-        "metric": "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12",
-        "value": value,
-        "unit": "unknowns/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": prec,
-        "data": "synthetic (seeded counter-based generator, "
-                + ("strictly dominant distribution)" if kind == 0 else "weakly dominant distribution)"),
-        "config": {"workload": args.workload, "n": n, "m": m, "layout": "interleaved",
-                   "precision": prec, "seed": seed,
-                   "generator": "dominant" if kind == 0 else "weak",
-                   "kernel_variant": (f"{variant['kind']}-{variant['nt']}x{variant['rpt']}"
-                                      if variant else None),
-                   "split": list(pcrpenta.split_tune_get()[:3]) if n > 1024 else None,
-                   "refine": ({"max_sweeps": MAX_SWEEPS, "tol": TOL, "sweeps_hist": sweep_hist}
-                              if prec != "f64" else None)},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel_ms": avg_kernel_s * 1e3},
-    }
-    if not args.no_cpu:
-        nsys = min(args.cpu_systems, m)
-        xs = y_check.view(n, m)[:, :nsys].cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(n, m, layout, kind, seed, xs, nsys, prec)
-    print(json.dumps(out), flush=True)
+    if args.workload == HEADLINE:
+        return bench_headline(args)
+    return bench_single(args)
 
 
 if __name__ == "__main__":

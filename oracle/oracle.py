@@ -66,6 +66,8 @@ def _declare(lib):
                                                        ctypes.POINTER(ctypes.c_int),
                                                        ctypes.c_int, ctypes.c_int]
     lib.oracle_pcr_uneven_f64.restype = None
+    lib.oracle_gbsv_uneven_f64.argtypes = lib.oracle_pcr_uneven_f64.argtypes
+    lib.oracle_gbsv_uneven_f64.restype = ctypes.c_int
     lib.oracle_max_threads.argtypes = []
     lib.oracle_max_threads.restype = ctypes.c_int
     return lib
@@ -180,6 +182,45 @@ def pcr_uneven(a, b, c, d, e, y, offsets, n_sys, nthreads: int = 0):
                                  off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                  ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(ns), nthreads)
     return x
+
+
+def gbsv_uneven(a, b, c, d, e, y, offsets, n_sys, nthreads: int = 0):
+    """dgbsv (kl = ku = 2) of every system of an uneven batch."""
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    ns = np.ascontiguousarray(n_sys, dtype=np.int32)
+    x = np.zeros(len(a))
+    bad = load().oracle_gbsv_uneven_f64(
+        *[_p(np.ascontiguousarray(v)) for v in (a, b, c, d, e, y)], _p(x),
+        off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+        ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(ns), nthreads)
+    if bad:
+        raise np.linalg.LinAlgError(f"dgbsv: {bad} singular systems")
+    return x
+
+
+def uneven_relerr(x, ref, offsets, n_sys):
+    """Per-system ||x - ref||_inf / ||ref||_inf of an uneven batch (array of m)."""
+    off = np.asarray(offsets, dtype=np.int64)
+    ns = np.asarray(n_sys, dtype=np.int64)
+    end = int(off[-1] + ns[-1])
+    num = np.maximum.reduceat(np.abs(x[:end] - ref[:end]), off)
+    den = np.maximum.reduceat(np.abs(ref[:end]), off)
+    # reduceat over a gap or an empty system would be wrong: systems are back to back
+    assert np.array_equal(off[1:], off[:-1] + ns[:-1])
+    return num / np.where(den == 0, 1.0, den)
+
+
+def load_native(outdir: str):
+    """Build this oracle for the host it runs on (gcc -O3 -march=native, used
+    for the CPU baseline only: -march=native code cannot travel between
+    machines) into outdir and make it the library every function here uses.
+    The explicit fma() calls are still the only fused operations, so results
+    keep the same bits."""
+    global _lib
+    path = os.path.join(outdir, "liboracle_native.so")
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={path}"], check=True)
+    _lib = _declare(ctypes.CDLL(path))
+    return path
 
 
 def max_threads() -> int:

@@ -181,3 +181,28 @@ void oracle_pcr_uneven_f64(const double *a, const double *b, const double *c,
         free(work);
     }
 }
+
+int oracle_gbsv_uneven_f64(const double *a, const double *b, const double *c,
+                           const double *d, const double *e, const double *y,
+                           double *x, const int64_t *off, const int *n_sys, int m,
+                           int nthreads)
+{
+    if (nthreads <= 0) nthreads = oracle_max_threads();
+    int nmax = 1, bad = 0;
+    for (int j = 0; j < m; ++j) if (n_sys[j] > nmax) nmax = n_sys[j];
+#pragma omp parallel num_threads(nthreads) reduction(+ : bad)
+    {
+        double *ab = (double *)malloc(sizeof(double) * 7 * (size_t)nmax);
+        int *ipiv = (int *)malloc(sizeof(int) * (size_t)nmax);
+#pragma omp for schedule(dynamic, 1)
+        for (int j = 0; j < m; ++j) {
+            const int64_t o = off[j];
+            if (oracle_gbsv_f64(a + o, b + o, c + o, d + o, e + o, y + o, x + o, n_sys[j], ab,
+                                ipiv) != 0)
+                ++bad;
+        }
+        free(ipiv);
+        free(ab);
+    }
+    return bad;
+}

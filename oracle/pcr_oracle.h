@@ -67,6 +67,12 @@ void oracle_pcr_uneven_f64(const double *a, const double *b, const double *c,
                            const double *d, const double *e, const double *y,
                            double *x, const int64_t *off, const int *n_sys, int m,
                            int nthreads);
+/* dgbsv (kl = ku = 2) of every system of an uneven batch; returns the number
+ * of singular systems. */
+int oracle_gbsv_uneven_f64(const double *a, const double *b, const double *c,
+                           const double *d, const double *e, const double *y,
+                           double *x, const int64_t *off, const int *n_sys, int m,
+                           int nthreads);
 int oracle_max_threads(void);
 
 #ifdef __cplusplus
