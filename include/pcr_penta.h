@@ -166,6 +166,47 @@ int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t *d_offsets, cons
                          double *y, void *stream);
 
 /*
+ * Factor once, solve many (no counterpart in the reference, which recomputes
+ * every multiplier on every solve: kernel.cu:72-75, 98-103, 122, 129).
+ * The matrix part of the solve depends on a..e only; these entry points
+ * compute it once into a caller-owned DEVICE buffer and then solve right-hand
+ * sides against it.
+ *   shared = 0: m systems in `layout`, one factor block per system;
+ *   shared = 1: a..e hold ONE system of n rows (row i at element i; m and
+ *               layout are ignored by the factor call) whose factor serves
+ *               every one of the m right-hand sides of a solve.
+ * pcrp_factor_size: bytes of the factor buffer (0 if unsupported).
+ * FP64: pcrp_factor_f64 keeps, per reduction step and row, the pair-equation
+ * multipliers and Reduction-III's gamma1/gamma2 (after the zero-pivot rule),
+ * plus the reduced c, b, d of the final 2x2 / 1x1 solve: (4 floor(log2 n) + 3)
+ * n doubles per system. pcrp_solve_factored_f64 then costs 4 FMA per row and
+ * step; its result equals pcrp_solve_f64 on the same matrices bit for bit
+ * (signed zeros aside). n <= 1024.
+ * FP32+refine: pcrp_factor_f32r keeps the 2x2-block FP32 factor of
+ * pcrp_solve_f32r_f64's one-workgroup kernel (the level multipliers, the last
+ * D^-1, the level count); pcrp_solve_factored_f32r_f64 runs the same
+ * refinement (same arguments and stopping rule) without refactoring, and
+ * still needs a..e for its FP64 residuals (the same arrays, and the same
+ * `shared` choice, as the factor call). n <= 2048.
+ * The solve arguments (n, m, layout, shared) must be those of the factor call;
+ * the factor buffer is read-only during solves, so solves of one factor may
+ * run concurrently on several streams.
+ */
+size_t pcrp_factor_size(int n, int m, int precision, int shared);
+int pcrp_factor_f64(const double *a, const double *b, const double *c,
+                    const double *d, const double *e, int n, int m, int layout,
+                    int shared, void *factor, size_t factor_bytes, void *stream);
+int pcrp_solve_factored_f64(const void *factor, double *y, int n, int m, int layout,
+                            int shared, void *stream);
+int pcrp_factor_f32r(const double *a, const double *b, const double *c,
+                     const double *d, const double *e, int n, int m, int layout,
+                     int shared, void *factor, size_t factor_bytes, void *stream);
+int pcrp_solve_factored_f32r_f64(const void *factor, const double *a, const double *b,
+                                 const double *c, const double *d, const double *e,
+                                 double *y, int n, int m, int layout, int shared,
+                                 int max_sweeps, double tol, int *sweeps_out, void *stream);
+
+/*
  * Heston ADI driver (the application of the PCR-Penta paper; the reference
  * ships no ADI code). European call under Heston, Douglas scheme, uniform
  * grid: Ns unknowns in S (S = i Smax/(Ns+1), i = 1..Ns; u = 0 at S = 0,
@@ -174,10 +215,17 @@ int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t *d_offsets, cons
  * in S (second order next to the S boundaries), second-order upwinding plus
  * central diffusion in v, explicit mixed term. Each step solves Nv S-lines
  * (contiguous, n = Ns) and Ns v-lines (interleaved, n = Nv) with
- * pcrp_solve_f64 or pcrp_solve_f32r_f64.
+ * pcrp_solve_f64 or pcrp_solve_f32r_f64 (the S-line solve stores Y1 + W,
+ * the v-line right-hand side, directly). reuse_factor = 1 factors the line
+ * matrices once before the first step: FP64 every S-line and the one matrix
+ * all v-lines share (the v-line coefficients depend on v only), FP32+refine
+ * the shared v-line matrix (its S-lines keep the full solve, which measured
+ * as fast as reading their factor); needs Ns <= 1024 (FP64) and Nv <= 1024
+ * (FP64) or 2048 (FP32+refine), else PCRP_ERR_UNSUPPORTED.
  * u: DEVICE array of Ns*Nv doubles, u[j*Ns + i-1], receives the price at
- * tau = steps*dt. use_graph = 1 captures one time step in a hipGraph and
- * replays it (needs a non-NULL stream; the call then synchronises `stream`
+ * tau = steps*dt. use_graph = 1 captures a sequence of up to 25 time steps
+ * in a hipGraph and replays it (plus one graph for the remaining steps;
+ * needs a non-NULL stream; the call then synchronises `stream`
  * before releasing the graph); use_graph = 0 enqueues every step.
  */
 /* Refinement settings of the FP32 line solves inside pcrp_heston_solve:
@@ -192,6 +240,9 @@ typedef struct {
     double dt;                  /* time step */
     int steps;                  /* number of steps */
     double theta_scheme;        /* Douglas theta (0.5) */
+    int reuse_factor;           /* 1: factor the constant line matrices once
+                                   (pcrp_factor_*), solve right-hand sides each
+                                   step; 0: full line solves every step */
 } pcrp_heston_params;
 
 size_t pcrp_heston_workspace_size(const pcrp_heston_params *p, int precision);

@@ -65,10 +65,13 @@ __device__ __forceinline__ Pair<double> make_pair(const Row<double>& L, const Ro
 // hq2: i+2d <= n-1. alL/alR/arL/arR: multipliers of EL/ER, consulted only when
 // a pivot EL.C / ER.C is exactly zero (kernel.cu:122,129 keep the multiplier
 // assigned last: eq2's if it exists, else eq1's).
+// gout (optional): receives the final multipliers g1, g2, which with the pair
+// multipliers al, ar are all a right-hand side needs (the factored solve).
 template <typename LoadAl>
 __device__ __forceinline__ Row<double> reduce_row(const Row<double>& M, const Pair<double>& EL,
                                                   const Pair<double>& ER, bool hp1, bool hq1,
-                                                  bool hp2, bool hq2, LoadAl&& load_multipliers)
+                                                  bool hp2, bool hq2, LoadAl&& load_multipliers,
+                                                  double* gout = nullptr)
 {
     double g1 = M.b / EL.C;   // kernel.cu:122
     double g2 = M.d / ER.C;   // kernel.cu:129
@@ -80,6 +83,10 @@ __device__ __forceinline__ Row<double> reduce_row(const Row<double>& M, const Pa
     }
     g1 = hp1 ? g1 : 0.0;
     g2 = hq1 ? g2 : 0.0;
+    if (gout) {
+        gout[0] = g1;
+        gout[1] = g2;
+    }
     Row<double> N;
     N.a = g1 * EL.A;
     N.b = __builtin_fma(g2, ER.A, __builtin_fma(g1, EL.B, -M.a));

@@ -21,9 +21,16 @@
 //   (I - th dt A1) Y1 = Y0 - th dt (A1 U + b1(tau_n)) + th dt b1(tau_n+1)
 //   (I - th dt A2) Y2 = Y1 - th dt (A2 U + b2(tau_n)) + th dt b2(tau_n+1)
 //   U_{n+1} = Y2.
-// A step is four launches on one stream with fixed pointers (explicit
-// kernel, S-line solve, combine kernel, v-line solve); the step index lives
-// in device memory, so one captured hipGraph replays every step.
+// A step is four launches on one stream with fixed pointers: the explicit
+// kernel, a one-thread kernel advancing the device-side step index, the
+// S-line solve, which stores Y1 + W (the v-line right-hand side) straight
+// into U, and the v-line solve in place. The step index lives in
+// device memory, so one captured hipGraph replays every step.
+// With reuse_factor the line matrices (constant over the run) are factored
+// once before the first step: FP64 keeps the PCR-Penta multipliers of every
+// S-line and the single v-line matrix (all v-lines share one matrix, since
+// A2 depends on v only), FP32+refine the 2x2-block factor of the v-line
+// matrix; each step then solves right-hand sides only for those lines.
 #include "pcr_penta.h"
 #include "pcrp_internal.h"
 
@@ -201,12 +208,14 @@ __device__ inline double apply_a0(const Grid& g, const double* U, int i, int j, 
 }
 
 // explicit stage: R1 = RHS of the S-line systems, W = v-line correction
-__global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, const int* step)
+// (step index: the device counter plus this step's offset within its chunk)
+__global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, const int* step,
+                                int off)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)g.ns * g.nv) return;
     const int j = (int)(idx / g.ns), i = (int)(idx % g.ns) + 1;
-    const double tau0 = (*step) * g.dt, tau1 = tau0 + g.dt;
+    const double tau0 = (*step + off) * g.dt, tau1 = tau0 + g.dt;
     const double bS0 = far_s(g, tau0), bS1 = far_s(g, tau1);
     const double u = U[idx];
     double w1[5], w2[5];   // each stencil's coefficients once per node
@@ -221,10 +230,15 @@ __global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, 
     W[idx] = -th * l2 + th * bnd_a2(g, i, j, bS1, w2);
 }
 
-__global__ void combine_kernel(Grid g, const double* Y1, const double* W, double* X, int* step)
+// the next chunk's first step index, once its explicit kernels have read
+// this one (one thread: a completion counter across the explicit kernel's
+// blocks measured 30 us per step in atomics)
+__global__ void advance_kernel(int* step, int by) { *step += by; }
+
+// X = Y1 + W, for grids whose S-line solve cannot store it directly
+__global__ void combine_kernel(Grid g, const double* Y1, const double* W, double* X)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx == 0) *step += 1;   // read by the next step's explicit kernel only
     if (idx >= (long long)g.ns * g.nv) return;
     X[idx] = Y1[idx] + W[idx];
 }
@@ -254,6 +268,8 @@ struct HestonWs {
     int* step;
     void* solver_ws;
     size_t solver_bytes;
+    void* fS;   // reuse_factor, FP64: factors of the S-lines
+    void* fV;   // reuse_factor: the shared factor of the v-line matrix
 };
 
 size_t solver_ws_bytes(const pcrp_heston_params* p, int precision)
@@ -263,12 +279,34 @@ size_t solver_ws_bytes(const pcrp_heston_params* p, int precision)
     return a > b ? a : b;
 }
 
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// factor buffers of the reuse_factor form (0 when not used)
+size_t factor_s_bytes(const pcrp_heston_params* p, int precision)
+{
+    if (!p->reuse_factor || precision != PCRP_PREC_F64) return 0;
+    return align256(pcrp_factor_size(p->ns, p->nv, PCRP_PREC_F64, 0));
+}
+
+size_t factor_v_bytes(const pcrp_heston_params* p, int precision)
+{
+    if (!p->reuse_factor) return 0;
+    return align256(pcrp_factor_size(p->nv, p->ns, precision, 1));
+}
+
+// the reuse_factor form needs line sizes the factored solves take
+bool reuse_supported(const pcrp_heston_params* p, int precision)
+{
+    if (!p->reuse_factor) return true;
+    if (factor_v_bytes(p, precision) == 0) return false;
+    return precision != PCRP_PREC_F64 || factor_s_bytes(p, precision) != 0;
+}
+
 size_t heston_ws_bytes(const pcrp_heston_params* p, int precision)
 {
     const size_t N = (size_t)p->ns * p->nv;
-    size_t bytes = 12 * N * sizeof(double) + 256;
-    bytes = (bytes + 255) / 256 * 256;
-    return bytes + solver_ws_bytes(p, precision);
+    return align256(12 * N * sizeof(double) + 256) + solver_ws_bytes(p, precision) +
+           factor_s_bytes(p, precision) + factor_v_bytes(p, precision);
 }
 
 HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
@@ -280,38 +318,149 @@ HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
                           &h.vb, &h.vc, &h.vd, &h.ve, &h.R1, &h.W};
     for (int k = 0; k < 12; ++k) *slots[k] = d + (size_t)k * N;
     h.step = reinterpret_cast<int*>(d + 12 * N);
-    size_t off = 12 * N * sizeof(double) + 256;
-    off = (off + 255) / 256 * 256;
-    h.solver_ws = static_cast<char*>(ws) + off;
+    char* c = static_cast<char*>(ws) + align256(12 * N * sizeof(double) + 256);
+    h.solver_ws = c;
     h.solver_bytes = solver_ws_bytes(p, precision);
+    c += h.solver_bytes;
+    h.fS = factor_s_bytes(p, precision) ? c : nullptr;
+    c += factor_s_bytes(p, precision);
+    h.fV = factor_v_bytes(p, precision) ? c : nullptr;
     return h;
 }
 
-int solve_lines(int precision, const double* a, const double* b, const double* c, const double* d,
-                const double* e, double* y, int n, int m, int layout, const HestonWs& h,
-                hipStream_t st)
+// Factors of the reuse_factor form, once, before the first step.
+int factor_lines(const Grid& g, int precision, const HestonWs& h, hipStream_t st)
 {
+    if (h.fS && factor_f64(h.sa, h.sb, h.sc, h.sd, h.se, g.ns, g.nv, 1, g.ns,
+                           static_cast<double*>(h.fS), st) != 0)
+        return PCRP_ERR_HIP;
+    if (!h.fV) return PCRP_OK;
+    // v-line 0's rows (element j at j*ns) are the matrix of every v-line
     if (precision == PCRP_PREC_F64)
-        return pcrp_solve_f64(a, b, c, d, e, y, n, m, layout, h.solver_ws, h.solver_bytes, st);
-    return pcrp_solve_f32r_f64(a, b, c, d, e, y, n, m, layout, PCRP_HESTON_MAX_SWEEPS,
-                               PCRP_HESTON_REFINE_TOL, nullptr, h.solver_ws, h.solver_bytes, st);
+        return factor_f64(h.va, h.vb, h.vc, h.vd, h.ve, g.nv, 1, g.ns, 0,
+                          static_cast<double*>(h.fV), st) == 0
+                   ? PCRP_OK
+                   : PCRP_ERR_HIP;
+    SolveArgsF32R A{h.va, h.vb, h.vc, h.vd, h.ve, nullptr, g.nv, g.ns, g.ns, 1, pcr_steps(g.nv),
+                    0, 0.0, nullptr};
+    A.shared = true;
+    A.ars = g.ns;
+    return lds_f32r_factor(A, h.fV, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
 }
 
-// one Douglas step: four launches, fixed pointers
-int enqueue_step(const Grid& g, int precision, double* U, const HestonWs& h, hipStream_t st)
+// S-lines: R1 (contiguous, m = nv systems of n = ns) solved, Y1 + W stored in
+// U. Returns PCRP_ERR_UNSUPPORTED when the solver has no such store for this
+// size (the caller then solves in place and adds W separately).
+int solve_s_lines(const Grid& g, int precision, double* U, const HestonWs& h, hipStream_t st)
+{
+    const int n = g.ns, m = g.nv;
+    if (precision == PCRP_PREC_F64 && h.fS) {
+        FactoredF64 A{static_cast<const double*>(h.fS), false, h.R1, U, h.W, n, m, 1, n};
+        return solve_factored_f64(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    }
+    if (precision == PCRP_PREC_F64) {
+        if (split_f64_levels(n, split_f64_get().lds_max) != 0) return PCRP_ERR_UNSUPPORTED;
+        SolveArgsF64 A{h.sa, h.sb, h.sc, h.sd, h.se, h.R1, n, m, 1, n, pcr_steps(n),
+                       SplitInfo{0, n, m, PCRP_LAYOUT_CONTIGUOUS, U}};
+        A.split.add = h.W;
+        const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, n, m));
+        return v->launch(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    }
+    if (n > lds_f32r_max_n() || window_preferred(n, m)) return PCRP_ERR_UNSUPPORTED;
+    SolveArgsF32R A{h.sa, h.sb, h.sc, h.sd, h.se, h.R1, n, m, 1, n, pcr_steps(n),
+                    PCRP_HESTON_MAX_SWEEPS, PCRP_HESTON_REFINE_TOL, nullptr};
+    A.out = U;
+    A.add = h.W;
+    return lds_f32r_solve(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+}
+
+// v-lines: U (interleaved, m = ns systems of n = nv) solved in place
+int solve_v_lines(const Grid& g, int precision, double* U, const HestonWs& h, hipStream_t st)
+{
+    const int n = g.nv, m = g.ns;
+    if (h.fV && precision == PCRP_PREC_F64) {
+        FactoredF64 A{static_cast<const double*>(h.fV), true, U, nullptr, nullptr, n, m, m, 1};
+        return solve_factored_f64(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    }
+    if (h.fV) {
+        SolveArgsF32R A{h.va, h.vb, h.vc, h.vd, h.ve, U, n, m, m, 1, pcr_steps(n),
+                        PCRP_HESTON_MAX_SWEEPS, PCRP_HESTON_REFINE_TOL, nullptr};
+        A.shared = true;
+        A.ars = m;
+        A.fct = h.fV;
+        return lds_f32r_solve(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+    }
+    if (precision == PCRP_PREC_F64)
+        return pcrp_solve_f64(h.va, h.vb, h.vc, h.vd, h.ve, U, n, m, PCRP_LAYOUT_INTERLEAVED,
+                              h.solver_ws, h.solver_bytes, st);
+    return pcrp_solve_f32r_f64(h.va, h.vb, h.vc, h.vd, h.ve, U, n, m, PCRP_LAYOUT_INTERLEAVED,
+                               PCRP_HESTON_MAX_SWEEPS, PCRP_HESTON_REFINE_TOL, nullptr,
+                               h.solver_ws, h.solver_bytes, st);
+}
+
+// One Douglas step (step index: device counter + off): three launches (four
+// when the S-line solve cannot store Y1 + W itself), fixed pointers.
+int enqueue_step(const Grid& g, int precision, double* U, const HestonWs& h, int off,
+                 hipStream_t st)
 {
     const long long N = (long long)g.ns * g.nv;
     const int bs = 256;
     const unsigned blocks = (unsigned)((N + bs - 1) / bs);
-    hipLaunchKernelGGL(explicit_kernel, dim3(blocks), dim3(bs), 0, st, g, U, h.R1, h.W, h.step);
+    hipLaunchKernelGGL(explicit_kernel, dim3(blocks), dim3(bs), 0, st, g, U, h.R1, h.W, h.step,
+                       off);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
-    int rc = solve_lines(precision, h.sa, h.sb, h.sc, h.sd, h.se, h.R1, g.ns, g.nv,
-                         PCRP_LAYOUT_CONTIGUOUS, h, st);
-    if (rc != PCRP_OK) return rc;
-    hipLaunchKernelGGL(combine_kernel, dim3(blocks), dim3(bs), 0, st, g, h.R1, h.W, U, h.step);
-    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
-    return solve_lines(precision, h.va, h.vb, h.vc, h.vd, h.ve, U, g.nv, g.ns,
-                       PCRP_LAYOUT_INTERLEAVED, h, st);
+    int rc = solve_s_lines(g, precision, U, h, st);
+    if (rc == PCRP_ERR_UNSUPPORTED) {
+        rc = precision == PCRP_PREC_F64
+                 ? pcrp_solve_f64(h.sa, h.sb, h.sc, h.sd, h.se, h.R1, g.ns, g.nv,
+                                  PCRP_LAYOUT_CONTIGUOUS, h.solver_ws, h.solver_bytes, st)
+                 : pcrp_solve_f32r_f64(h.sa, h.sb, h.sc, h.sd, h.se, h.R1, g.ns, g.nv,
+                                       PCRP_LAYOUT_CONTIGUOUS, PCRP_HESTON_MAX_SWEEPS,
+                                       PCRP_HESTON_REFINE_TOL, nullptr, h.solver_ws,
+                                       h.solver_bytes, st);
+        if (rc != PCRP_OK) return rc;
+        hipLaunchKernelGGL(combine_kernel, dim3(blocks), dim3(bs), 0, st, g, h.R1, h.W, U);
+        if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    } else if (rc != PCRP_OK) {
+        return rc;
+    }
+    return solve_v_lines(g, precision, U, h, st);
+}
+
+// `count` consecutive steps, then the counter moves past them
+int enqueue_steps(const Grid& g, int precision, double* U, const HestonWs& h, int count,
+                  hipStream_t st)
+{
+    for (int k = 0; k < count; ++k) {
+        const int rc = enqueue_step(g, precision, U, h, k, st);
+        if (rc != PCRP_OK) return rc;
+    }
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, st, h.step, count);
+    return hipGetLastError() == hipSuccess ? PCRP_OK : PCRP_ERR_HIP;
+}
+
+// Steps per captured graph: one graph launch per chunk instead of per step
+// (each graph launch left ~8 us between steps on MI355X), with the counter
+// advanced once per chunk.
+constexpr int kStepsPerGraph = 25;
+
+// Captures `count` steps into an executable graph.
+int capture_steps(const Grid& g, int precision, double* U, const HestonWs& h, int count,
+                  hipStream_t st, hipGraphExec_t* exec)
+{
+    hipGraph_t graph = nullptr;
+    *exec = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return PCRP_ERR_HIP;
+    const int rc = enqueue_steps(g, precision, U, h, count, st);
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    int out = rc != PCRP_OK ? rc : (ec != hipSuccess ? PCRP_ERR_HIP : PCRP_OK);
+    if (out == PCRP_OK && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        *exec = nullptr;
+        out = PCRP_ERR_HIP;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    return out;
 }
 
 }  // namespace
@@ -334,6 +483,8 @@ int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph,
     if ((long long)p->ns * p->nv > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
     if (precision != PCRP_PREC_F64 && precision != PCRP_PREC_F32_REFINE) return PCRP_ERR_ARG;
     if (!(p->dt > 0.0) || !(p->smax > 0.0) || !(p->vmax > 0.0)) return PCRP_ERR_ARG;
+    if (p->reuse_factor != 0 && p->reuse_factor != 1) return PCRP_ERR_ARG;
+    if (!reuse_supported(p, precision)) return PCRP_ERR_UNSUPPORTED;
     if (ws_bytes < heston_ws_bytes(p, precision)) return PCRP_ERR_WORKSPACE;
     hipStream_t st = static_cast<hipStream_t>(stream);
     const Grid g = make_grid(p);
@@ -345,36 +496,31 @@ int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph,
                        h.sd, h.se, h.va, h.vb, h.vc, h.vd, h.ve);
     hipLaunchKernelGGL(payoff_kernel, dim3(blocks), dim3(bs), 0, st, g, u, h.step);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    const int frc = factor_lines(g, precision, h, st);
+    if (frc != PCRP_OK) return frc;
     if (!use_graph) {
         for (int n = 0; n < p->steps; ++n) {
-            const int rc = enqueue_step(g, precision, u, h, st);
+            const int rc = enqueue_steps(g, precision, u, h, 1, st);
             if (rc != PCRP_OK) return rc;
         }
         return PCRP_OK;
     }
     if (p->steps == 0) return PCRP_OK;
     if (st == nullptr) return PCRP_ERR_ARG;   // the null stream cannot be captured
-    // capture one step, replay it `steps` times
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return PCRP_ERR_HIP;
-    const int rc = enqueue_step(g, precision, u, h, st);
-    const hipError_t ec = hipStreamEndCapture(st, &graph);
-    if (rc != PCRP_OK || ec != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc != PCRP_OK ? rc : PCRP_ERR_HIP;
-    }
-    int out = PCRP_OK;
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    // one graph of up to kStepsPerGraph steps replayed, and one for the rest
+    const int K = p->steps < kStepsPerGraph ? p->steps : kStepsPerGraph;
+    const int full = p->steps / K, rest = p->steps % K;
+    hipGraphExec_t exec = nullptr, exec_rest = nullptr;
+    int out = capture_steps(g, precision, u, h, K, st, &exec);
+    if (out == PCRP_OK && rest > 0) out = capture_steps(g, precision, u, h, rest, st, &exec_rest);
+    for (int c = 0; c < full && out == PCRP_OK; ++c)
+        if (hipGraphLaunch(exec, st) != hipSuccess) out = PCRP_ERR_HIP;
+    if (out == PCRP_OK && exec_rest && hipGraphLaunch(exec_rest, st) != hipSuccess)
         out = PCRP_ERR_HIP;
-    } else {
-        for (int n = 0; n < p->steps && out == PCRP_OK; ++n)
-            if (hipGraphLaunch(exec, st) != hipSuccess) out = PCRP_ERR_HIP;
-        // replays finish before the executable graph is released
-        if (hipStreamSynchronize(st) != hipSuccess) out = PCRP_ERR_HIP;
-        if (hipGraphExecDestroy(exec) != hipSuccess) out = PCRP_ERR_HIP;
-    }
-    (void)hipGraphDestroy(graph);
+    // replays finish before the executable graphs are released
+    if ((exec || exec_rest) && hipStreamSynchronize(st) != hipSuccess) out = PCRP_ERR_HIP;
+    if (exec && hipGraphExecDestroy(exec) != hipSuccess) out = PCRP_ERR_HIP;
+    if (exec_rest && hipGraphExecDestroy(exec_rest) != hipSuccess) out = PCRP_ERR_HIP;
     return out;
 }
 

@@ -17,7 +17,8 @@ struct SplitInfo {
     int S;            // steps already done in HBM passes (0: not a split solve)
     int n0, m0;       // original system size and count
     int out_layout;   // caller's layout (PCRP_LAYOUT_*) for x
-    double* x;        // caller's y (receives x)
+    double* x;        // caller's y (receives x); S == 0: x's array (y's indexing)
+    const double* add = nullptr;   // S == 0: x + add[] is stored (y's indexing)
 };
 
 struct SolveArgsF64 {
@@ -76,6 +77,12 @@ struct SolveArgsF32R {
     int max_sweeps;
     double tol;
     int* sweeps_out;
+    // one-workgroup kernel only (lds_f32r_solve / lds_f32r_factor):
+    bool shared = false;          // a..e hold one system (row i at i*ars) for every y
+    long long ars = 0;            // shared: row stride of a..e
+    double* out = nullptr;        // x goes here (null: into y)
+    const double* add = nullptr;  // optional: x + add[] is stored (y's indexing)
+    const void* fct = nullptr;    // cached FP32 factor (pcrp_factor_f32r)
 };
 
 // Variant the tuning table assigns to an LDS solve of size n (pcrp_api.hip).
@@ -103,6 +110,9 @@ int pass_shape_single(int shape);
 
 int lds_f32r_max_n();
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);
+// Writes the FP32 block factor of A's systems (of the one system when
+// A.shared) to F; a later lds_f32r_solve with A.fct = F skips the factor.
+int lds_f32r_factor(const SolveArgsF32R& A, void* F, hipStream_t st);
 
 // FP32 split solve + FP64 refinement for n above lds_f32r_max_n().
 size_t split_f32r_workspace(int n, int m);
@@ -127,6 +137,24 @@ int window_f32r_uneven(const double* a, const double* b, const double* c, const 
                        const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
                        int max_sweeps, double tol, int* sweeps_out, void* ws, size_t ws_bytes,
                        hipStream_t st, bool& fell_back);
+
+// Factor once / solve many (pcrp_factor.hip). A factor holds, per system,
+// (4 steps + 3) n doubles; shared: one factor for every right-hand side.
+struct FactoredF64 {
+    const double* F;
+    bool shared;
+    double* y;            // right-hand sides
+    double* out;          // x (null: in place in y)
+    const double* add;    // optional: x + add[] is stored (same indexing as y)
+    int n, m;
+    long long rs, ss;     // element i of right-hand side j at j*ss + i*rs
+};
+long long factor_f64_words(int n);
+int factor_f64(const double* a, const double* b, const double* c, const double* d,
+               const double* e, int n, int msys, long long rs, long long ss, double* F,
+               hipStream_t st);
+int solve_factored_f64(const FactoredF64& A, hipStream_t st);
+size_t factor_f32r_bytes(int n);
 
 inline int pcr_steps(long long n)
 {

@@ -73,12 +73,29 @@ __device__ __forceinline__ void block_max2(double& u, double& v, double (&red)[2
 // phase. The G systems then run the same solve on their own LDS regions;
 // every barrier is workgroup-wide and a system that has converged idles
 // until the last one stops.
-template <int NT, int BR, int LMAX, int G>
-__global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
+// MODE: kSolve (factor, then the sweeps), kFactor (factor only: the
+// multipliers, D^-1 of the last level and the level count go to fct, nothing
+// else is written), kCached (the factor is read from fct instead of being
+// computed; the sweeps and their FP64 residuals are the same). The matrix
+// is read at ga..ge[j*ass + i*ars]: ass = 0 gives every system one shared
+// matrix (its factor, one block of fct, serves all of them). x goes to gx at
+// y's element, plus add[] there when add is not null.
+enum { kSolve = 0, kFactor = 1, kCached = 2 };
+
+// Bytes of one system's cached factor for NB block rows and LMAX levels:
+// the level count (16 bytes), alpha and beta of every level, D^-1.
+__host__ __device__ constexpr long long f32r_factor_bytes_nb(int NB, int LMAX)
+{
+    return 16 + 16LL * NB * (2 * LMAX + 1);
+}
+
+template <int NT, int BR, int LMAX, int G, int MODE, int MINW = 1>
+__global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
-    const double* __restrict__ gd, const double* __restrict__ ge, double* __restrict__ gy,
-    int n, int m, long long rs, long long ss, int max_sweeps, double tol,
-    int* __restrict__ sweeps_out)
+    const double* __restrict__ gd, const double* __restrict__ ge, const double* gy,
+    double* gx, const double* __restrict__ add, int n, int m, long long rs, long long ss,
+    long long ars, long long ass, int max_sweeps, double tol, int* __restrict__ sweeps_out,
+    float4* __restrict__ fct)
 {
     constexpr int NB = NT * BR;        // block rows
     constexpr int NMAX = 2 * NB;       // rows
@@ -105,7 +122,16 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
     const int grp = xcd_remap(blockIdx.x, m / G);
     const int sys = grp * G + gs;
     const long long base = (long long)sys * ss;
+    const long long abase = (long long)sys * ass;
     const int nb = (n + 1) >> 1;
+    // this system's factor block (the shared one when ass == 0)
+    float4* const Fc = reinterpret_cast<float4*>(reinterpret_cast<char*>(fct) +
+                                                 (ass == 0 ? 0LL : (long long)sys) *
+                                                     f32r_factor_bytes_nb(NB, LMAX));
+    int* const Fc_levels = reinterpret_cast<int*>(Fc);
+    float4* const Fc_al = Fc + 1;
+    float4* const Fc_be = Fc_al + (long long)LMAX * NB;
+    float4* const Fc_fin = Fc_be + (long long)LMAX * NB;
     double(&A)[6][NMAX] = sA[gs];
     float4(&F)[3][NB] = sF[gs];
     float2(&Yx)[2][NB] = sY[gs];
@@ -127,8 +153,14 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
             for (int i = (int)threadIdx.x; i < n; i += NT * G) {
-                const double2* src = reinterpret_cast<const double2*>(arr[f] + (long long)i * rs + j0);
                 const int p = lds_pos(i);
+                if (f < 5 && ass == 0) {   // one shared matrix row for the G systems
+                    const double w = band(f, i, arr[f][(long long)i * ars]);
+#pragma unroll
+                    for (int h = 0; h < G; ++h) sA[h][f][p] = w;
+                    continue;
+                }
+                const double2* src = reinterpret_cast<const double2*>(arr[f] + (long long)i * rs + j0);
 #pragma unroll
                 for (int h = 0; h < G / 2; ++h) {
                     const double2 w = src[h];
@@ -154,9 +186,10 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
             if (i < n) {
                 const int p = lds_pos(i);
                 if constexpr (G == 1) {
-                    const long long g = base + (long long)i * rs;
+                    const long long g = abase + (long long)i * ars;
                     a = band(0, i, ga[g]); b = band(1, i, gb[g]); c = gc[g];
-                    d = band(3, i, gd[g]); e = band(4, i, ge[g]); y = gy[g];
+                    d = band(3, i, gd[g]); e = band(4, i, ge[g]);
+                    y = MODE == kFactor ? 0.0 : gy[base + (long long)i * rs];
                     A[0][p] = a; A[1][p] = b; A[2][p] = c; A[3][p] = d; A[4][p] = e;
                     A[5][p] = y;
                 } else {
@@ -176,11 +209,27 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
     // Truncated once a level's multipliers are all below kTruncF32 (every
     // system of the workgroup); the sweeps then run the same levels.
     M2 cA[LMAX][BR], cB[LMAX][BR];     // alpha, beta per level
+    M2 Fin[BR];
     int levels = 0;
     while ((1 << levels) < nb) ++levels;
+    if constexpr (MODE == kCached) {
+        // the factor of an earlier kFactor launch on the same matrix
+        levels = Fc_levels[0];
+#pragma unroll
+        for (int s = 0; s < LMAX; ++s)
+            if (s < levels)
+#pragma unroll
+                for (int q = 0; q < BR; ++q) {
+                    const int k = t + NT * q;
+                    cA[s][q] = m2(Fc_al[(long long)s * NB + k]);
+                    cB[s][q] = m2(Fc_be[(long long)s * NB + k]);
+                }
+#pragma unroll
+        for (int q = 0; q < BR; ++q) Fin[q] = m2(Fc_fin[t + NT * q]);
+    }
 #pragma unroll
     for (int s = 0; s < LMAX; ++s) {
-        if (s < levels) {
+        if (MODE != kCached && s < levels) {
             float am = 0.f;   // largest multiplier entry of this thread's block rows
             const int delta = 1 << s;
 #pragma unroll
@@ -221,9 +270,26 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
             if (!any) levels = s + 1;   // uniform: the remaining levels are negligible
         }
     }
-    M2 Fin[BR];
+    if constexpr (MODE != kCached) {
 #pragma unroll
-    for (int q = 0; q < BR; ++q) Fin[q] = minv(D[q]);
+        for (int q = 0; q < BR; ++q) Fin[q] = minv(D[q]);
+    }
+    if constexpr (MODE == kFactor) {
+        // one system per workgroup (G == 1): its factor block
+#pragma unroll
+        for (int s = 0; s < LMAX; ++s)
+            if (s < levels)
+#pragma unroll
+                for (int q = 0; q < BR; ++q) {
+                    const int k = t + NT * q;
+                    Fc_al[(long long)s * NB + k] = f4(cA[s][q]);
+                    Fc_be[(long long)s * NB + k] = f4(cB[s][q]);
+                }
+#pragma unroll
+        for (int q = 0; q < BR; ++q) Fc_fin[t + NT * q] = f4(Fin[q]);
+        if (t == 0) Fc_levels[0] = levels;
+        return;
+    }
 
     // ---------------- solve + refinement sweeps ----------------
     int sweep = 0;
@@ -359,33 +425,77 @@ __global__ __launch_bounds__(NT * G) void lds_f32r_kernel(
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int i = 2 * k + h;
-            if (i < n) gy[base + (long long)i * rs] = X[q][h];
+            if (i < n) {
+                const long long g = base + (long long)i * rs;
+                gx[g] = add ? X[q][h] + add[g] : X[q][h];
+            }
         }
     }
     if (sweeps_out && t == 0) sweeps_out[sys] = sweep;
 }
 
-template <int NT, int BR, int G>
-static int launch_f32r(const SolveArgsF32R& A, hipStream_t st)
+template <int NB>
+constexpr int lmax_for()
 {
-    constexpr int LMAX = (NT * BR >= 1024) ? 10 : (NT * BR >= 512) ? 9 : (NT * BR >= 256) ? 8
-                       : (NT * BR >= 128) ? 7 : 6;
+    return (NB >= 1024) ? 10 : (NB >= 512) ? 9 : (NB >= 256) ? 8 : (NB >= 128) ? 7 : 6;
+}
+
+// mode kFactor: one workgroup per matrix (A.m systems, or one when shared)
+template <int NT, int BR, int G>
+static int launch_f32r(const SolveArgsF32R& A, hipStream_t st, int mode = kSolve,
+                       void* F = nullptr)
+{
+    constexpr int LMAX = lmax_for<NT * BR>();
     static_assert((1 << LMAX) >= NT * BR, "LMAX covers ceil(log2 block rows)");
-    hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, G>), dim3(A.m / G), dim3(NT * G), 0, st,
-                       A.a, A.b, A.c, A.d, A.e, A.y, A.n, A.m, A.rs, A.ss, A.max_sweeps, A.tol,
-                       A.sweeps_out);
+    const long long ars = A.shared ? A.ars : A.rs, ass = A.shared ? 0 : A.ss;
+    double* const gx = A.out ? A.out : A.y;
+    float4* const f4p = static_cast<float4*>(const_cast<void*>(mode == kFactor ? F : A.fct));
+    if (mode == kFactor) {
+        if constexpr (G == 1)
+            hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, 1, kFactor>),
+                               dim3(A.shared ? 1 : A.m), dim3(NT), 0, st, A.a, A.b, A.c, A.d,
+                               A.e, A.y, gx, A.add, A.n, A.shared ? 1 : A.m, A.rs, A.ss, ars, ass,
+                               0, 0.0, nullptr, f4p);
+        else
+            return -2;
+    } else if (A.fct) {
+        hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, G, kCached>), dim3(A.m / G),
+                           dim3(NT * G), 0, st, A.a, A.b, A.c, A.d, A.e, A.y, gx, A.add, A.n,
+                           A.m, A.rs, A.ss, ars, ass, A.max_sweeps, A.tol, A.sweeps_out, f4p);
+    } else {
+        hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, G, kSolve>), dim3(A.m / G),
+                           dim3(NT * G), 0, st, A.a, A.b, A.c, A.d, A.e, A.y, gx, A.add, A.n,
+                           A.m, A.rs, A.ss, ars, ass, A.max_sweeps, A.tol, A.sweeps_out, f4p);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int lds_f32r_max_n() { return 2048; }
+
+// Block rows (capacity) and levels of the one-workgroup variant for size n:
+// the cached factor of a system takes f32r_factor_bytes_nb of them.
+static void f32r_shape(int n, int& nb_cap, int& lmax)
+{
+    nb_cap = n <= 128 ? 64 : n <= 256 ? 128 : n <= 512 ? 256 : n <= 1024 ? 512 : 1024;
+    lmax = nb_cap >= 1024 ? 10 : nb_cap >= 512 ? 9 : nb_cap >= 256 ? 8 : nb_cap >= 128 ? 7 : 6;
+}
+
+size_t factor_f32r_bytes(int n)
+{
+    int nb, lm;
+    f32r_shape(n, nb, lm);
+    return (size_t)f32r_factor_bytes_nb(nb, lm);
+}
 
 // Grouped loading needs an interleaved batch (rows m apart, systems adjacent)
 // whose size is a multiple of the group and six 16-byte aligned arrays.
 static bool groupable(const SolveArgsF32R& A, int G)
 {
     const auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
-    return A.ss == 1 && A.rs == A.m && A.m % G == 0 && al(A.a) && al(A.b) && al(A.c) &&
-           al(A.d) && al(A.e) && al(A.y);
+    // a cached factor of separate matrices is one block per system (levels may differ)
+    if (A.fct && !A.shared) return false;
+    return A.ss == 1 && A.rs == A.m && A.m % G == 0 && al(A.y) &&
+           (A.shared || (al(A.a) && al(A.b) && al(A.c) && al(A.d) && al(A.e)));
 }
 
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
@@ -421,6 +531,17 @@ int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
         return groupable(A, 2) && A.m > cus && A.m <= 2 * cus ? launch_f32r<256, 2, 2>(A, st)
                                                               : launch_f32r<256, 2, 1>(A, st);
     if (n <= 2048) return launch_f32r<512, 2, 1>(A, st);
+    return -2;
+}
+
+int lds_f32r_factor(const SolveArgsF32R& A, void* F, hipStream_t st)
+{
+    const int n = A.n;
+    if (n <= 128) return launch_f32r<64, 1, 1>(A, st, kFactor, F);
+    if (n <= 256) return launch_f32r<128, 1, 1>(A, st, kFactor, F);
+    if (n <= 512) return launch_f32r<128, 2, 1>(A, st, kFactor, F);
+    if (n <= 1024) return launch_f32r<256, 2, 1>(A, st, kFactor, F);
+    if (n <= 2048) return launch_f32r<512, 2, 1>(A, st, kFactor, F);
     return -2;
 }
 

@@ -183,7 +183,8 @@ template <int KIND, int NT, int RPT>
 __device__ __forceinline__ void lds_solve_regs(
     Row<double> (&r)[RPT], double* gx, long long xbase, long long xrs, int n, int steps,
     double (&sR)[6][NT * RPT],
-    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1]);
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1],
+    const double* __restrict__ add = nullptr);
 
 template <int KIND, int NT, int RPT>
 __device__ __forceinline__ void lds_solve_one(
@@ -191,19 +192,22 @@ __device__ __forceinline__ void lds_solve_one(
     const double* __restrict__ gd, const double* __restrict__ ge, const double* gy,
     double* gx, long long base, long long rs, long long xbase, long long xrs, int n,
     int steps, double (&sR)[6][NT * RPT],
-    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1])
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1],
+    const double* __restrict__ add = nullptr)
 {
     Row<double> r[RPT];
     lds_load_rows<NT, RPT>(ga, gb, gc, gd, ge, gy, base, rs, n, r);
-    lds_solve_regs<KIND, NT, RPT>(r, gx, xbase, xrs, n, steps, sR, sE);
+    lds_solve_regs<KIND, NT, RPT>(r, gx, xbase, xrs, n, steps, sR, sE, add);
 }
 
-// The solve proper, from rows already in registers.
+// The solve proper, from rows already in registers. add (optional): x +
+// add[] is stored, at x's own element.
 template <int KIND, int NT, int RPT>
 __device__ __forceinline__ void lds_solve_regs(
     Row<double> (&r)[RPT], double* gx, long long xbase, long long xrs, int n, int steps,
     double (&sR)[6][NT * RPT],
-    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1])
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1],
+    const double* __restrict__ add)
 {
     const int t = threadIdx.x;
 
@@ -333,10 +337,14 @@ __device__ __forceinline__ void lds_solve_regs(
                     cii = sR[2][i + delta]; bii = sR[1][i + delta]; yii = sR[5][i + delta];
                 }
                 const double deno = __builtin_fma(r[k].c, cii, -(bii * r[k].d));
-                gx[g] = __builtin_fma(cii, r[k].y, -(r[k].d * yii)) / deno;
-                gx[g + (long long)delta * xrs] = __builtin_fma(r[k].c, yii, -(bii * r[k].y)) / deno;
+                const double x0 = __builtin_fma(cii, r[k].y, -(r[k].d * yii)) / deno;
+                const double x1 = __builtin_fma(r[k].c, yii, -(bii * r[k].y)) / deno;
+                const long long g1 = g + (long long)delta * xrs;
+                gx[g] = add ? x0 + add[g] : x0;
+                gx[g1] = add ? x1 + add[g1] : x1;
             } else if (i < delta) {
-                gx[g] = r[k].y / r[k].c;
+                const double x0 = r[k].y / r[k].c;
+                gx[g] = add ? x0 + add[g] : x0;
             }
         }
     }
@@ -354,8 +362,9 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
 
     const int sys = xcd_remap(blockIdx.x, m);
     const long long base = (long long)sys * ss;
-    // where x goes: in place, or (split path) back to the caller's layout
-    double* gx = gy;
+    // where x goes: in place (or to split.x at y's element), or (split path)
+    // back to the caller's layout
+    double* gx = split.x ? split.x : gy;
     long long xbase = base, xrs = rs;
     if (split.S > 0) {
         // sub-system sys = R*m0 + j: rows R + 2^S*l of original system j
@@ -367,7 +376,8 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
             xrs = 1LL << split.S;
         }
     }
-    lds_solve_one<KIND, NT, RPT>(ga, gb, gc, gd, ge, gy, gx, base, rs, xbase, xrs, n, steps, sR, sE);
+    lds_solve_one<KIND, NT, RPT>(ga, gb, gc, gd, ge, gy, gx, base, rs, xbase, xrs, n, steps, sR, sE,
+                                 split.S > 0 ? nullptr : split.add);
 }
 
 // Persistent batch form: a resident grid walks the systems, and while it

@@ -92,6 +92,17 @@ def _declare(lib):
     lib.pcrp_split_autotune.argtypes = [ctypes.c_int] * 4 + [_vp] * 8 + \
         [ctypes.c_size_t, ctypes.c_int, _vp, ip, ip, fp]
     lib.pcrp_split_autotune.restype = ctypes.c_int
+    lib.pcrp_factor_size.argtypes = [ctypes.c_int] * 4
+    lib.pcrp_factor_size.restype = ctypes.c_size_t
+    lib.pcrp_factor_f64.argtypes = [_vp] * 5 + [ctypes.c_int] * 4 + [_vp, ctypes.c_size_t, _vp]
+    lib.pcrp_factor_f64.restype = ctypes.c_int
+    lib.pcrp_solve_factored_f64.argtypes = [_vp, _vp] + [ctypes.c_int] * 4 + [_vp]
+    lib.pcrp_solve_factored_f64.restype = ctypes.c_int
+    lib.pcrp_factor_f32r.argtypes = lib.pcrp_factor_f64.argtypes
+    lib.pcrp_factor_f32r.restype = ctypes.c_int
+    lib.pcrp_solve_factored_f32r_f64.argtypes = ([_vp] * 7 + [ctypes.c_int] * 5 +
+                                                 [ctypes.c_double, _vp, _vp])
+    lib.pcrp_solve_factored_f32r_f64.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -191,6 +202,55 @@ def solve_refined(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, m
     if own:
         _keep_until_done(ws, stream)
     _check(st, "pcrp_solve_f32r_f64")
+    return y
+
+
+# ---- factor once, solve many -------------------------------------------------
+
+def factor_size(n: int, m: int, precision: int = PREC_F64, shared: bool = False) -> int:
+    return int(load().pcrp_factor_size(n, m, precision, int(bool(shared))))
+
+
+def factor(a, b, c, d, e, n: int, m: int, layout: int = INTERLEAVED, precision: int = PREC_F64,
+           shared: bool = False, stream=None, out=None):
+    """Factor m systems (or, with shared=True, the one system of n rows held by
+    a..e) once; returns the factor as a uint8 device tensor for solve_factored."""
+    import torch
+    msys = 1 if shared else m
+    _validate([a, b, c, d, e], n, msys)
+    nb = factor_size(n, m, precision, shared)
+    if nb == 0:
+        raise PCRPError(7, f"factor: n = {n} not supported for precision {precision}")
+    F = out if out is not None else torch.empty(nb, dtype=torch.uint8, device=a.device)
+    fn = load().pcrp_factor_f64 if precision == PREC_F64 else load().pcrp_factor_f32r
+    st = fn(*[_ptr(t) for t in (a, b, c, d, e)], n, m, layout, int(bool(shared)), _ptr(F),
+            F.numel(), _stream(stream))
+    _check(st, "pcrp_factor")
+    return F
+
+
+def solve_factored(F, y, n: int, m: int, layout: int = INTERLEAVED, shared: bool = False,
+                   stream=None):
+    """FP64: y <- A^{-1} y from a factor of factor(..., precision=PREC_F64)."""
+    _validate([y], n, m)
+    st = load().pcrp_solve_factored_f64(_ptr(F), _ptr(y), n, m, layout, int(bool(shared)),
+                                        _stream(stream))
+    _check(st, "pcrp_solve_factored_f64")
+    return y
+
+
+def solve_factored_refined(F, a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED,
+                           shared: bool = False, max_sweeps: int = 8, tol: float = 1e-12,
+                           sweeps=None, stream=None):
+    """FP32 + FP64 refinement from a factor of factor(..., precision=PREC_F32_REFINE);
+    a..e as given to factor() (the FP64 residuals need the matrix)."""
+    _validate([y], n, m)
+    _validate([a, b, c, d, e], n, 1 if shared else m)
+    sp = _ptr(sweeps) if sweeps is not None else None
+    st = load().pcrp_solve_factored_f32r_f64(_ptr(F), *[_ptr(t) for t in (a, b, c, d, e, y)], n, m,
+                                             layout, int(bool(shared)), max_sweeps, tol, sp,
+                                             _stream(stream))
+    _check(st, "pcrp_solve_factored_f32r_f64")
     return y
 
 
@@ -407,7 +467,7 @@ class HestonParams(ctypes.Structure):
                 ("sigma", ctypes.c_double), ("rho", ctypes.c_double),
                 ("r", ctypes.c_double), ("strike", ctypes.c_double),
                 ("dt", ctypes.c_double), ("steps", ctypes.c_int),
-                ("theta_scheme", ctypes.c_double)]
+                ("theta_scheme", ctypes.c_double), ("reuse_factor", ctypes.c_int)]
 
 
 # refinement settings of the FP32 line solves in the Heston driver
@@ -417,7 +477,8 @@ HESTON_MAX_SWEEPS = 8
 
 # Heston ADI problem: Ns = 1024, Nv = 512 unknowns, 200 steps of 1/200
 HESTON_GRID = dict(ns=1024, nv=512, smax=800.0, vmax=5.0, kappa=3.0, theta_v=0.04, sigma=0.15,
-               rho=-0.7, r=0.03, strike=100.0, dt=1.0 / 200, steps=200, theta_scheme=0.5)
+               rho=-0.7, r=0.03, strike=100.0, dt=1.0 / 200, steps=200, theta_scheme=0.5,
+               reuse_factor=0)
 
 
 def heston_params(**kw) -> HestonParams:

@@ -155,3 +155,34 @@ def test_uneven_refined_argument_checks(lib):
     # (n = 5000: 2500 block rows, 2 HBM levels), 56 B per block row each
     assert lib.pcrp_uneven_workspace_size(off.ctypes.data, big.ctypes.data, 1, 1) >= \
         8 * 5000 + 2 * 56 * 2500
+
+
+def test_factor_argument_checks(lib):
+    """Factor-once entry points: sizes, null pointers, unsupported n, factor
+    buffer size; all rejected before any launch."""
+    p = _fake(5)
+    F = _fake(1)[0]
+    # (4 floor(log2 n) + 3) n doubles per system
+    assert lib.pcrp_factor_size(512, 10, 0, 0) == 10 * (4 * 9 + 3) * 512 * 8
+    assert lib.pcrp_factor_size(512, 10, 0, 1) == (4 * 9 + 3) * 512 * 8
+    assert lib.pcrp_factor_size(1025, 10, 0, 0) == 0          # FP64 factor: n <= 1024
+    assert lib.pcrp_factor_size(2048, 3, 1, 0) > 0 and lib.pcrp_factor_size(2049, 3, 1, 0) == 0
+    assert lib.pcrp_factor_size(100, 4, 0, 2) == 0
+    need = lib.pcrp_factor_size(100, 4, 0, 0)
+    assert lib.pcrp_factor_f64(*p, 0, 4, 1, 0, F, need, None) == 1
+    assert lib.pcrp_factor_f64(*p, 100, 4, 1, 0, None, need, None) == 2
+    assert lib.pcrp_factor_f64(*p, 100, 4, 1, 0, F, need - 1, None) == 4
+    assert lib.pcrp_factor_f64(*p, 100, 4, 7, 0, F, need, None) == 3
+    assert lib.pcrp_factor_f64(*p, 100, 4, 1, 3, F, need, None) == 6
+    assert lib.pcrp_factor_f64(*p, 1025, 4, 1, 0, F, 1 << 30, None) == 7
+    assert lib.pcrp_solve_factored_f64(F, None, 100, 4, 1, 0, None) == 2
+    assert lib.pcrp_solve_factored_f64(F, _fake(1)[0], 100, 0, 1, 0, None) == 1
+    assert lib.pcrp_solve_factored_f64(F, _fake(1)[0], 100, 4, 9, 0, None) == 3
+    need32 = lib.pcrp_factor_size(100, 4, 1, 0)
+    assert lib.pcrp_factor_f32r(*p, 100, 4, 1, 0, F, need32 - 1, None) == 4
+    assert lib.pcrp_factor_f32r(*p, 4000, 4, 1, 0, F, 1 << 30, None) == 7
+    y = _fake(1)[0]
+    f = lib.pcrp_solve_factored_f32r_f64
+    assert f(F, *p, y, 100, 4, 1, 0, -1, 1e-12, None, None) == 6
+    assert f(F, *p, None, 100, 4, 1, 0, 4, 1e-12, None, None) == 2
+    assert f(F, *p, y, 100, 4, 1, 0, 4, float("nan"), None, None) == 6

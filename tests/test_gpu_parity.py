@@ -809,9 +809,22 @@ def test_heston_graph_replay_equals_eager(gpu):
     _say("heston-graph")
     kw = dict(ns=128, nv=64, steps=30)
     for prec in (gpu.PREC_F64, gpu.PREC_F32_REFINE):
-        Ue, _ = _heston_gpu(gpu, prec, False, **kw)
-        Ug, _ = _heston_gpu(gpu, prec, True, **kw)
-        assert np.array_equal(Ue, Ug), prec
+        for reuse in (0, 1):
+            Ue, _ = _heston_gpu(gpu, prec, False, reuse_factor=reuse, **kw)
+            Ug, _ = _heston_gpu(gpu, prec, True, reuse_factor=reuse, **kw)
+            assert np.array_equal(Ue, Ug), (prec, reuse)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_heston_factor_reuse_equals_full_solves(gpu, prec):
+    """The Heston grid, all 200 steps: factoring the line matrices once (FP64:
+    every S-line and the shared v-line matrix; FP32+refine: the shared v-line
+    matrix) gives the same price grid bit for bit as solving every line in
+    full each step (the factored solves run the same operations)."""
+    _say("heston-reuse", prec)
+    U0, _ = _heston_gpu(gpu, prec, True, reuse_factor=0)
+    U1, _ = _heston_gpu(gpu, prec, True, reuse_factor=1)
+    assert np.array_equal(U0, U1), np.abs(U0 - U1).max()
 
 
 def test_heston_price_and_refine(gpu):
