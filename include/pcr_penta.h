@@ -40,7 +40,8 @@ enum {
     PCRP_ERR_WORKSPACE = 4,      /* ws_bytes < pcrp_workspace_size(...) */
     PCRP_ERR_HIP = 5,            /* a HIP runtime call or kernel launch failed */
     PCRP_ERR_ARG = 6,            /* other invalid argument (max_sweeps, tol, kind ...) */
-    PCRP_ERR_UNSUPPORTED = 7     /* valid request this build cannot serve */
+    PCRP_ERR_UNSUPPORTED = 7,    /* valid request this build cannot serve */
+    PCRP_ERR_SCRATCH = 8         /* needs more than PCRP_NOALLOC_SCRATCH_BYTES */
 };
 
 /* Batch layouts. Element i of system j lives at
@@ -74,6 +75,24 @@ int pcrp_solve_f64(const double *a, const double *b, const double *c,
                    const double *d, const double *e, double *y,
                    int n, int m, int layout,
                    void *ws, size_t ws_bytes, void *stream);
+
+/*
+ * Plain FP64 solve with no workspace argument, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and paths as pcrp_solve_f64; the split path's
+ * workspace (n > 1024) is a fixed device array of PCRP_NOALLOC_SCRATCH_BYTES
+ * held on every GPU the library runs on until the process ends. Requests
+ * whose pcrp_workspace_size exceeds it return PCRP_ERR_SCRATCH (about 96
+ * bytes per unknown: up to ~700,000 unknowns when n > 1024). All calls on one
+ * GPU share that GPU's array: calls that overlap in time on one GPU race and
+ * must be serialised by the caller. The first call on a device looks the
+ * array's address up (hipGetSymbolAddress, not a stream operation).
+ */
+#define PCRP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pcrp_solve_f64_noalloc(const double *a, const double *b, const double *c,
+                           const double *d, const double *e, double *y,
+                           int n, int m, int layout, void *stream);
 
 /*
  * FP32 inner solve + FP64 iterative refinement (no counterpart in the

@@ -147,6 +147,7 @@ const char* pcrp_strerror(int status)
         case PCRP_ERR_HIP: return "HIP runtime error";
         case PCRP_ERR_ARG: return "invalid argument";
         case PCRP_ERR_UNSUPPORTED: return "unsupported by this build";
+        case PCRP_ERR_SCRATCH: return "request needs more than the fixed no-allocation scratch";
         default: return "unknown status";
     }
 }

@@ -24,6 +24,7 @@ PREC_F32_REFINE = 1
 STATUS = {
     0: "PCRP_OK", 1: "PCRP_ERR_INVALID_SIZE", 2: "PCRP_ERR_NULL", 3: "PCRP_ERR_LAYOUT",
     4: "PCRP_ERR_WORKSPACE", 5: "PCRP_ERR_HIP", 6: "PCRP_ERR_ARG", 7: "PCRP_ERR_UNSUPPORTED",
+    8: "PCRP_ERR_SCRATCH",
 }
 
 
@@ -103,6 +104,8 @@ def _declare(lib):
     lib.pcrp_solve_factored_f32r_f64.argtypes = ([_vp] * 7 + [ctypes.c_int] * 5 +
                                                  [ctypes.c_double, _vp, _vp])
     lib.pcrp_solve_factored_f32r_f64.restype = ctypes.c_int
+    lib.pcrp_solve_f64_noalloc.argtypes = [_vp] * 6 + [ctypes.c_int] * 3 + [_vp]
+    lib.pcrp_solve_f64_noalloc.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
