@@ -318,15 +318,36 @@ int pcrp_autotune(int precision, int n, int m, int layout,
 int pcrp_split_tune_set(int precision, int lds_max, int k_fuse, int shape);
 int pcrp_split_tune_get(int precision, int *lds_max, int *k_fuse, int *shape,
                         int *n_shapes);
-/* Time every (k_fuse, shape) pair of the split path on the caller's batch
- * (n must be above the LDS threshold; ws as for pcrp_solve_f64), keep the
- * fastest. Synchronises `stream`. */
+/* Time every (lds_max, k_fuse, shape) of the split path on the caller's
+ * batch (n must be above the current LDS threshold; ws as for pcrp_solve_f64;
+ * lds_max from 256 to 1024, those whose workspace fits ws_bytes), keep the
+ * fastest (pcrp_split_tune_get reads the chosen lds_max). Synchronises
+ * `stream`. */
 int pcrp_split_autotune(int precision, int n, int m, int layout,
                         const double *a, const double *b, const double *c,
                         const double *d, const double *e, const double *y,
                         double *y_work, void *ws, size_t ws_bytes, int reps, void *stream,
                         int *best_k, int *best_shape, float *best_ms);
-/* Persist / restore the tables (LDS variants and split tunables) as text. */
+/* FP32 refinement tunables (no counterpart in the reference): win_levels
+ * (2..7, default 5) = reduction levels a window of a long system runs, with
+ * 2^win_levels - 1 halo block rows per side (more levels: fewer batches fall
+ * back to the split path, more redundant halo work); window_min_unknowns
+ * (default 2^23) = batch size in unknowns from which systems of n in
+ * (1024, 2048] take the window path instead of the one-workgroup kernel.
+ * Results meet the same 1e-12 bar at every setting. */
+int pcrp_f32r_tune_set(int win_levels, long long window_min_unknowns);
+int pcrp_f32r_tune_get(int *win_levels, long long *window_min_unknowns);
+/* Time pcrp_solve_f32r_f64 on the caller's batch (n > 1024; ws as for that
+ * call) for win_levels 3..6 and, for n <= 2048, against the one-workgroup
+ * kernel; keep the fastest (window_min_unknowns moves to include or exclude
+ * this batch size). Synchronises `stream`; y is copied into y_work before
+ * each timed solve. window_path: 1 if the window path won. */
+int pcrp_f32r_autotune(int n, int m, int layout, const double *a, const double *b,
+                       const double *c, const double *d, const double *e, const double *y,
+                       double *y_work, void *ws, size_t ws_bytes, int reps, void *stream,
+                       int *best_levels, int *window_path, float *best_ms);
+/* Persist / restore every table (LDS variants, split tunables including the
+ * LDS/HBM threshold, FP32 refinement tunables) as text. */
 int pcrp_tune_save(const char *path);
 int pcrp_tune_load(const char *path);
 

@@ -157,7 +157,9 @@ size_t pcrp_workspace_size(int n, int m, int precision)
     if (check_shape(n, m) != PCRP_OK) return 0;
     if (precision == PCRP_PREC_F64) return split_f64_workspace(n, m);
     if (precision == PCRP_PREC_F32_REFINE && n > lds_f32r_max_n()) return split_f32r_workspace(n, m);
-    if (precision == PCRP_PREC_F32_REFINE && window_preferred(n, m))
+    // n in (1024, 2048]: enough for the window path whatever the tunable
+    // window threshold says (the one-workgroup kernel needs none)
+    if (precision == PCRP_PREC_F32_REFINE && n > 1024)
         return std::max(window_f32r_workspace_uniform(n, m, PCRP_LAYOUT_INTERLEAVED),
                         window_f32r_workspace_uniform(n, m, PCRP_LAYOUT_CONTIGUOUS));
     return 0;
@@ -333,9 +335,14 @@ int pcrp_split_autotune(int precision, int n, int m, int layout, const double* a
     const size_t bytes = sizeof(double) * (size_t)n * (size_t)m;
     SplitTune best = keep;
     float bt = -1.f;
+    // the LDS/HBM threshold too: every power of two from 256 up to the LDS
+    // kernel's capacity whose workspace fits the caller's, below n
+    for (int lm = 256; lm <= lds_f64_max_n() && st == PCRP_OK; lm <<= 1)
     for (int k = 1; k <= 6 && st == PCRP_OK; ++k) {
         for (int sh = 0; sh < split_f64_shape_count() && st == PCRP_OK; ++sh) {
-            if (split_f64_set(SplitTune{keep.lds_max, k, sh}) != 0) continue;   // halo too wide
+            if (split_f64_levels(n, lm) == 0) continue;
+            if (split_f64_set(SplitTune{lm, k, sh}) != 0) continue;   // halo too wide
+            if (split_f64_workspace(n, m) > ws_bytes) continue;
             float tmin = -1.f;
             for (int r = 0; r < reps; ++r) {
                 float t = 0.f;
@@ -349,7 +356,7 @@ int pcrp_split_autotune(int precision, int n, int m, int layout, const double* a
                 }
                 if (tmin < 0.f || t < tmin) tmin = t;
             }
-            if (st == PCRP_OK && (bt < 0.f || tmin < bt)) { bt = tmin; best = SplitTune{keep.lds_max, k, sh}; }
+            if (st == PCRP_OK && (bt < 0.f || tmin < bt)) { bt = tmin; best = SplitTune{lm, k, sh}; }
         }
     }
     (void)hipEventDestroy(e0);
@@ -362,6 +369,90 @@ int pcrp_split_autotune(int precision, int n, int m, int layout, const double* a
     return PCRP_OK;
 }
 
+int pcrp_f32r_tune_set(int win_levels, long long window_min_unknowns)
+{
+    return f32r_tune_set(F32RTune{win_levels, window_min_unknowns}) == 0 ? PCRP_OK : PCRP_ERR_ARG;
+}
+
+int pcrp_f32r_tune_get(int* win_levels, long long* window_min_unknowns)
+{
+    const F32RTune t = f32r_tune_get();
+    if (win_levels) *win_levels = t.win_levels;
+    if (window_min_unknowns) *window_min_unknowns = t.window_min_unknowns;
+    return PCRP_OK;
+}
+
+int pcrp_f32r_autotune(int n, int m, int layout, const double* a, const double* b, const double* c,
+                       const double* d, const double* e, const double* y, double* y_work,
+                       void* ws, size_t ws_bytes, int reps, void* stream, int* best_levels,
+                       int* window_path, float* best_ms)
+{
+    int st = check_shape(n, m);
+    if (st != PCRP_OK) return st;
+    if (!a || !b || !c || !d || !e || !y || !y_work) return PCRP_ERR_NULL;
+    if (reps < 1) return PCRP_ERR_ARG;
+    if (n <= 1024) return PCRP_ERR_ARG;   // no window path below
+    long long rs, ss;
+    if (!layout_strides(n, m, layout, &rs, &ss)) return PCRP_ERR_LAYOUT;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return PCRP_ERR_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return PCRP_ERR_HIP; }
+    const size_t bytes = sizeof(double) * (size_t)n * (size_t)m;
+    const F32RTune keep = f32r_tune_get();
+    const auto timed = [&](float& tmin) {
+        tmin = -1.f;
+        for (int r = 0; r < reps; ++r) {
+            float t = 0.f;
+            if (hipMemcpyAsync(y_work, y, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipEventRecord(e0, s) != hipSuccess ||
+                pcrp_solve_f32r_f64(a, b, c, d, e, y_work, n, m, layout, 8, 1e-12, nullptr, ws,
+                                    ws_bytes, s) != PCRP_OK ||
+                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                return PCRP_ERR_HIP;
+            if (tmin < 0.f || t < tmin) tmin = t;
+        }
+        return PCRP_OK;
+    };
+    // window levels (the window path taken whenever it applies) ...
+    F32RTune best = keep;
+    float bt = -1.f;
+    const bool needs_ws = window_f32r_workspace_uniform(n, m, layout) <= ws_bytes && ws;
+    for (int wl = 3; wl <= 6 && st == PCRP_OK && needs_ws; ++wl) {
+        f32r_tune_set(F32RTune{wl, 0});
+        float t;
+        st = timed(t);
+        if (st == PCRP_OK && (bt < 0.f || t < bt)) { bt = t; best = F32RTune{wl, keep.window_min_unknowns}; }
+    }
+    // ... then, for n <= 2048, against the one-workgroup kernel: the window
+    // threshold moves to just above or at this batch's size
+    int use_window = needs_ws ? 1 : 0;
+    if (st == PCRP_OK && n <= lds_f32r_max_n()) {
+        f32r_tune_set(F32RTune{best.win_levels, 1LL << 62});
+        float t1;
+        st = timed(t1);
+        const long long nm = (long long)n * m;
+        if (st == PCRP_OK) {
+            if (bt < 0.f || t1 < bt) {
+                use_window = 0;
+                bt = t1;
+                best.window_min_unknowns = std::max(best.window_min_unknowns, nm + 1);
+            } else {
+                best.window_min_unknowns = std::min(best.window_min_unknowns, nm);
+            }
+        }
+    }
+    f32r_tune_set(st == PCRP_OK ? best : keep);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != PCRP_OK) return st;
+    if (best_levels) *best_levels = best.win_levels;
+    if (window_path) *window_path = use_window;
+    if (best_ms) *best_ms = bt;
+    return PCRP_OK;
+}
+
 int pcrp_tune_save(const char* path)
 {
     if (!path) return PCRP_ERR_NULL;
@@ -370,6 +461,8 @@ int pcrp_tune_save(const char* path)
     fprintf(f, "# pcrp tuning table: precision bucket(ceil log2 n) variant kind nt rpt\n");
     const SplitTune sp = split_f64_get();
     fprintf(f, "split %d %d %d\n", sp.lds_max, sp.k_fuse, sp.shape);
+    const F32RTune fr = f32r_tune_get();
+    fprintf(f, "f32r %d %lld\n", fr.win_levels, fr.window_min_unknowns);
     for (int p = 0; p < 2; ++p)
         for (int b = 0; b < kBuckets; ++b) {
             const LdsVariantF64* v = g_tuned[p][b] >= 0 ? lds_f64_variant(g_tuned[p][b]) : nullptr;
@@ -391,6 +484,12 @@ int pcrp_tune_load(const char* path)
         int lm, kf, sh;
         if (sscanf(line, "split %d %d %d", &lm, &kf, &sh) == 3) {
             if (split_f64_set(SplitTune{lm, kf, sh}) != 0) st = PCRP_ERR_ARG;
+            continue;
+        }
+        int wl;
+        long long wmin;
+        if (sscanf(line, "f32r %d %lld", &wl, &wmin) == 2) {
+            if (f32r_tune_set(F32RTune{wl, wmin}) != 0) st = PCRP_ERR_ARG;
             continue;
         }
         int p, b, id, kind, nt, rpt;

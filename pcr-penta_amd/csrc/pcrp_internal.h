@@ -127,10 +127,21 @@ size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m);
 // untouched; the caller then runs the pass/stage path.
 size_t window_f32r_workspace(long long elems, int m, bool transposed);
 size_t window_f32r_workspace_uniform(int n, int m, int layout);
-// n <= 2048 batches the window path beats the one-workgroup kernel on
-// (n > 1024 and at least 2^23 unknowns; below, the read-back and the
-// transposes cost more than the windows save)
-inline bool window_preferred(int n, int m) { return n > 1024 && (long long)n * m >= (1LL << 23); }
+// FP32 refinement tunables: levels a window of a longer system runs (halo
+// 2^L - 1 block rows per side; 2..7, default 5) and the batch size (unknowns)
+// from which n in (1024, 2048] takes the window path instead of the
+// one-workgroup kernel (default 2^23; below, the read-back and the
+// transposes cost more than the windows save).
+struct F32RTune {
+    int win_levels;
+    long long window_min_unknowns;
+};
+F32RTune f32r_tune_get();
+int f32r_tune_set(const F32RTune& t);
+inline bool window_preferred(int n, int m)
+{
+    return n > 1024 && (long long)n * m >= f32r_tune_get().window_min_unknowns;
+}
 int window_f32r_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
                         bool& fell_back);
 int window_f32r_uneven(const double* a, const double* b, const double* c, const double* d,

@@ -13,13 +13,14 @@
 // Polizzi & Sameh, Parallel Computing 32 (2006) 177-194):
 //   * a system of nb <= kWinNB block rows is one window, every level it needs
 //     (exact, like the one-workgroup kernel);
-//   * a longer system is cut into windows of at most kWinT interior block
-//     rows (balanced: ceil(nb / windows)) with kWinH = 2^kWinL - 1 halo block
-//     rows on each side; a window runs at most
-//     kWinL levels with neighbours outside the window treated as absent, so
-//     the values at its interior rows equal those of the whole-system PCR
-//     (the halo rows absorb the edge). A window whose interior multipliers are
-//     still above kTruncF32 after kWinL levels sets `nonlocal`, and the batch
+//   * a longer system is cut into windows of at most T = kWinNB - 2H interior
+//     block rows (balanced: ceil(nb / windows)) with H = 2^L - 1 halo block
+//     rows on each side (L: the tunable window level count, 5 by default);
+//     a window runs at most L levels with neighbours outside the window
+//     treated as absent, so the values at its interior rows equal those of
+//     the whole-system PCR (the halo rows absorb the edge). A window whose
+//     interior multipliers are still above kTruncF32 after L levels sets
+//     `nonlocal`, and the batch
 //     is solved by the pass/stage path instead (host read-back after the
 //     first sweep, which writes only workspace buffers: y is still the
 //     right-hand side).
@@ -54,9 +55,13 @@ namespace {
 
 constexpr int kWinNT = 256, kWinBR = 2;
 constexpr int kWinNB = kWinNT * kWinBR;    // block rows per window (LDS records)
-constexpr int kWinL = 5;                   // levels a window of a longer system may run
-constexpr int kWinH = (1 << kWinL) - 1;    // halo block rows per side: reach of kWinL levels
-constexpr int kWinT = kWinNB - 2 * kWinH;  // interior block rows
+// Levels L a window of a longer system may run (tunable, f32r_tune: default
+// 5): halo H = 2^L - 1 block rows per side (the reach of L levels), interior
+// T = kWinNB - 2H block rows. kWinLMax bounds L (workspace sizes use it).
+constexpr int kWinLMax = 7;
+constexpr int kWinTMin = kWinNB - 2 * ((1 << kWinLMax) - 1);
+inline int win_halo(int L) { return (1 << L) - 1; }
+inline int win_interior(int L) { return kWinNB - 2 * win_halo(L); }
 
 // Block row K holds rows 2K - shift and 2K - shift + 1 (rows outside 0..n-1
 // are identity rows): a contiguous system at an odd element offset takes
@@ -67,7 +72,7 @@ struct WSys {
     long long off;   // element of row 0 (row i at off + i * rs)
     long long w0;    // first window of the system (uneven batches)
     int n, nb, nw, shift;
-    int T, pad;      // interior block rows per window (balanced: ceil(nb / nw) <= kWinT)
+    int T, pad;      // interior block rows per window (balanced: ceil(nb / nw) <= interior)
 };
 
 struct WArgs {
@@ -85,6 +90,7 @@ struct WArgs {
     int m, nwin, uniform;
     int nwu;   // uniform batches: windows per system
     int vec;   // rs == 1 and every array 16-byte aligned
+    int wl, wh;   // levels a window of a longer system runs, its halo block rows per side
 };
 
 // Interleaved uniform batches (more than one system) are transposed first:
@@ -94,9 +100,9 @@ inline bool window_transposed(int m, long long rs)
     return m > 1 && rs == m;
 }
 
-__host__ __device__ inline int windows_for(int nb)
+__host__ __device__ inline int windows_for(int nb, int T)
 {
-    return nb <= kWinNB ? 1 : (nb + kWinT - 1) / kWinT;
+    return nb <= kWinNB ? 1 : (nb + T - 1) / T;
 }
 
 // Window w -> (system j, window index wi). Uniform batches of systems with
@@ -190,15 +196,15 @@ __global__ __launch_bounds__(NT) void wsweep_kernel(WArgs A)
         if (A.stop[j]) continue;   // uniform over the workgroup
         const WSys S = A.sys[j];
         const bool full = S.nw == 1;
-        const int kb = full ? 0 : wi * S.T - kWinH;     // block row of window slot 0
+        const int kb = full ? 0 : wi * S.T - A.wh;      // block row of window slot 0
         const int k0 = full ? 0 : wi * S.T;             // interior block rows [k0, k1)
         const int k1 = full ? S.nb : min(S.nb, k0 + S.T);
         // block rows the window holds: the interior and its halos, inside the
         // system (the slots past them are idle)
-        const int klo = full ? 0 : max(0, k0 - kWinH);
-        const int khi = full ? S.nb : min(S.nb, k1 + kWinH);
+        const int klo = full ? 0 : max(0, k0 - A.wh);
+        const int khi = full ? S.nb : min(S.nb, k1 + A.wh);
         int lmax = 0;
-        if (full) { while ((1 << lmax) < S.nb) ++lmax; } else { lmax = kWinL; }
+        if (full) { while ((1 << lmax) < S.nb) ++lmax; } else { lmax = A.wl; }
 
         BRow r[BR];
         double xv[BR][2];   // x of the previous sweep at the rows (first sweep: y)
@@ -418,11 +424,11 @@ __global__ __launch_bounds__(256) void wtranspose_out_kernel(const double* __res
 constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
-// Bound on the windows of `elems` rows in m systems: windows_for(nb) <=
-// nb / kWinT + 1 and nb <= n / 2 + 1 per system.
+// Bound on the windows of `elems` rows in m systems at any window level
+// count: windows_for(nb, T) <= nb / kWinTMin + 1 and nb <= n / 2 + 1 per system.
 size_t lev_capacity(long long elems, int m)
 {
-    return (size_t)elems / (2 * kWinT) + 2 * (size_t)m + 64;
+    return (size_t)elems / (2 * kWinTMin) + 2 * (size_t)m + 64;
 }
 
 // Direct: x buffer and copy of y (16 B per element). Transposed (interleaved
@@ -469,11 +475,12 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         rs = 1;
         for (int j = 0; j < m; ++j) sys[j].off = (long long)j * n;
     }
+    const int wl = f32r_tune_get().win_levels, wT = win_interior(wl);
     long long nwin = 0;
     for (auto& s : sys) {
         s.shift = rs == 1 ? (int)(s.off & 1) : 0;
         s.nb = (s.n + s.shift + 1) / 2;
-        s.nw = windows_for(s.nb);
+        s.nw = windows_for(s.nb, wT);
         s.T = (s.nb + s.nw - 1) / s.nw;
         s.w0 = nwin;
         nwin += s.nw;
@@ -512,7 +519,7 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     WArgs A{ca, transposed ? six + N : b, transposed ? six + 2 * N : c,
             transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
             nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0,
-            uniform ? sys[0].nw : 0, 0};
+            uniform ? sys[0].nw : 0, 0, wl, win_halo(wl)};
     {
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
@@ -560,6 +567,17 @@ int window_run(const double* a, const double* b, const double* c, const double* 
 }
 
 }  // namespace
+
+static F32RTune g_f32r = {5, 1LL << 23};
+
+F32RTune f32r_tune_get() { return g_f32r; }
+
+int f32r_tune_set(const F32RTune& t)
+{
+    if (t.win_levels < 2 || t.win_levels > kWinLMax || t.window_min_unknowns < 0) return -1;
+    g_f32r = t;
+    return 0;
+}
 
 size_t window_f32r_workspace(long long elems, int m, bool transposed)
 {

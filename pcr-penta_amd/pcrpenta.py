@@ -106,6 +106,13 @@ def _declare(lib):
     lib.pcrp_solve_factored_f32r_f64.restype = ctypes.c_int
     lib.pcrp_solve_f64_noalloc.argtypes = [_vp] * 6 + [ctypes.c_int] * 3 + [_vp]
     lib.pcrp_solve_f64_noalloc.restype = ctypes.c_int
+    lib.pcrp_f32r_tune_set.argtypes = [ctypes.c_int, ctypes.c_longlong]
+    lib.pcrp_f32r_tune_set.restype = ctypes.c_int
+    lib.pcrp_f32r_tune_get.argtypes = [ip, ctypes.POINTER(ctypes.c_longlong)]
+    lib.pcrp_f32r_tune_get.restype = ctypes.c_int
+    lib.pcrp_f32r_autotune.argtypes = [ctypes.c_int] * 3 + [_vp] * 8 + \
+        [ctypes.c_size_t, ctypes.c_int, _vp, ip, ip, fp]
+    lib.pcrp_f32r_autotune.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -336,6 +343,36 @@ def split_autotune(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, 
                                     ctypes.byref(sh), ctypes.byref(ms))
     _check(st, "pcrp_split_autotune")
     return k.value, sh.value, ms.value
+
+
+def f32r_tune_get():
+    """(win_levels, window_min_unknowns) of the FP32 refinement paths."""
+    wl, wm = ctypes.c_int(), ctypes.c_longlong()
+    _check(load().pcrp_f32r_tune_get(ctypes.byref(wl), ctypes.byref(wm)), "pcrp_f32r_tune_get")
+    return wl.value, wm.value
+
+
+def f32r_tune_set(win_levels: int, window_min_unknowns: int):
+    _check(load().pcrp_f32r_tune_set(win_levels, window_min_unknowns), "pcrp_f32r_tune_set")
+
+
+def f32r_autotune(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, reps: int = 2,
+                  stream=None, ws=None):
+    """Time the window levels (and, for n <= 2048, the one-workgroup kernel) on this
+    batch; keeps the fastest. Returns (win_levels, window_path, ms)."""
+    import torch
+    _validate([a, b, c, d, e, y], n, m)
+    if ws is None:
+        ws, nb = _workspace(n, m, PREC_F32_REFINE, y.device)
+    else:
+        nb = ws.numel()
+    work = torch.empty_like(y)
+    wl, wp, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+    st = load().pcrp_f32r_autotune(n, m, layout, *[_ptr(t) for t in (a, b, c, d, e, y)],
+                                   _ptr(work), _ptr(ws), nb, reps, _stream(stream),
+                                   ctypes.byref(wl), ctypes.byref(wp), ctypes.byref(ms))
+    _check(st, "pcrp_f32r_autotune")
+    return wl.value, wp.value, ms.value
 
 
 def tune_save(path: str):

@@ -857,9 +857,68 @@ def test_split_autotune_keeps_results_exact(gpu, oracle_lib):
         k, sh, ms = gpu.split_autotune(*dev, n, m, I, reps=1)
         assert 1 <= k <= 6 and ms > 0
         assert gpu.split_tune_get()[1:3] == (k, sh)
+        assert gpu.split_tune_get()[0] in (256, 512, 1024)   # the LDS/HBM threshold is tuned too
         assert np.array_equal(_np(dev[5]), ins[5])
         x = _run_f64(gpu, dev, n, m, I)
         assert np.array_equal(x, oracle_lib.pcr_batch(*ins, n, m, I))
     finally:
         gpu.split_tune_set(*saved[:3])
         torch.cuda.synchronize()
+
+
+# ---------------- FP32 refinement tunables ----------------
+
+@pytest.mark.parametrize("levels", [2, 3, 4, 5, 6, 7])
+def test_refined_window_levels_meet_1e12(gpu, oracle_lib, levels):
+    """Every window level count (halo 2^L - 1 block rows per side) meets 1e-12
+    against dgbsv: few levels make the weak generator's windows non-local, and
+    those batches fall back to the split path."""
+    saved = gpu.f32r_tune_get()
+    try:
+        gpu.f32r_tune_set(levels, saved[1])
+        for kind, layout, n, m in ((0, I, 40000, 3), (1, C, 33001, 2), (1, I, 2000, 40)):
+            _say("window-levels", levels, kind, layout, n, m)
+            if n <= 2048:
+                gpu.f32r_tune_set(levels, 0)   # the window path for this small batch too
+            dev = gpu.generate(kind, 700 + levels, n, m, layout)
+            ins = [_np(t).copy() for t in dev]
+            x, sw = _refined(gpu, dev, n, m, layout)
+            ref = oracle_lib.gbsv_batch(*ins, n, m, layout)
+            err = oracle_lib.normwise_relerr(x, ref, n, m, layout)
+            assert err <= 1e-12, (levels, kind, layout, n, err)
+            assert sw.min() >= 1 and sw.max() <= 3, sw
+            gpu.f32r_tune_set(levels, saved[1])
+    finally:
+        gpu.f32r_tune_set(*saved)
+
+
+def test_f32r_autotune_keeps_results(gpu, oracle_lib, tmp_path):
+    """The FP32 autotuner picks window levels and the window / one-workgroup
+    choice on a real batch; the results stay within 1e-12, the tunables
+    survive a save / load round trip."""
+    saved_f = gpu.f32r_tune_get()
+    saved_s = gpu.split_tune_get()
+    try:
+        for n, m in ((1800, 300), (6000, 6)):
+            _say("f32r-autotune", n, m)
+            dev = gpu.generate(0, 44 + n, n, m, I)
+            ins = [_np(t).copy() for t in dev]
+            wl, wp, ms = gpu.f32r_autotune(*dev, n, m, I, reps=1)
+            assert 3 <= wl <= 6 and ms > 0 and wp in (0, 1)
+            assert gpu.f32r_tune_get()[0] == wl
+            if n <= 2048:
+                assert (gpu.f32r_tune_get()[1] <= n * m) == bool(wp)
+            assert np.array_equal(_np(dev[5]), ins[5])
+            x, sw = _refined(gpu, dev, n, m, I)
+            ref = oracle_lib.gbsv_batch(*ins, n, m, I)
+            assert oracle_lib.normwise_relerr(x, ref, n, m, I) <= 1e-12
+        path = str(tmp_path / "tune.txt")
+        cur = gpu.f32r_tune_get()
+        gpu.tune_save(path)
+        gpu.f32r_tune_set(4, 123)
+        gpu.tune_load(path)
+        assert gpu.f32r_tune_get() == cur
+    finally:
+        gpu.f32r_tune_set(*saved_f)
+        gpu.split_tune_set(*saved_s[:3])
+
