@@ -12,10 +12,18 @@
  * (a,b,c,d,e,y) of the reference kernels, performanalysis.h:53-59.
  *
  * All array pointers are DEVICE pointers (hipMalloc'd or equivalent) unless
- * noted. All entry points are asynchronous on `stream` (a hipStream_t; NULL
- * means the null stream), never synchronise the device, never allocate device
- * memory, and return a status code (PCRP_OK = 0) instead of exiting the
- * process (the reference's checkCuda macro, performanalysis.h:12-20, exits).
+ * noted. Entry points never allocate device memory and return a status code
+ * (PCRP_OK = 0) instead of exiting the process (the reference's checkCuda
+ * macro, performanalysis.h:12-20, exits). Solves are asynchronous on
+ * `stream` (a hipStream_t; NULL means the null stream) with these
+ * exceptions: pcrp_solve_f32r_f64 on the window path (long systems, or
+ * large batches of n in (1024, 2048]) synchronises its stream once, after
+ * the first sweep, and the uneven entry points copy their plan tables from
+ * host memory on every call; neither can be captured in a hipGraph. The plan
+ * API (pcrp_plan_create / pcrp_plan_execute) does that work once at plan
+ * time and executes without any synchronisation or host copy (capturable).
+ * The autotuners, pcrp_plan_create and pcrp_heston_solve with use_graph = 1
+ * synchronise their stream.
  * The diagonals a..e are never modified; y is overwritten by the solution
  * (the reference overwrites a..e as well when n > 1024, kernel.cu:304-306).
  */
@@ -148,8 +156,9 @@ int pcrp_solve_f32r_f64(const double *a, const double *b, const double *c,
  * with pcrp_solve_f64 (bit for bit). No counterpart in the reference, whose
  * driver runs one (m, n) shape per launch (main.cu:245-260).
  * The plan tables are copied to the workspace on `stream` each call, from a
- * host buffer that lives only during the call (so the call cannot be
- * captured in a graph).
+ * host buffer that lives to the end of the call (hipMemcpyAsync stages
+ * pageable memory before returning), so the call cannot be captured in a
+ * graph: pcrp_plan_create uploads them once instead.
  */
 size_t pcrp_uneven_workspace_size(const int64_t *h_offsets, const int *h_n, int m,
                                   int precision);
@@ -183,6 +192,49 @@ int pcrp_solve_uneven_f32r_f64(const double *a, const double *b, const double *c
 int pcrp_generate_uneven(int kind, uint64_t seed, const int64_t *d_offsets, const int *d_n,
                          int m, double *a, double *b, double *c, double *d, double *e,
                          double *y, void *stream);
+
+/*
+ * Plan / execute (no counterpart in the reference, whose only "solve call" is
+ * a launch pair synchronised after every launch, main.cu:46-87). Of the
+ * one-call entry points above, the uneven ones upload their routing tables
+ * from host memory on every call, and FP32+refine of long systems reads
+ * back, after its first sweep, whether the window path applies (one stream
+ * synchronisation); neither can be captured in a hipGraph. A plan does that
+ * work once:
+ *   pcrp_plan_create: binds a..e (their values at this call: the plan keeps
+ *     matrix-dependent data), the batch shape and the workspace; uploads the
+ *     tables, runs the window path's locality test on the matrix and, for
+ *     interleaved batches, keeps system-contiguous copies of a..e in the
+ *     workspace. Synchronises `stream` (PCRP_ERR_ARG on a capturing stream).
+ *     Uniform batch: h_offsets = h_n = NULL, n, m, layout as for
+ *     pcrp_solve_f64. Uneven batch: h_offsets, h_n (HOST arrays) and m as for
+ *     pcrp_solve_uneven_f64 (n and layout ignored).
+ *   pcrp_plan_execute: solves y in place with the plan's route, launching
+ *     kernels only (no allocation, no host copy, no synchronisation: it can
+ *     be captured and replayed). Results equal the one-call entry points'
+ *     (FP64 bit for bit; FP32+refine the same sweeps and bits). max_sweeps,
+ *     tol, sweeps_out as for pcrp_solve_f32r_f64 (ignored for FP64).
+ * The workspace (pcrp_plan_workspace_size bytes) belongs to the plan until
+ * pcrp_plan_destroy: executes of one plan must not overlap in time.
+ */
+typedef struct pcrp_plan pcrp_plan;
+enum {
+    PCRP_ROUTE_LDS = 0,          /* one workgroup per system */
+    PCRP_ROUTE_SPLIT = 1,        /* HBM passes + LDS stage per residue sub-system */
+    PCRP_ROUTE_WINDOW = 2,       /* FP32+refine: one windowed pass per sweep */
+    PCRP_ROUTE_UNEVEN_F64 = 3,   /* uneven FP64: pass rounds + persistent LDS queue */
+    PCRP_ROUTE_UNEVEN_STAGE = 4  /* uneven FP32+refine: pass rounds + LDS stage per sweep */
+};
+size_t pcrp_plan_workspace_size(int precision, int n, int m, const int64_t *h_offsets,
+                                const int *h_n);
+int pcrp_plan_create(pcrp_plan **plan, int precision, const double *a, const double *b,
+                     const double *c, const double *d, const double *e, int n, int m,
+                     int layout, const int64_t *h_offsets, const int *h_n, void *ws,
+                     size_t ws_bytes, void *stream);
+int pcrp_plan_execute(const pcrp_plan *plan, double *y, int max_sweeps, double tol,
+                      int *sweeps_out, void *stream);
+int pcrp_plan_route(const pcrp_plan *plan);
+int pcrp_plan_destroy(pcrp_plan *plan);
 
 /*
  * Factor once, solve many (no counterpart in the reference, which recomputes

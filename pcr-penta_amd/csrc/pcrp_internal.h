@@ -118,6 +118,25 @@ int lds_f32r_factor(const SolveArgsF32R& A, void* F, hipStream_t st);
 size_t split_f32r_workspace(int n, int m);
 int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_bytes,
                      hipStream_t st);
+int split_f32r_passes(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st);
+
+// Plan forms of the uneven routes (pcrp_uneven.hip, pcrp_uneven_f32r.hip):
+// tables uploaded once at plan time, launches only at execute.
+struct UnevenF64Plan;
+UnevenF64Plan* uneven_f64_plan(const double* a, const double* b, const double* c, const double* d,
+                               const double* e, const int64_t* h_off, const int* h_n, int m,
+                               void* ws, size_t ws_bytes, hipStream_t st, int& rc);
+void uneven_f64_plan_free(UnevenF64Plan* U);
+int uneven_f64_plan_execute(const UnevenF64Plan* U, double* y, hipStream_t st);
+size_t uneven_f64_workspace(const int64_t* h_off, const int* h_n, int m);
+struct UnevenF32RPlan;
+UnevenF32RPlan* uneven_f32r_plan_tables(const int64_t* h_off, const int* h_n, int m, void* ws,
+                                        size_t ws_bytes, hipStream_t st, int& rc);
+void uneven_f32r_plan_free(UnevenF32RPlan* U);
+int uneven_f32r_plan_execute(const UnevenF32RPlan* U, const double* a, const double* b,
+                             const double* c, const double* d, const double* e, double* y, int m,
+                             int max_sweeps, double tol, int* sweeps_out, void* ws,
+                             hipStream_t st);
 // FP32 inner solve + FP64 refinement over an uneven batch (pcrp_uneven_f32r.hip).
 size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m);
 
@@ -166,6 +185,27 @@ int factor_f64(const double* a, const double* b, const double* c, const double* 
                hipStream_t st);
 int solve_factored_f64(const FactoredF64& A, hipStream_t st);
 size_t factor_f32r_bytes(int n);
+
+// Plan form of the window path (pcrp_window_f32r.hip): the matrix-only
+// preparation once (synchronous), then capturable executes.
+struct WindowLayoutOffsets {
+    size_t state, stop, flag, sys, lev, x0, x1, y0, six, bytes;
+};
+struct WindowPlan {
+    WindowLayoutOffsets L;
+    bool local, transposed, uniform;
+    long long rs, elems, nwin;
+    int n, m, nwu, wl;
+    void* ws;
+};
+int window_f32r_plan_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
+                             WindowPlan& P);
+int window_f32r_plan_uneven(const double* a, const double* b, const double* c, const double* d,
+                            const double* e, const int64_t* h_off, const int* h_n, int m,
+                            void* ws, size_t ws_bytes, hipStream_t st, WindowPlan& P);
+int window_f32r_plan_execute(const WindowPlan& P, const double* a, const double* b,
+                             const double* c, const double* d, const double* e, double* y,
+                             int max_sweeps, double tol, int* sweeps_out, hipStream_t st);
 
 inline int pcr_steps(long long n)
 {

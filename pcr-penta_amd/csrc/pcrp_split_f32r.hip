@@ -215,6 +215,12 @@ int split_f32r_solve(const SolveArgsF32R& A, int layout, void* ws, size_t ws_byt
     const int wrc = window_f32r_uniform(A, ws, ws_bytes, st, fell_back);
     if (wrc != PCRP_OK) return wrc == PCRP_ERR_WORKSPACE ? -3 : -1;
     if (!fell_back) return 0;
+    return split_f32r_passes(A, ws, ws_bytes, st);
+}
+
+// The pass/stage form alone (kernel launches only: capturable).
+int split_f32r_passes(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st)
+{
     const int n0 = A.n, m0 = A.m;
     const Layout P = plan(n0, m0);
     const int S = P.S;

@@ -155,14 +155,21 @@ size_t uneven_f64_workspace(const int64_t* h_off, const int* h_n, int m)
     return P.bytes;
 }
 
-int uneven_f64_solve(const double* a, const double* b, const double* c, const double* d,
-                     const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
-                     void* ws, size_t ws_bytes, hipStream_t st)
+namespace {
+
+// y of the next solve into the device pointer table (a plan binds a..e and
+// the workspace, not y)
+__global__ void set_rhs_kernel(const double** sets, double* y)
 {
-    UnevenPlan P;
-    int rc = plan_uneven(h_off, h_n, m, P);
-    if (rc != PCRP_OK) return rc;
-    if (!ws || ws_bytes < P.bytes) return PCRP_ERR_WORKSPACE;
+    if (threadIdx.x == 0) sets[5] = y;
+}
+
+// The plan's tables (pointer sets, LDS jobs, pass-round tables) to the front
+// of the workspace, from `host` (which must outlive the copy).
+int upload_plan(const UnevenPlan& P, const double* a, const double* b, const double* c,
+                const double* d, const double* e, double* y, void* ws, std::vector<char>& host,
+                hipStream_t st)
+{
     char* w = static_cast<char*>(ws);
     // array sets: 0 = caller's arrays, 1 and 2 = pass ping-pong buffers
     const double* sets[18] = {a, b, c, d, e, y};
@@ -170,7 +177,7 @@ int uneven_f64_solve(const double* a, const double* b, const double* c, const do
         sets[6 + v] = P.any_split
                           ? reinterpret_cast<double*>(w + P.off_arrays) + (size_t)v * P.total
                           : nullptr;
-    std::vector<char> host(P.off_arrays);
+    host.assign(P.off_arrays, 0);
     std::copy(reinterpret_cast<const char*>(P.qstart),
               reinterpret_cast<const char*>(P.qstart) + sizeof(P.qstart),
               host.begin() + P.off_counter + kQueues * sizeof(unsigned int));
@@ -182,8 +189,16 @@ int uneven_f64_solve(const double* a, const double* b, const double* c, const do
         std::copy(reinterpret_cast<const char*>(P.rounds[r].data()),
                   reinterpret_cast<const char*>(P.rounds[r].data() + P.rounds[r].size()),
                   host.begin() + P.round_off[r]);
-    if (hipMemcpyAsync(w, host.data(), host.size(), hipMemcpyHostToDevice, st) != hipSuccess)
-        return PCRP_ERR_HIP;
+    return hipMemcpyAsync(w, host.data(), host.size(), hipMemcpyHostToDevice, st) == hipSuccess
+               ? PCRP_OK
+               : PCRP_ERR_HIP;
+}
+
+// The pass rounds and the persistent LDS launch over uploaded tables.
+int launch_plan(const UnevenPlan& P, double* y, void* ws, hipStream_t st)
+{
+    char* w = static_cast<char*>(ws);
+    int rc = PCRP_OK;
     const double* const* dsets = reinterpret_cast<const double* const*>(w + P.off_sets);
     // pass rounds: round r reads set (r == 0 ? caller : 1 + (r-1)%2), writes 1 + r%2
     for (size_t r = 0; r < P.rounds.size(); ++r) {
@@ -204,6 +219,60 @@ int uneven_f64_solve(const double* a, const double* b, const double* c, const do
     rc = v->launch_queue(dsets, y, reinterpret_cast<const LdsJob*>(w + P.off_jobs),
                          (int)P.jobs.size(), qstart, counters, st);
     return rc == 0 ? PCRP_OK : PCRP_ERR_HIP;
+}
+
+}  // namespace
+
+int uneven_f64_solve(const double* a, const double* b, const double* c, const double* d,
+                     const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
+                     void* ws, size_t ws_bytes, hipStream_t st)
+{
+    UnevenPlan P;
+    int rc = plan_uneven(h_off, h_n, m, P);
+    if (rc != PCRP_OK) return rc;
+    if (!ws || ws_bytes < P.bytes) return PCRP_ERR_WORKSPACE;
+    // the host buffer lives to the end of this call (hipMemcpyAsync stages
+    // pageable memory before it returns)
+    std::vector<char> host;
+    rc = upload_plan(P, a, b, c, d, e, y, ws, host, st);
+    if (rc != PCRP_OK) return rc;
+    return launch_plan(P, y, ws, st);
+}
+
+// Plan form: tables uploaded once (synchronous), then capturable executes
+// that only set the right-hand side pointer and launch.
+struct UnevenF64Plan {
+    UnevenPlan P;
+    void* ws;
+};
+
+UnevenF64Plan* uneven_f64_plan(const double* a, const double* b, const double* c, const double* d,
+                               const double* e, const int64_t* h_off, const int* h_n, int m,
+                               void* ws, size_t ws_bytes, hipStream_t st, int& rc)
+{
+    UnevenF64Plan* U = new UnevenF64Plan;
+    U->ws = ws;
+    rc = plan_uneven(h_off, h_n, m, U->P);
+    if (rc == PCRP_OK && (!ws || ws_bytes < U->P.bytes)) rc = PCRP_ERR_WORKSPACE;
+    std::vector<char> host;
+    if (rc == PCRP_OK) rc = upload_plan(U->P, a, b, c, d, e, nullptr, ws, host, st);
+    if (rc == PCRP_OK && hipStreamSynchronize(st) != hipSuccess) rc = PCRP_ERR_HIP;
+    if (rc != PCRP_OK) {
+        delete U;
+        return nullptr;
+    }
+    return U;
+}
+
+void uneven_f64_plan_free(UnevenF64Plan* U) { delete U; }
+
+int uneven_f64_plan_execute(const UnevenF64Plan* U, double* y, hipStream_t st)
+{
+    char* w = static_cast<char*>(U->ws);
+    hipLaunchKernelGGL(set_rhs_kernel, dim3(1), dim3(64), 0, st,
+                       reinterpret_cast<const double**>(w + U->P.off_sets), y);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    return launch_plan(U->P, y, U->ws, st);
 }
 
 }  // namespace pcrp

@@ -311,24 +311,15 @@ size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m)
     return std::max(P.bytes, window_f32r_workspace(P.rows, m, false));
 }
 
-int uneven_f32r_solve(const double* a, const double* b, const double* c, const double* d,
-                      const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
-                      int max_sweeps, double tol, int* sweeps_out, void* ws, size_t ws_bytes,
-                      hipStream_t st)
+namespace {
+
+// Pass/stage route, tables: the host plan's system, job and pass-round
+// tables to the front of the workspace in one copy (from a host buffer that
+// must outlive the copy: `host`).
+int upload_tables(const Plan& P, void* ws, std::vector<char>& host, hipStream_t st)
 {
-    Plan P;
-    int rc = plan_uneven(h_off, h_n, m, P);
-    if (rc != PCRP_OK) return rc;
-    if (!ws || ws_bytes < std::max(P.bytes, window_f32r_workspace(P.rows, m, false)))
-        return PCRP_ERR_WORKSPACE;
-    // diagonally dominant batches: one windowed pass per sweep
-    bool fell_back = true;
-    rc = window_f32r_uneven(a, b, c, d, e, y, h_off, h_n, m, max_sweeps, tol, sweeps_out, ws,
-                            ws_bytes, st, fell_back);
-    if (rc != PCRP_OK || !fell_back) return rc;
     char* w = static_cast<char*>(ws);
-    // host tables -> the front of the workspace in one copy
-    std::vector<char> host(P.off_y0, 0);
+    host.assign(P.off_y0, 0);
     std::copy(reinterpret_cast<const char*>(P.sys.data()),
               reinterpret_cast<const char*>(P.sys.data() + P.sys.size()), host.begin() + P.off_sys);
     std::copy(reinterpret_cast<const char*>(P.jobs.data()),
@@ -338,9 +329,57 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
         std::copy(reinterpret_cast<const char*>(P.rounds[r].data()),
                   reinterpret_cast<const char*>(P.rounds[r].data() + P.rounds[r].size()),
                   host.begin() + P.round_off[r]);
-    if (hipMemcpyAsync(w + P.off_sys, host.data() + P.off_sys, host.size() - P.off_sys,
-                       hipMemcpyHostToDevice, st) != hipSuccess)
-        return PCRP_ERR_HIP;
+    return hipMemcpyAsync(w + P.off_sys, host.data() + P.off_sys, host.size() - P.off_sys,
+                          hipMemcpyHostToDevice, st) == hipSuccess
+               ? PCRP_OK
+               : PCRP_ERR_HIP;
+}
+
+int launch_pass_stage(const Plan& P, const double* a, const double* b, const double* c,
+                      const double* d, const double* e, double* y, int m, int max_sweeps,
+                      double tol, int* sweeps_out, void* ws, hipStream_t st);
+
+}  // namespace
+
+// Plan form of the pass/stage route: tables uploaded once (synchronous),
+// launches only at execute.
+struct UnevenF32RPlan {
+    Plan P;
+};
+
+UnevenF32RPlan* uneven_f32r_plan_tables(const int64_t* h_off, const int* h_n, int m, void* ws,
+                                        size_t ws_bytes, hipStream_t st, int& rc)
+{
+    UnevenF32RPlan* U = new UnevenF32RPlan;
+    rc = plan_uneven(h_off, h_n, m, U->P);
+    if (rc == PCRP_OK && (!ws || ws_bytes < U->P.bytes)) rc = PCRP_ERR_WORKSPACE;
+    std::vector<char> host;
+    if (rc == PCRP_OK) rc = upload_tables(U->P, ws, host, st);
+    if (rc == PCRP_OK && hipStreamSynchronize(st) != hipSuccess) rc = PCRP_ERR_HIP;
+    if (rc != PCRP_OK) {
+        delete U;
+        return nullptr;
+    }
+    return U;
+}
+
+void uneven_f32r_plan_free(UnevenF32RPlan* U) { delete U; }
+
+int uneven_f32r_plan_execute(const UnevenF32RPlan* U, const double* a, const double* b,
+                             const double* c, const double* d, const double* e, double* y, int m,
+                             int max_sweeps, double tol, int* sweeps_out, void* ws,
+                             hipStream_t st)
+{
+    return launch_pass_stage(U->P, a, b, c, d, e, y, m, max_sweeps, tol, sweeps_out, ws, st);
+}
+
+namespace {
+
+int launch_pass_stage(const Plan& P, const double* a, const double* b, const double* c,
+                      const double* d, const double* e, double* y, int m, int max_sweeps,
+                      double tol, int* sweeps_out, void* ws, hipStream_t st)
+{
+    char* w = static_cast<char*>(ws);
     SysState* state = reinterpret_cast<SysState*>(w + P.off_state);
     int* stop = reinterpret_cast<int*>(w + P.off_stop);
     const SysInfo* dsys = reinterpret_cast<const SysInfo*>(w + P.off_sys);
@@ -399,6 +438,31 @@ int uneven_f32r_solve(const double* a, const double* b, const double* c, const d
         if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     }
     return PCRP_OK;
+}
+
+}  // namespace
+
+int uneven_f32r_solve(const double* a, const double* b, const double* c, const double* d,
+                      const double* e, double* y, const int64_t* h_off, const int* h_n, int m,
+                      int max_sweeps, double tol, int* sweeps_out, void* ws, size_t ws_bytes,
+                      hipStream_t st)
+{
+    Plan P;
+    int rc = plan_uneven(h_off, h_n, m, P);
+    if (rc != PCRP_OK) return rc;
+    if (!ws || ws_bytes < std::max(P.bytes, window_f32r_workspace(P.rows, m, false)))
+        return PCRP_ERR_WORKSPACE;
+    // diagonally dominant batches: one windowed pass per sweep
+    bool fell_back = true;
+    rc = window_f32r_uneven(a, b, c, d, e, y, h_off, h_n, m, max_sweeps, tol, sweeps_out, ws,
+                            ws_bytes, st, fell_back);
+    if (rc != PCRP_OK || !fell_back) return rc;
+    // the tables are copied from a host buffer that lives to the end of this
+    // call (the copy is staged before hipMemcpyAsync returns for pageable memory)
+    std::vector<char> host;
+    rc = upload_tables(P, ws, host, st);
+    if (rc != PCRP_OK) return rc;
+    return launch_pass_stage(P, a, b, c, d, e, y, m, max_sweeps, tol, sweeps_out, ws, st);
 }
 
 }  // namespace pcrp

@@ -433,9 +433,7 @@ size_t lev_capacity(long long elems, int m)
 
 // Direct: x buffer and copy of y (16 B per element). Transposed (interleaved
 // uniform batches): six system-contiguous arrays and two x buffers (64 B).
-struct WLayout {
-    size_t state, stop, flag, sys, lev, x0, x1, y0, six, bytes;
-};
+using WLayout = WindowLayoutOffsets;
 
 WLayout wlayout(long long elems, int m, bool transposed)
 {
@@ -566,6 +564,152 @@ int window_run(const double* a, const double* b, const double* c, const double* 
     return PCRP_OK;
 }
 
+__global__ void clear_flag_kernel(int* f) { *f = 0; }
+
+// ---- plan form: everything that depends on the matrix only, once ----------
+// window_plan prepares the batch (system table, for interleaved uniform
+// batches the system-contiguous copies of a..e, each window's level count
+// from one level-finding sweep, the locality flag read back: synchronous);
+// window_plan_execute then runs only kernels (capturable, no host copy, no
+// synchronisation): every sweep reuses the levels, and y is transposed in
+// and out when the batch is.
+int window_plan(const double* a, const double* b, const double* c, const double* d,
+                const double* e, std::vector<WSys>& sys, bool uniform, long long rs,
+                long long elems, bool transposed, int n, void* ws, size_t ws_bytes,
+                hipStream_t st, WindowPlan& P)
+{
+    P = WindowPlan{};
+    P.local = false;
+    const int m = (int)sys.size();
+    if (transposed) {
+        rs = 1;
+        for (int j = 0; j < m; ++j) sys[j].off = (long long)j * n;
+    }
+    const int wl = f32r_tune_get().win_levels, wT = win_interior(wl);
+    long long nwin = 0;
+    for (auto& s : sys) {
+        s.shift = rs == 1 ? (int)(s.off & 1) : 0;
+        s.nb = (s.n + s.shift + 1) / 2;
+        s.nw = windows_for(s.nb, wT);
+        s.T = (s.nb + s.nw - 1) / s.nw;
+        s.w0 = nwin;
+        nwin += s.nw;
+    }
+    const WLayout L = wlayout(elems, m, transposed);
+    if (nwin > 0x7fffffffLL || (size_t)nwin > lev_capacity(elems, m)) return PCRP_OK;   // not local
+    if (!ws || ws_bytes < L.bytes) return PCRP_ERR_WORKSPACE;
+    char* w = static_cast<char*>(ws);
+    SysState* state = reinterpret_cast<SysState*>(w + L.state);
+    int* stop = reinterpret_cast<int*>(w + L.stop);
+    int* flag = reinterpret_cast<int*>(w + L.flag);
+    WSys* dsys = reinterpret_cast<WSys*>(w + L.sys);
+    double* x0 = reinterpret_cast<double*>(w + L.x0);
+    double* six = reinterpret_cast<double*>(w + L.six);
+    unsigned char* lev = reinterpret_cast<unsigned char*>(w + L.lev);
+    const long long N = elems;
+    if (hipMemcpyAsync(dsys, sys.data(), sys.size() * sizeof(WSys), hipMemcpyHostToDevice, st) !=
+        hipSuccess)
+        return PCRP_ERR_HIP;
+    const unsigned sblocks = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, st, flag);
+    hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
+    const dim3 tgrid((unsigned)((n + kTrRows - 1) / kTrRows), (unsigned)((m + kTrSys - 1) / kTrSys));
+    if (transposed) {
+        if (tgrid.x > 0x7fffffffu || tgrid.y > 65535u) return PCRP_ERR_INVALID_SIZE;
+        // a..e once: the plan keeps their system-contiguous copies
+        hipLaunchKernelGGL(wtranspose_in_kernel, dim3(tgrid.x, tgrid.y, 5), dim3(256), 0, st,
+                           Six{{a, b, c, d, e, nullptr}}, six, n, m);
+    }
+    WArgs A{transposed ? six : a, transposed ? six + N : b, transposed ? six + 2 * N : c,
+            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
+            nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0,
+            uniform ? sys[0].nw : 0, 0, wl, win_halo(wl)};
+    // level-finding sweep: its right-hand side is any finite array of the
+    // batch's shape (the level counts and the flag depend on the matrix only)
+    A.rhs = A.c;
+    A.xout = x0;
+    hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR, true>), dim3((unsigned)nwin), dim3(kWinNT),
+                       0, st, A);
+    if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    int h = 0;
+    if (hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return PCRP_ERR_HIP;
+    P.local = h == 0;
+    P.L = L;
+    P.transposed = transposed;
+    P.uniform = uniform;
+    P.rs = rs;
+    P.elems = elems;
+    P.nwin = nwin;
+    P.n = n;
+    P.m = m;
+    P.nwu = uniform ? sys[0].nw : 0;
+    P.wl = wl;
+    P.ws = ws;
+    return PCRP_OK;
+}
+
+int window_plan_execute(const WindowPlan& P, const double* a, const double* b, const double* c,
+                        const double* d, const double* e, double* y, int max_sweeps, double tol,
+                        int* sweeps_out, hipStream_t st)
+{
+    if (!P.local) return PCRP_ERR_ARG;
+    char* w = static_cast<char*>(P.ws);
+    const WLayout& L = P.L;
+    SysState* state = reinterpret_cast<SysState*>(w + L.state);
+    int* stop = reinterpret_cast<int*>(w + L.stop);
+    int* flag = reinterpret_cast<int*>(w + L.flag);
+    WSys* dsys = reinterpret_cast<WSys*>(w + L.sys);
+    double* x0 = reinterpret_cast<double*>(w + L.x0);
+    double* x1 = reinterpret_cast<double*>(w + L.x1);
+    double* y0 = reinterpret_cast<double*>(w + L.y0);
+    double* six = reinterpret_cast<double*>(w + L.six);
+    unsigned char* lev = reinterpret_cast<unsigned char*>(w + L.lev);
+    const bool transposed = P.transposed;
+    const long long N = P.elems;
+    const int m = P.m, n = P.n;
+    static const int gWin = resident_grid(wsweep_kernel<kWinNT, kWinBR, false>, kWinNT);
+    static const int gCopy = resident_grid(wcopy_kernel, 256);
+    const unsigned sblocks = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
+    const dim3 tgrid((unsigned)((n + kTrRows - 1) / kTrRows), (unsigned)((m + kTrSys - 1) / kTrSys));
+    if (transposed)
+        hipLaunchKernelGGL(wtranspose_in_kernel, dim3(tgrid.x, tgrid.y, 1), dim3(256), 0, st,
+                           Six{{y, nullptr, nullptr, nullptr, nullptr, nullptr}}, six + 5 * N, n,
+                           m);
+    WArgs A{transposed ? six : a, transposed ? six + N : b, transposed ? six + 2 * N : c,
+            transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
+            nullptr, P.rs, dsys, state, stop, flag, lev, m, (int)P.nwin, P.uniform ? 1 : 0,
+            P.nwu, 0, P.wl, win_halo(P.wl)};
+    {
+        const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+        A.vec = P.rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
+                al16(y) && al16(x0) && al16(x1) && al16(y0) && al16(six) ? 1 : 0;
+    }
+    double* const xbuf[3] = {x0, x1, transposed ? x0 : y};
+    const int cyc = transposed ? 2 : 3;
+    for (int it = 0; it <= max_sweeps; ++it) {
+        A.rhs = transposed ? six + 5 * N : (it == 0 ? y : y0);
+        A.y0 = transposed || it > 0 ? nullptr : y0;
+        A.xin = it == 0 ? nullptr : xbuf[(it - 1) % cyc];
+        A.xout = xbuf[it % cyc];
+        hipLaunchKernelGGL((wsweep_kernel<kWinNT, kWinBR, false>),
+                           dim3(walk_grid(P.nwin, it < kFullGridSweeps ? 0 : gWin)), dim3(kWinNT),
+                           0, st, A);
+        hipLaunchKernelGGL(decide_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m,
+                           max_sweeps, tol, sweeps_out);
+        if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
+    }
+    if (transposed)
+        hipLaunchKernelGGL(wtranspose_out_kernel, dim3(tgrid.x, tgrid.y), dim3(256), 0, st, xbuf[0],
+                           xbuf[1], state, y, n, m);
+    else
+        hipLaunchKernelGGL(wcopy_kernel, dim3(walk_grid(P.nwin, gCopy)), dim3(256), 0, st, A,
+                           xbuf[0], xbuf[1], y);
+    return hipGetLastError() == hipSuccess ? PCRP_OK : PCRP_ERR_HIP;
+}
+
 }  // namespace
 
 static F32RTune g_f32r = {5, 1LL << 23};
@@ -618,6 +762,37 @@ int window_f32r_uneven(const double* a, const double* b, const double* c, const 
     }
     return window_run(a, b, c, d, e, y, sys, false, 1, elems, false, 0, max_sweeps, tol,
                       sweeps_out, ws, ws_bytes, st, fell_back);
+}
+
+int window_f32r_plan_uniform(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStream_t st,
+                             WindowPlan& P)
+{
+    std::vector<WSys> sys(A.m);
+    const int nb = (A.n + 1) / 2;
+    for (int j = 0; j < A.m; ++j) sys[j] = WSys{(long long)j * A.ss, 0, A.n, nb, 0, 0};
+    return window_plan(A.a, A.b, A.c, A.d, A.e, sys, true, A.rs, (long long)A.n * A.m,
+                       window_transposed(A.m, A.rs), A.n, ws, ws_bytes, st, P);
+}
+
+int window_f32r_plan_uneven(const double* a, const double* b, const double* c, const double* d,
+                            const double* e, const int64_t* h_off, const int* h_n, int m,
+                            void* ws, size_t ws_bytes, hipStream_t st, WindowPlan& P)
+{
+    std::vector<WSys> sys(m);
+    long long elems = 0;
+    for (int j = 0; j < m; ++j) {
+        if (h_n[j] < 1 || h_off[j] < 0) return PCRP_ERR_INVALID_SIZE;
+        sys[j] = WSys{h_off[j], 0, h_n[j], (h_n[j] + 1) / 2, 0, 0};
+        elems = std::max(elems, (long long)h_off[j] + h_n[j]);
+    }
+    return window_plan(a, b, c, d, e, sys, false, 1, elems, false, 0, ws, ws_bytes, st, P);
+}
+
+int window_f32r_plan_execute(const WindowPlan& P, const double* a, const double* b,
+                             const double* c, const double* d, const double* e, double* y,
+                             int max_sweeps, double tol, int* sweeps_out, hipStream_t st)
+{
+    return window_plan_execute(P, a, b, c, d, e, y, max_sweeps, tol, sweeps_out, st);
 }
 
 }  // namespace pcrp

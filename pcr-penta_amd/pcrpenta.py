@@ -113,6 +113,17 @@ def _declare(lib):
     lib.pcrp_f32r_autotune.argtypes = [ctypes.c_int] * 3 + [_vp] * 8 + \
         [ctypes.c_size_t, ctypes.c_int, _vp, ip, ip, fp]
     lib.pcrp_f32r_autotune.restype = ctypes.c_int
+    lib.pcrp_plan_workspace_size.argtypes = [ctypes.c_int] * 3 + [_vp, _vp]
+    lib.pcrp_plan_workspace_size.restype = ctypes.c_size_t
+    lib.pcrp_plan_create.argtypes = ([ctypes.POINTER(_vp), ctypes.c_int] + [_vp] * 5 +
+                                     [ctypes.c_int] * 3 + [_vp, _vp, _vp, ctypes.c_size_t, _vp])
+    lib.pcrp_plan_create.restype = ctypes.c_int
+    lib.pcrp_plan_execute.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp]
+    lib.pcrp_plan_execute.restype = ctypes.c_int
+    lib.pcrp_plan_route.argtypes = [_vp]
+    lib.pcrp_plan_route.restype = ctypes.c_int
+    lib.pcrp_plan_destroy.argtypes = [_vp]
+    lib.pcrp_plan_destroy.restype = ctypes.c_int
     lib.pcrp_tune_save.argtypes = [ctypes.c_char_p]
     lib.pcrp_tune_save.restype = ctypes.c_int
     lib.pcrp_tune_load.argtypes = [ctypes.c_char_p]
@@ -213,6 +224,63 @@ def solve_refined(a, b, c, d, e, y, n: int, m: int, layout: int = INTERLEAVED, m
         _keep_until_done(ws, stream)
     _check(st, "pcrp_solve_f32r_f64")
     return y
+
+
+# ---- plan / execute ------------------------------------------------------------
+
+ROUTE_LDS, ROUTE_SPLIT, ROUTE_WINDOW, ROUTE_UNEVEN_F64, ROUTE_UNEVEN_STAGE = range(5)
+
+
+class Plan:
+    """A solve plan (include/pcr_penta.h): matrix-dependent setup once, then
+    executes that only launch kernels (capturable in a CUDA/HIP graph)."""
+
+    def __init__(self, a, b, c, d, e, n: int = 0, m: int = 0, layout: int = INTERLEAVED,
+                 precision: int = PREC_F64, offsets=None, sizes=None, stream=None):
+        import numpy as np
+        import torch
+        lib = load()
+        self._lib = lib
+        self.precision = precision
+        if offsets is not None:
+            self._off = np.ascontiguousarray(offsets, dtype=np.int64)
+            self._n = np.ascontiguousarray(sizes, dtype=np.int32)
+            m = len(self._n)
+            po, pn = self._off.ctypes.data, self._n.ctypes.data
+        else:
+            _validate([a, b, c, d, e], n, m)
+            po = pn = None
+        nb = int(lib.pcrp_plan_workspace_size(precision, n, m, po, pn))
+        self.ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=a.device)
+        self._keep = (a, b, c, d, e)   # the plan binds these arrays
+        h = _vp()
+        st = lib.pcrp_plan_create(ctypes.byref(h), precision, *[_ptr(t) for t in (a, b, c, d, e)],
+                                  n, m, layout, po, pn, _ptr(self.ws), self.ws.numel(),
+                                  _stream(stream))
+        _check(st, "pcrp_plan_create")
+        self._h = h
+        self.n, self.m = n, m
+
+    @property
+    def route(self) -> int:
+        return self._lib.pcrp_plan_route(self._h)
+
+    def execute(self, y, max_sweeps: int = 8, tol: float = 1e-12, sweeps=None, stream=None):
+        sp = _ptr(sweeps) if sweeps is not None else None
+        st = self._lib.pcrp_plan_execute(self._h, _ptr(y), max_sweeps, tol, sp, _stream(stream))
+        _check(st, "pcrp_plan_execute")
+        return y
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            self._lib.pcrp_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 # ---- factor once, solve many -------------------------------------------------
