@@ -213,11 +213,12 @@ def tune_uneven_split(B):
     a, b, c, d, e, y0 = B["arrs"]
     off, nn = B["off"], B["nn"]
     stream = torch.cuda.current_stream()
-    lds_max, _, _, n_shapes = pcrpenta.split_tune_get()
+    _, _, _, n_shapes = pcrpenta.split_tune_get()
     best = None
     ys_t = y0.clone()
-    for kf in (2, 3, 4, 5):
-        for sh in range(n_shapes):
+    for lds_max, kf, sh in ((lm, kf, sh) for lm in (512, 1024) for kf in (2, 3, 4, 5)
+                            for sh in range(n_shapes)):
+        if True:
             try:
                 pcrpenta.split_tune_set(lds_max, kf, sh)
             except pcrpenta.PCRPError:
@@ -234,10 +235,75 @@ def tune_uneven_split(B):
             torch.cuda.synchronize()
             t = e0.elapsed_time(e1)
             if best is None or t < best[0]:
-                best = (t, kf, sh)
+                best = (t, kf, sh, lds_max)
             del ws
-    pcrpenta.split_tune_set(lds_max, best[1], best[2])
-    return {"k_fuse": best[1], "shape": best[2], "ms": best[0]}
+    pcrpenta.split_tune_set(best[3], best[1], best[2])
+    return {"lds_max": best[3], "k_fuse": best[1], "shape": best[2], "ms": best[0]}
+
+
+def tune_uneven_f32r(B):
+    """Window level count for the FP32 refinement of this batch (halo 2^L - 1
+    block rows): the fastest of L = 4, 5, 6 on one solve each."""
+    import torch
+    import pcrpenta
+    a, b, c, d, e, y0 = B["arrs"]
+    off, nn = B["off"], B["nn"]
+    keep = pcrpenta.f32r_tune_get()
+    nb = pcrpenta.uneven_workspace_size(off, nn, pcrpenta.PREC_F32_REFINE)
+    ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y0.device)
+    y = y0.clone()
+    stream = torch.cuda.current_stream()
+    best = None
+    for wl in (4, 5, 6):
+        pcrpenta.f32r_tune_set(wl, keep[1])
+        y.copy_(y0)
+        pcrpenta.solve_uneven_refined(a, b, c, d, e, y, off, nn, stream=stream, ws=ws)
+        y.copy_(y0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        pcrpenta.solve_uneven_refined(a, b, c, d, e, y, off, nn, stream=stream, ws=ws)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1)
+        if best is None or t < best[0]:
+            best = (t, wl)
+    pcrpenta.f32r_tune_set(best[1], keep[1])
+    return {"win_levels": best[1], "ms": best[0]}
+
+
+def plan_execute_ms(B, prec, K):
+    """Mean event-timed execute of a plan (pcrp_plan_create once: routing
+    tables, window levels; executes launch kernels only) over K pristine
+    right-hand sides."""
+    import torch
+    import pcrpenta
+    a, b, c, d, e, y0 = B["arrs"]
+    pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
+    if "off" in B:
+        plan = pcrpenta.Plan(a, b, c, d, e, precision=pid, offsets=B["off"], sizes=B["nn"])
+    else:
+        plan = pcrpenta.Plan(a, b, c, d, e, B["n"], B["m"], B["layout"], precision=pid)
+    ys = [y0.clone() for _ in range(2)]
+
+    def step(k):
+        plan.execute(ys[k % 2], max_sweeps=MAX_SWEEPS, tol=TOL)
+        ys[k % 2].copy_(y0)   # (not timed: events bracket the execute only)
+
+    stream = torch.cuda.current_stream()
+    step(0)
+    torch.cuda.synchronize()
+    tot = 0.0
+    for k in range(K):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        plan.execute(ys[k % 2], max_sweeps=MAX_SWEEPS, tol=TOL)
+        e1.record(stream)
+        ys[k % 2].copy_(y0)
+        torch.cuda.synchronize()
+        tot += e0.elapsed_time(e1)
+    route = plan.route
+    plan.destroy()
+    return {"ms": tot / K, "route": ["lds", "split", "window", "uneven-queue", "uneven-stage"][route]}
 
 
 def run_uneven(B, prec, K, W):
@@ -284,12 +350,15 @@ def bench_headline(args):
     K, W = args.steps, args.warmup
     B = uneven_batch()
     off, nn, total, m = B["off"], B["nn"], B["total"], B["m"]
-    tuned = None
+    tuned = tuned32 = None
     if not args.no_tune:
         tuned = tune_uneven_split(B)
         log({"split_autotune": tuned})
+        tuned32 = tune_uneven_f32r(B)
+        log({"f32r_autotune": tuned32})
     f64 = run_uneven(B, "f64", K, W)
     f32 = run_uneven(B, "f32r", K, W)
+    plan32 = plan_execute_ms(B, "f32r", 5)
     ratio = f32["value"] / f64["value"]
     out = {
         "metric": METRIC, "value": f64["value"], "unit": "unknowns/s", "n_gpus": 1,
@@ -308,10 +377,15 @@ def bench_headline(args):
         "f32r": {
             "value": f32["value"], "unit": "unknowns/s", "ms_per_step": f32["ms_per_step"],
             "ratio_vs_f64": ratio, "dtype": "f32 inner solve, f64 residual and update",
-            "refine": {"max_sweeps": MAX_SWEEPS, "tol": TOL, "sweeps_hist": f32["sweeps_hist"]},
+            "refine": {"max_sweeps": MAX_SWEEPS, "tol": TOL, "sweeps_hist": f32["sweeps_hist"],
+                       "win_levels": pcrpenta.f32r_tune_get()[0], "autotune": tuned32},
             "roofline": roofline(total, f32["solve_s"], "uneven_f32r_m10000",
                                  "FP32+refine, whole solve per step (window-path sweeps, one "
                                  "kernel each, and the final copy)"),
+            "plan_execute": {**plan32,
+                             "unknowns_per_s": total / (plan32["ms"] / 1e3),
+                             "note": "repeat solves of the same matrices: pcrp_plan_create "
+                                     "once, then capturable executes"},
         },
     }
     if not args.no_cpu:
@@ -387,18 +461,20 @@ def run_uniform(n, m, prec, kind, K, W, tune=True, layout=None):
     step(0, y_check)
     torch.cuda.synchronize()
     out = {"value": n * m / (wall / K), "ms_per_step": wall * 1e3 / K, "solve_s": solve_s,
-           "tuned": tuned, "x": y_check, "layout": layout}
+           "tuned": tuned, "x": y_check, "layout": layout,
+           "batch": {"arrs": (a, b, c, d, e, y0), "n": n, "m": m, "layout": layout}}
     if prec != "f64":
         out["sweeps_hist"] = torch.bincount(sweeps.long()).tolist()
     return out
 
 
-def run_heston(prec, K, W):
-    """K timed Heston ADI runs (200 Douglas steps each, one hipGraph per step)."""
+def run_heston(prec, K, W, reuse=False):
+    """K timed Heston ADI runs (200 Douglas steps each, captured 25 steps per
+    hipGraph); reuse: the line matrices factored once per run."""
     import torch
     import pcrpenta
     pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
-    p = pcrpenta.heston_params()
+    p = pcrpenta.heston_params(reuse_factor=1 if reuse else 0)
     s = torch.cuda.Stream()
     u = torch.empty(p.ns * p.nv, dtype=torch.float64, device="cuda")
     ws = torch.empty(int(pcrpenta.load().pcrp_heston_workspace_size(
@@ -425,15 +501,25 @@ def also_workloads(args):
     out = {}
     h64 = run_heston("f64", 5, 2)
     h32 = run_heston("f32r", 5, 2)
+    h64r = run_heston("f64", 5, 2, reuse=True)
+    h32r = run_heston("f32r", 5, 2, reuse=True)
     p = h64["p"]
     hS, hv = p.smax / (p.ns + 1), p.vmax / p.nv
     U64 = h64["u"].view(p.nv, p.ns).cpu().numpy()
     U32 = h32["u"].view(p.nv, p.ns).cpu().numpy()
     out["heston_adi"] = {
-        "grid": [p.ns, p.nv], "time_steps": p.steps, "graph": "one hipGraph per time step",
+        "grid": [p.ns, p.nv], "time_steps": p.steps,
+        "graph": "25 Douglas steps per hipGraph, replayed",
         "f64_ms_per_run": h64["ms_per_run"], "f32r_ms_per_run": h32["ms_per_run"],
         "f64_unknowns_per_s": h64["value"], "f32r_unknowns_per_s": h32["value"],
         "ratio_f32r_vs_f64": h64["ms_per_run"] / h32["ms_per_run"],
+        "factor_reuse": {
+            "f64_ms_per_run": h64r["ms_per_run"], "f32r_ms_per_run": h32r["ms_per_run"],
+            "ratio_f32r_reuse_vs_plain_f64": h64["ms_per_run"] / h32r["ms_per_run"],
+            "ratio_f32r_reuse_vs_f64_reuse": h64r["ms_per_run"] / h32r["ms_per_run"],
+            "note": "line matrices factored once per run: FP64 every S-line and the shared "
+                    "v-line matrix, FP32+refine the shared v-line matrix (bit-identical "
+                    "results to the full solves)"},
         "f32r_vs_f64_grid_relerr": float(np.abs(U32 - U64).max() / np.abs(U64).max()),
         "price_at_S100_v0.04_f64": float(_interp_price(U64, p, hS, hv)),
         "refine": {"tol": 1e-12, "max_sweeps": 8},
@@ -443,6 +529,7 @@ def also_workloads(args):
                               ("hbm_n1048576_m32", 1 << 20, 32, 1)):
         r64 = run_uniform(n, m, "f64", kind, K, W)
         r32 = run_uniform(n, m, "f32r", kind, K, W)
+        p32 = plan_execute_ms(r32["batch"], "f32r", 5)
         out[label] = {
             "n": n, "m": m, "generator": "dominant" if kind == 0 else "weak",
             "layout": "interleaved",
@@ -452,6 +539,8 @@ def also_workloads(args):
             "f32r_roofline_frac": BYTES_PER_UNKNOWN * n * m / r32["solve_s"] / HBM_PEAK,
             "ratio_f32r_vs_f64": r32["value"] / r64["value"],
             "f32r_sweeps_hist": r32["sweeps_hist"],
+            "f32r_plan_execute_ms": p32["ms"], "f32r_plan_route": p32["route"],
+            "ratio_f32r_plan_vs_f64": r64["ms_per_step"] / p32["ms"],
             "f64_tuned": (r64["tuned"] or {}).get("variant", (r64["tuned"] or {}).get("split")),
         }
         del r64, r32
