@@ -33,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "pcr-penta_amd"))
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 BYTES_PER_UNKNOWN = 56     # 7 FP64 words
+VALU_PEAK = 256 * 2.4e9    # wave64 VALU instructions/s: 1 per CU per cycle
 METRIC = "unknowns solved/s + % HBM roofline; FP32+refine vs FP64 at rel err <=1e-12"
 HEADLINE = "uneven_m10000"
 
@@ -54,6 +55,9 @@ WORKLOADS = {
     # 200-step run
     "heston_adi_f64": (-1, 0, "f64", 0),
     "heston_adi_f32r": (-1, 0, "f32r", 0),
+    # n = -2: the ADI driver with the line matrices factored once per run
+    "heston_adi_f64_reuse": (-2, 0, "f64", 0),
+    "heston_adi_f32r_reuse": (-2, 0, "f32r", 0),
 }
 MAX_SWEEPS, TOL = 8, 1e-12
 RHS_BYTES_CAP = 64e9       # pristine right-hand sides kept in HBM for the timed steps
@@ -78,6 +82,8 @@ def parse():
     ap.add_argument("--split", default=None, metavar="K_FUSE,SHAPE",
                     help="fix the split path's steps per pass and pass tile shape "
                          "(implies --no-tune for the split path)")
+    ap.add_argument("--win-levels", type=int, default=None, metavar="L",
+                    help="fix the FP32 refinement's window level count (2..7)")
     args = ap.parse_args()
     if args.split:
         kf, sh = (int(v) for v in args.split.split(","))
@@ -127,18 +133,38 @@ def measured_traffic(workload):
         return json.load(f).get(workload, {}).get("bytes_per_step")
 
 
+def measured_valu(workload):
+    """VALU wave-instructions of one solve (SQ_INSTS_VALU summed over every
+    solver kernel of one step, rocprofv3); None for workloads not profiled."""
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    with open(tpath) as f:
+        return json.load(f).get(workload, {}).get("valu_insts_per_step")
+
+
 def roofline(unknowns, solve_s, workload, note):
     achieved = BYTES_PER_UNKNOWN * unknowns / solve_s
-    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": measured_traffic(workload),
-            "solve_ms": solve_s * 1e3, "note": note}
+    out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK, "traffic": measured_traffic(workload),
+           "solve_ms": solve_s * 1e3, "note": note}
+    insts = measured_valu(workload)
+    if insts:
+        # issue bound: one wave64 VALU instruction per CU per cycle (4 SIMDs,
+        # 16 lanes each), 256 CUs at 2.4 GHz; FP64 FMA issues at that rate
+        out["valu"] = {"insts_per_solve": insts, "achieved": insts / solve_s / 1e9,
+                       "peak": VALU_PEAK / 1e9, "unit": "G wave-instructions/s",
+                       "frac": insts / solve_s / VALU_PEAK}
+    return out
 
 
 def apply_split(args):
+    import pcrpenta
     if args.split:
-        import pcrpenta
         lds_max = pcrpenta.split_tune_get()[0]
         pcrpenta.split_tune_set(lds_max, *args.split)
+    if args.win_levels is not None:
+        pcrpenta.f32r_tune_set(args.win_levels, pcrpenta.f32r_tune_get()[1])
 
 
 # ---------------------------------------------------------------- CPU side
@@ -218,25 +244,24 @@ def tune_uneven_split(B):
     ys_t = y0.clone()
     for lds_max, kf, sh in ((lm, kf, sh) for lm in (512, 1024) for kf in (2, 3, 4, 5)
                             for sh in range(n_shapes)):
-        if True:
-            try:
-                pcrpenta.split_tune_set(lds_max, kf, sh)
-            except pcrpenta.PCRPError:
-                continue
-            nb = pcrpenta.uneven_workspace_size(off, nn)
-            ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y0.device)
-            ys_t.copy_(y0)
-            pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            ys_t.copy_(y0)
-            e0.record(stream)
-            pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            t = e0.elapsed_time(e1)
-            if best is None or t < best[0]:
-                best = (t, kf, sh, lds_max)
-            del ws
+        try:
+            pcrpenta.split_tune_set(lds_max, kf, sh)
+        except pcrpenta.PCRPError:
+            continue
+        nb = pcrpenta.uneven_workspace_size(off, nn)
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=y0.device)
+        ys_t.copy_(y0)
+        pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ys_t.copy_(y0)
+        e0.record(stream)
+        pcrpenta.solve_uneven(a, b, c, d, e, ys_t, off, nn, stream=stream, ws=ws)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1)
+        if best is None or t < best[0]:
+            best = (t, kf, sh, lds_max)
+        del ws
     pcrpenta.split_tune_set(best[3], best[1], best[2])
     return {"lds_max": best[3], "k_fuse": best[1], "shape": best[2], "ms": best[0]}
 
@@ -569,15 +594,16 @@ def bench_single(args):
     torch.cuda.set_device(0)
     if args.tune_load:
         pcrpenta.tune_load(args.tune_load)
-    if n == -1:
-        r = run_heston(prec, K, W)
+    if n < 0:
+        r = run_heston(prec, K, W, reuse=n == -2)
         p = r["p"]
         out = {"metric": METRIC, "value": r["value"], "unit": "unknowns/s", "n_gpus": 1,
                "steps": K, "warmup": W, "ms_per_step": r["ms_per_run"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": prec,
                "data": "synthetic (Heston European call, Ns = 1024, Nv = 512, 200 steps of 1/200)",
                "config": {"workload": args.workload, "ns": p.ns, "nv": p.nv,
-                          "time_steps": p.steps, "graph": "one hipGraph per time step",
+                          "time_steps": p.steps, "graph": "25 Douglas steps per hipGraph",
+                          "factor_reuse": n == -2,
                           "unknowns_per_run": r["unknowns"]}}
         print(json.dumps(out), flush=True)
         return
@@ -585,6 +611,8 @@ def bench_single(args):
         B = uneven_batch(kind)
         if not args.no_tune and prec == "f64":
             log({"split_autotune": tune_uneven_split(B)})
+        elif not args.no_tune:
+            log({"f32r_autotune": tune_uneven_f32r(B)})
         r = run_uneven(B, prec, K, W)
         unknowns = B["total"]
         cfg = {"workload": args.workload, "m": m, "unknowns": unknowns,

@@ -381,7 +381,7 @@ int pcrp_split_autotune(int precision, int n, int m, int layout,
                         double *y_work, void *ws, size_t ws_bytes, int reps, void *stream,
                         int *best_k, int *best_shape, float *best_ms);
 /* FP32 refinement tunables (no counterpart in the reference): win_levels
- * (2..7, default 5) = reduction levels a window of a long system runs, with
+ * (2..7, default 4) = reduction levels a window of a long system runs, with
  * 2^win_levels - 1 halo block rows per side (more levels: fewer batches fall
  * back to the split path, more redundant halo work); window_min_unknowns
  * (default 2^23) = batch size in unknowns from which systems of n in

@@ -712,7 +712,7 @@ int window_plan_execute(const WindowPlan& P, const double* a, const double* b, c
 
 }  // namespace
 
-static F32RTune g_f32r = {5, 1LL << 23};
+static F32RTune g_f32r = {4, 1LL << 23};
 
 F32RTune f32r_tune_get() { return g_f32r; }
 
