@@ -93,13 +93,66 @@ def time_point(m: int, n: int, runs: int = AVG_RUN):
     return tuple(out)
 
 
+def read_dat(path):
+    """{(m, n): (t1, t2, t3)} from a file in the row format above; header
+    lines and the reference's zero rows (points it did not run) are skipped."""
+    rows = {}
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) != 5:
+                continue
+            try:
+                m, n = int(parts[0]), int(parts[1])
+                t = tuple(float(v) for v in parts[2:])
+            except ValueError:
+                continue
+            if all(v > 0 for v in t):
+                rows[(m, n)] = t
+    return rows
+
+
+def compare(ours_path, ref_path, cols=(26, 260, 1040, 1248, 2080)):
+    """Markdown table: the reference's fastest column (Splitting 1, its t1)
+    against this build's t1 (same input, same system-contiguous layout) and
+    t2 (interleaved), as speed-ups, at the batch sizes `cols`; then the
+    median / min / max speed-up over every point both files hold."""
+    import statistics
+    ours, ref = read_dat(ours_path), read_dat(ref_path)
+    common = sorted(set(ours) & set(ref))
+    out = ["| n \\ m | " + " | ".join(f"m={m}" for m in cols) + " |",
+           "|---|" + "---|" * len(cols)]
+    for n in SIZES:
+        cells = []
+        for m in cols:
+            if (m, n) in ours and (m, n) in ref:
+                r, o = ref[(m, n)][0], ours[(m, n)]
+                cells.append(f"{r:.3f} / {o[0]:.3f} ms = {r / o[0]:.0f}x")
+            elif (m, n) in ours:
+                cells.append(f"- / {ours[(m, n)][0]:.3f} ms")
+            else:
+                cells.append("-")
+        out.append(f"| {n} | " + " | ".join(cells) + " |")
+    for k, label in ((0, "t1 (contiguous)"), (1, "t2 (interleaved)"), (2, "t3 (FP32+refine)")):
+        sp = [ref[p][0] / ours[p][k] for p in common]
+        out.append(f"\nSpeed-up of {label} over the reference's Splitting 1 at the {len(sp)} "
+                   f"common points: median {statistics.median(sp):.1f}x, min {min(sp):.1f}x, "
+                   f"max {max(sp):.1f}x.")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--compare", nargs=2, metavar=("OURS", "REFERENCE"), default=None,
+                    help="print a speed-up table of two .dat files and exit")
     ap.add_argument("--out", default=OUT_NAME)
     ap.add_argument("--runs", type=int, default=AVG_RUN)
     ap.add_argument("--batches", type=int, nargs="*", default=None)
     ap.add_argument("--sizes", type=int, nargs="*", default=None)
     args = ap.parse_args(argv)
+    if args.compare:
+        print(compare(*args.compare))
+        return 0
     with open(args.out, "w"):
         pass
     for m, n in grid(args.batches, args.sizes):
