@@ -296,26 +296,30 @@ def tune_uneven_f32r(B):
     return {"win_levels": best[1], "ms": best[0]}
 
 
-def plan_execute_ms(B, prec, K):
+def plan_execute_ms(B, prec, K, win_levels=None):
     """Mean event-timed execute of a plan (pcrp_plan_create once: routing
     tables, window levels; executes launch kernels only) over K pristine
-    right-hand sides."""
+    right-hand sides. win_levels: FP32 window level count for this plan
+    (2 makes the config generators' batches non-local, i.e. routes them to the
+    pass/stage path that systems without decaying couplings take)."""
     import torch
     import pcrpenta
     a, b, c, d, e, y0 = B["arrs"]
     pid = pcrpenta.PREC_F64 if prec == "f64" else pcrpenta.PREC_F32_REFINE
+    keep = pcrpenta.f32r_tune_get()
+    if win_levels is not None:
+        pcrpenta.f32r_tune_set(win_levels, keep[1])
     if "off" in B:
         plan = pcrpenta.Plan(a, b, c, d, e, precision=pid, offsets=B["off"], sizes=B["nn"])
     else:
         plan = pcrpenta.Plan(a, b, c, d, e, B["n"], B["m"], B["layout"], precision=pid)
+    pcrpenta.f32r_tune_set(*keep)   # the plan keeps what it was created with
     ys = [y0.clone() for _ in range(2)]
-
-    def step(k):
-        plan.execute(ys[k % 2], max_sweeps=MAX_SWEEPS, tol=TOL)
-        ys[k % 2].copy_(y0)   # (not timed: events bracket the execute only)
-
+    m = len(B["nn"]) if "off" in B else B["m"]
+    sweeps = torch.zeros(m, dtype=torch.int32, device=y0.device)
     stream = torch.cuda.current_stream()
-    step(0)
+    plan.execute(ys[0], max_sweeps=MAX_SWEEPS, tol=TOL, sweeps=sweeps)
+    ys[0].copy_(y0)
     torch.cuda.synchronize()
     tot = 0.0
     for k in range(K):
@@ -328,7 +332,10 @@ def plan_execute_ms(B, prec, K):
         tot += e0.elapsed_time(e1)
     route = plan.route
     plan.destroy()
-    return {"ms": tot / K, "route": ["lds", "split", "window", "uneven-queue", "uneven-stage"][route]}
+    out = {"ms": tot / K, "route": ["lds", "split", "window", "uneven-queue", "uneven-stage"][route]}
+    if prec != "f64":
+        out["sweeps_hist"] = torch.bincount(sweeps.long()).tolist()
+    return out
 
 
 def run_uneven(B, prec, K, W):
@@ -384,6 +391,7 @@ def bench_headline(args):
     f64 = run_uneven(B, "f64", K, W)
     f32 = run_uneven(B, "f32r", K, W)
     plan32 = plan_execute_ms(B, "f32r", 5)
+    stage32 = plan_execute_ms(B, "f32r", 3, win_levels=2)
     ratio = f32["value"] / f64["value"]
     out = {
         "metric": METRIC, "value": f64["value"], "unit": "unknowns/s", "n_gpus": 1,
@@ -411,6 +419,10 @@ def bench_headline(args):
                              "unknowns_per_s": total / (plan32["ms"] / 1e3),
                              "note": "repeat solves of the same matrices: pcrp_plan_create "
                                      "once, then capturable executes"},
+            "stage_path": {**stage32,
+                           "ratio_vs_f64": f64["ms_per_step"] / stage32["ms"],
+                           "note": "the pass/stage path that batches whose couplings do not "
+                                   "decay take (forced here with a 2-level window plan)"},
         },
     }
     if not args.no_cpu:
@@ -555,6 +567,7 @@ def also_workloads(args):
         r64 = run_uniform(n, m, "f64", kind, K, W)
         r32 = run_uniform(n, m, "f32r", kind, K, W)
         p32 = plan_execute_ms(r32["batch"], "f32r", 5)
+        s32 = plan_execute_ms(r32["batch"], "f32r", 3, win_levels=2) if n > 1024 else None
         out[label] = {
             "n": n, "m": m, "generator": "dominant" if kind == 0 else "weak",
             "layout": "interleaved",
@@ -566,6 +579,7 @@ def also_workloads(args):
             "f32r_sweeps_hist": r32["sweeps_hist"],
             "f32r_plan_execute_ms": p32["ms"], "f32r_plan_route": p32["route"],
             "ratio_f32r_plan_vs_f64": r64["ms_per_step"] / p32["ms"],
+            "f32r_stage_path": s32 and {**s32, "ratio_vs_f64": r64["ms_per_step"] / s32["ms"]},
             "f64_tuned": (r64["tuned"] or {}).get("variant", (r64["tuned"] or {}).get("split")),
         }
         del r64, r32
