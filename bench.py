@@ -554,9 +554,8 @@ def also_workloads(args):
             "f64_ms_per_run": h64r["ms_per_run"], "f32r_ms_per_run": h32r["ms_per_run"],
             "ratio_f32r_reuse_vs_plain_f64": h64["ms_per_run"] / h32r["ms_per_run"],
             "ratio_f32r_reuse_vs_f64_reuse": h64r["ms_per_run"] / h32r["ms_per_run"],
-            "note": "line matrices factored once per run: FP64 every S-line and the shared "
-                    "v-line matrix, FP32+refine the shared v-line matrix (bit-identical "
-                    "results to the full solves)"},
+            "note": "the shared v-line matrix factored once per run, in both precisions "
+                    "(bit-identical results to the full solves)"},
         "f32r_vs_f64_grid_relerr": float(np.abs(U32 - U64).max() / np.abs(U64).max()),
         "price_at_S100_v0.04_f64": float(_interp_price(U64, p, hS, hv)),
         "refine": {"tol": 1e-12, "max_sweeps": 8},

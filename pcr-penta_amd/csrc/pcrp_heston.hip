@@ -26,11 +26,13 @@
 // S-line solve, which stores Y1 + W (the v-line right-hand side) straight
 // into U, and the v-line solve in place. The step index lives in
 // device memory, so one captured hipGraph replays every step.
-// With reuse_factor the line matrices (constant over the run) are factored
-// once before the first step: FP64 keeps the PCR-Penta multipliers of every
-// S-line and the single v-line matrix (all v-lines share one matrix, since
-// A2 depends on v only), FP32+refine the 2x2-block factor of the v-line
-// matrix; each step then solves right-hand sides only for those lines.
+// With reuse_factor the v-line matrix (constant over the run, and one matrix
+// for every v-line, since A2 depends on v only) is factored once before the
+// first step: FP64 keeps its PCR-Penta multipliers, FP32+refine its 2x2-block
+// factor; each step then solves right-hand sides only for the v-lines. The
+// S-lines keep the full solve: their matrices differ per line, and reading a
+// per-line factor (43 doubles per row at n = 1024, 180 MB per S-line solve)
+// took longer than recomputing it (FP64: 36.2 vs 34.9 us per S-line solve).
 #include "pcr_penta.h"
 #include "pcrp_internal.h"
 
@@ -268,7 +270,6 @@ struct HestonWs {
     int* step;
     void* solver_ws;
     size_t solver_bytes;
-    void* fS;   // reuse_factor, FP64: factors of the S-lines
     void* fV;   // reuse_factor: the shared factor of the v-line matrix
 };
 
@@ -281,13 +282,7 @@ size_t solver_ws_bytes(const pcrp_heston_params* p, int precision)
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
-// factor buffers of the reuse_factor form (0 when not used)
-size_t factor_s_bytes(const pcrp_heston_params* p, int precision)
-{
-    if (!p->reuse_factor || precision != PCRP_PREC_F64) return 0;
-    return align256(pcrp_factor_size(p->ns, p->nv, PCRP_PREC_F64, 0));
-}
-
+// factor buffer of the reuse_factor form (0 when not used)
 size_t factor_v_bytes(const pcrp_heston_params* p, int precision)
 {
     if (!p->reuse_factor) return 0;
@@ -297,16 +292,14 @@ size_t factor_v_bytes(const pcrp_heston_params* p, int precision)
 // the reuse_factor form needs line sizes the factored solves take
 bool reuse_supported(const pcrp_heston_params* p, int precision)
 {
-    if (!p->reuse_factor) return true;
-    if (factor_v_bytes(p, precision) == 0) return false;
-    return precision != PCRP_PREC_F64 || factor_s_bytes(p, precision) != 0;
+    return !p->reuse_factor || factor_v_bytes(p, precision) != 0;
 }
 
 size_t heston_ws_bytes(const pcrp_heston_params* p, int precision)
 {
     const size_t N = (size_t)p->ns * p->nv;
     return align256(12 * N * sizeof(double) + 256) + solver_ws_bytes(p, precision) +
-           factor_s_bytes(p, precision) + factor_v_bytes(p, precision);
+           factor_v_bytes(p, precision);
 }
 
 HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
@@ -322,18 +315,13 @@ HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
     h.solver_ws = c;
     h.solver_bytes = solver_ws_bytes(p, precision);
     c += h.solver_bytes;
-    h.fS = factor_s_bytes(p, precision) ? c : nullptr;
-    c += factor_s_bytes(p, precision);
     h.fV = factor_v_bytes(p, precision) ? c : nullptr;
     return h;
 }
 
-// Factors of the reuse_factor form, once, before the first step.
+// The factor of the reuse_factor form, once, before the first step.
 int factor_lines(const Grid& g, int precision, const HestonWs& h, hipStream_t st)
 {
-    if (h.fS && factor_f64(h.sa, h.sb, h.sc, h.sd, h.se, g.ns, g.nv, 1, g.ns,
-                           static_cast<double*>(h.fS), st) != 0)
-        return PCRP_ERR_HIP;
     if (!h.fV) return PCRP_OK;
     // v-line 0's rows (element j at j*ns) are the matrix of every v-line
     if (precision == PCRP_PREC_F64)
@@ -354,10 +342,6 @@ int factor_lines(const Grid& g, int precision, const HestonWs& h, hipStream_t st
 int solve_s_lines(const Grid& g, int precision, double* U, const HestonWs& h, hipStream_t st)
 {
     const int n = g.ns, m = g.nv;
-    if (precision == PCRP_PREC_F64 && h.fS) {
-        FactoredF64 A{static_cast<const double*>(h.fS), false, h.R1, U, h.W, n, m, 1, n};
-        return solve_factored_f64(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
-    }
     if (precision == PCRP_PREC_F64) {
         if (split_f64_levels(n, split_f64_get().lds_max) != 0) return PCRP_ERR_UNSUPPORTED;
         SolveArgsF64 A{h.sa, h.sb, h.sc, h.sd, h.se, h.R1, n, m, 1, n, pcr_steps(n),

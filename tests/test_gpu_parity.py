@@ -817,10 +817,9 @@ def test_heston_graph_replay_equals_eager(gpu):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_heston_factor_reuse_equals_full_solves(gpu, prec):
-    """The Heston grid, all 200 steps: factoring the line matrices once (FP64:
-    every S-line and the shared v-line matrix; FP32+refine: the shared v-line
-    matrix) gives the same price grid bit for bit as solving every line in
-    full each step (the factored solves run the same operations)."""
+    """The Heston grid, all 200 steps: factoring the shared v-line matrix once
+    gives the same price grid bit for bit as solving every line in full each
+    step (the factored solves run the same operations)."""
     _say("heston-reuse", prec)
     U0, _ = _heston_gpu(gpu, prec, True, reuse_factor=0)
     U1, _ = _heston_gpu(gpu, prec, True, reuse_factor=1)
