@@ -44,6 +44,45 @@ def test_generator_matches_numpy_restatement(oracle_lib):
                     assert np.array_equal(u, v)
 
 
+@pytest.mark.parametrize("kind,off,clo,chi", [(0, 1.0, 4.5, 5.5), (1, 0.5, 2.2, 2.6)])
+def test_generator_follows_config_distributions(oracle_lib, kind, off, clo, chi):
+    """The seeded inputs against the configs' stated distributions, not
+    against a copy of the generator's formula: off-diagonals U[-off, off]
+    with out-of-range entries exactly zero, main diagonal U[clo, chi], RHS
+    U[-1, 1] (Kolmogorov-Smirnov against each uniform, mean and variance),
+    every row strictly diagonally dominant, systems and diagonals drawn
+    independently, the same seed reproducing the same batch."""
+    from scipy import stats
+    n, m = 4096, 8
+    arrs = oracle_lib.gen_batch(kind, 7, n, m, oracle_lib.INTERLEAVED)
+    sys_ = [oracle_lib.system(arrs, j, n, m, oracle_lib.INTERLEAVED) for j in range(m)]
+    for a, b, c, d, e, y in sys_:
+        assert a[0] == 0.0 and a[1] == 0.0 and b[0] == 0.0
+        assert d[n - 1] == 0.0 and e[n - 2] == 0.0 and e[n - 1] == 0.0
+        assert np.all(np.abs(b) + np.abs(d) + np.abs(a) + np.abs(e) < c)
+    pools = {
+        "a": (np.concatenate([s[0][2:] for s in sys_]), -off, off),
+        "b": (np.concatenate([s[1][1:] for s in sys_]), -off, off),
+        "c": (np.concatenate([s[2] for s in sys_]), clo, chi),
+        "d": (np.concatenate([s[3][:-1] for s in sys_]), -off, off),
+        "e": (np.concatenate([s[4][:-2] for s in sys_]), -off, off),
+        "y": (np.concatenate([s[5] for s in sys_]), -1.0, 1.0),
+    }
+    for name, (v, lo, hi) in pools.items():
+        assert v.min() >= lo and v.max() <= hi, name
+        assert stats.kstest(v, "uniform", args=(lo, hi - lo)).pvalue > 1e-4, name
+        N = len(v)
+        assert abs(v.mean() - (lo + hi) / 2) < 5 * (hi - lo) / np.sqrt(12 * N), name
+        assert abs(v.var() / ((hi - lo) ** 2 / 12) - 1) < 0.05, name
+    # independent systems and diagonals: correlations near zero
+    assert abs(np.corrcoef(sys_[0][5], sys_[1][5])[0, 1]) < 0.08
+    assert abs(np.corrcoef(sys_[0][1][1:], sys_[0][3][1:])[0, 1]) < 0.08
+    again = oracle_lib.gen_batch(kind, 7, n, m, oracle_lib.INTERLEAVED)
+    assert all(np.array_equal(u, v) for u, v in zip(arrs, again))
+    other = oracle_lib.gen_batch(kind, 8, n, m, oracle_lib.INTERLEAVED)
+    assert not np.array_equal(arrs[5], other[5])
+
+
 def test_pcr80_matches_dgbsv(oracle_lib, golden):
     """The same oracle source in 80-bit arithmetic agrees with LAPACK dgbsv to
     ~1e-16: the recurrences (terms, signs, indices, guards, step count, final
