@@ -16,10 +16,13 @@
  * (PCRP_OK = 0) instead of exiting the process (the reference's checkCuda
  * macro, performanalysis.h:12-20, exits). Solves are asynchronous on
  * `stream` (a hipStream_t; NULL means the null stream) with these
- * exceptions: pcrp_solve_f32r_f64 on the window path (long systems, or
- * large batches of n in (1024, 2048]) synchronises its stream once, after
- * the first sweep, and the uneven entry points copy their plan tables from
- * host memory on every call; neither can be captured in a hipGraph. The plan
+ * exceptions: pcrp_solve_f32r_f64 and pcrp_solve_uneven_f32r_f64 on the
+ * window path (long systems, uneven batches, or large batches of n in
+ * (1024, 2048]) synchronise their stream once, after the first sweep, to
+ * read a locality flag (pcrp_solve_f32r_f64 under stream capture takes the
+ * split path instead, with no read-back); the uneven entry points copy
+ * their plan tables from host memory on every call, so they cannot be
+ * captured. The plan
  * API (pcrp_plan_create / pcrp_plan_execute) does that work once at plan
  * time and executes without any synchronisation or host copy (capturable).
  * The autotuners, pcrp_plan_create and pcrp_heston_solve with use_graph = 1
