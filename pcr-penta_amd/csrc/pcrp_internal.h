@@ -189,7 +189,7 @@ size_t factor_f32r_bytes(int n);
 // Plan form of the window path (pcrp_window_f32r.hip): the matrix-only
 // preparation once (synchronous), then capturable executes.
 struct WindowLayoutOffsets {
-    size_t state, stop, flag, sys, lev, x0, x1, y0, six, bytes;
+    size_t state, stop, flag, sys, lev, wmap, x0, x1, y0, six, bytes;
 };
 struct WindowPlan {
     WindowLayoutOffsets L;

@@ -91,6 +91,7 @@ struct WArgs {
     int nwu;   // uniform batches: windows per system
     int vec;   // rs == 1 and every array 16-byte aligned
     int wl, wh;   // levels a window of a longer system runs, its halo block rows per side
+    const int* wmap = nullptr;   // uneven batches: the system of every window
 };
 
 // Interleaved uniform batches (more than one system) are transposed first:
@@ -111,7 +112,9 @@ __host__ __device__ inline int windows_for(int nb, int T)
 // faster than system-major order; systems of at most 8 windows go
 // system-major with runs of 8 on one XCD (neighbouring windows share halo
 // rows in that XCD's L2: 2.5% faster at n = 2048, m = 16384). Uneven batches
-// are system-major (a binary search on the first-window table).
+// are system-major: a table gives each window's system (a binary search over
+// the systems' first windows, 14 dependent loads at 10000 systems, cost a
+// window its first microseconds; the copy-out kernel spent 0.33 ms on it).
 __device__ __forceinline__ void window_of(const WArgs& A, int w, int& j, int& wi)
 {
     if (A.uniform) {
@@ -127,13 +130,20 @@ __device__ __forceinline__ void window_of(const WArgs& A, int w, int& j, int& wi
         }
         return;
     }
-    int lo = 0, hi = A.m - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (A.sys[mid].w0 <= w) lo = mid; else hi = mid - 1;
+    j = A.wmap[w];
+    wi = (int)(w - A.sys[j].w0);
+}
+
+// The window -> system table of an uneven batch: system j's windows are
+// w0 .. w0 + nw - 1.
+__global__ __launch_bounds__(256) void wmap_kernel(const WSys* __restrict__ sys, int m,
+                                                   int* __restrict__ wmap)
+{
+    for (int j = blockIdx.x; j < m; j += gridDim.x) {
+        const long long w0 = sys[j].w0;
+        const int nw = sys[j].nw;
+        for (int k = threadIdx.x; k < nw; k += blockDim.x) wmap[w0 + k] = j;
     }
-    j = lo;
-    wi = (int)(w - A.sys[lo].w0);
 }
 
 __device__ __forceinline__ double2 ld2(const double* p)
@@ -444,6 +454,7 @@ WLayout wlayout(long long elems, int m, bool transposed)
     L.flag = o;  o = align_up(o + sizeof(int));
     L.sys = o;   o = align_up(o + (size_t)m * sizeof(WSys));
     L.lev = o;   o = align_up(o + lev_capacity(elems, m));
+    L.wmap = o;  o = align_up(o + lev_capacity(elems, m) * sizeof(int));
     L.x0 = o;    o = align_up(o + (size_t)elems * sizeof(double));
     L.x1 = o;    o = align_up(o + (size_t)elems * sizeof(double));
     L.y0 = o;    if (!transposed) o = align_up(o + (size_t)elems * sizeof(double));
@@ -522,6 +533,12 @@ int window_run(const double* a, const double* b, const double* c, const double* 
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
                 al16(y) && al16(x0) && al16(x1) && al16(y0) && al16(six) ? 1 : 0;
+    }
+    if (!uniform) {
+        int* wmap = reinterpret_cast<int*>(w + L.wmap);
+        hipLaunchKernelGGL(wmap_kernel, dim3((unsigned)(m < 4096 ? m : 4096)), dim3(256), 0, st,
+                           dsys, m, wmap);
+        A.wmap = wmap;
     }
     // Direct form: sweep it writes x0, x1, y for it % 3 = 0, 1, 2; the first
     // sweep reads y and copies it to y0, later sweeps read y0, so the usual
@@ -624,6 +641,12 @@ int window_plan(const double* a, const double* b, const double* c, const double*
             transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
             nullptr, rs, dsys, state, stop, flag, lev, m, (int)nwin, uniform ? 1 : 0,
             uniform ? sys[0].nw : 0, 0, wl, win_halo(wl)};
+    if (!uniform) {
+        int* wmap = reinterpret_cast<int*>(w + L.wmap);
+        hipLaunchKernelGGL(wmap_kernel, dim3((unsigned)(m < 4096 ? m : 4096)), dim3(256), 0, st,
+                           dsys, m, wmap);
+        A.wmap = wmap;
+    }
     // level-finding sweep: its right-hand side is any finite array of the
     // batch's shape (the level counts and the flag depend on the matrix only)
     A.rhs = A.c;
@@ -682,6 +705,7 @@ int window_plan_execute(const WindowPlan& P, const double* a, const double* b, c
             transposed ? six + 3 * N : d, transposed ? six + 4 * N : e, nullptr, nullptr, nullptr,
             nullptr, P.rs, dsys, state, stop, flag, lev, m, (int)P.nwin, P.uniform ? 1 : 0,
             P.nwu, 0, P.wl, win_halo(P.wl)};
+    if (!P.uniform) A.wmap = reinterpret_cast<const int*>(w + L.wmap);
     {
         const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
         A.vec = P.rs == 1 && al16(A.a) && al16(A.b) && al16(A.c) && al16(A.d) && al16(A.e) &&
