@@ -175,21 +175,17 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
 // system the residue-interleaved index of row l of sub-system q (q < 2^Sin)
 // is q + l*2^Sin, i.e. the row's own index, so each pass reads and writes
 // the same element positions (in the caller's arrays or a same-shaped
-// workspace). tab lists the systems that still need passes in this round.
+// workspace). tab lists the systems that still need passes in this round;
+// bmap gives every block its entry of tab (a binary search over the entries,
+// 13 dependent loads at 6000 systems, delayed every block's first loads).
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     const double* const* __restrict__ sets, int src, int dst, const PassSys* __restrict__ tab,
-    int ntab, int Sin)
+    const int* __restrict__ bmap, int Sin)
 {
     __shared__ double sR[6][TB * G];
-    // locate this block's system: last entry with block0 <= blockIdx.x
     const long long b = blockIdx.x;
-    int lo = 0, hi = ntab - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab[mid].block0 <= b) lo = mid; else hi = mid - 1;
-    }
-    const PassSys P = tab[lo];
+    const PassSys P = tab[bmap[b]];
     const int local = (int)(b - P.block0);
     int grp, tile;
     if (!tile_of_block(local, ((1 << Sin) + G - 1) / G, P.tiles, grp, tile)) return;
@@ -230,20 +226,20 @@ static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, in
 
 template <int TB, int G, int RPT>
 static int launch_pass_uneven_t(const double* const* sets, int src, int dst, const PassSys* tab,
-                                int ntab, long long blocks, int Sin, hipStream_t st)
+                                const int* bmap, long long blocks, int Sin, hipStream_t st)
 {
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffffLL) return -2;
     hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT),
-                       0, st, sets, src, dst, tab, ntab, Sin);
+                       0, st, sets, src, dst, tab, bmap, Sin);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 using PassFn = int (*)(const double* const*, double* const*, int, int, int, int, long long,
                        long long, int, hipStream_t);
 
-using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, int, long long, int,
-                             hipStream_t);
+using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, const int*, long long,
+                             int, hipStream_t);
 
 struct PassShape {
     int tb, g, rpt;
@@ -283,9 +279,9 @@ int pass_shape_single(int shape)
 }
 
 int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
-                       int ntab, long long blocks, int Sin, hipStream_t st)
+                       const int* bmap, long long blocks, int Sin, hipStream_t st)
 {
-    return kPassShapes[shape].uneven(sets, src, dst, tab, ntab, blocks, Sin, st);
+    return kPassShapes[shape].uneven(sets, src, dst, tab, bmap, blocks, Sin, st);
 }
 
 // Tunables of the split path: LDS/HBM threshold (sub-systems at most this

@@ -28,6 +28,17 @@ namespace pcrp {
 
 namespace {
 
+// The block -> entry table of a pass round over `tab` (entry i owns blocks
+// tab[i].block0 .. tab[i+1].block0 - 1, the last one up to `blocks`).
+__global__ void pass_block_map_kernel(const PassSys* __restrict__ tab, int ntab, long long blocks,
+                                      int* __restrict__ bmap)
+{
+    for (int i = blockIdx.x; i < ntab; i += gridDim.x) {
+        const long long b0 = tab[i].block0, b1 = i + 1 < ntab ? tab[i + 1].block0 : blocks;
+        for (long long b = b0 + threadIdx.x; b < b1; b += blockDim.x) bmap[b] = i;
+    }
+}
+
 __global__ void reset_counters_kernel(unsigned int* c)
 {
     if (threadIdx.x < kQueues) c[threadIdx.x] = 0u;
@@ -47,7 +58,7 @@ struct UnevenPlan {
     bool any_split = false;
     // workspace byte offsets
     size_t off_counter = 0, off_sets = 0, off_jobs = 0, off_rounds = 0, off_arrays = 0, bytes = 0;
-    std::vector<size_t> round_off;
+    std::vector<size_t> round_off, round_map_off;
 };
 
 int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
@@ -140,6 +151,10 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
         P.round_off.push_back(o);
         o = align_up(o + t.size() * sizeof(PassSys));
     }
+    for (long long blocks : P.round_blocks) {   // written on the device, after the upload
+        P.round_map_off.push_back(o);
+        o = align_up(o + (size_t)blocks * sizeof(int));
+    }
     P.off_arrays = o;
     if (P.any_split) o += 2 * 6 * (size_t)P.total * sizeof(double);
     P.bytes = o;
@@ -177,7 +192,8 @@ int upload_plan(const UnevenPlan& P, const double* a, const double* b, const dou
         sets[6 + v] = P.any_split
                           ? reinterpret_cast<double*>(w + P.off_arrays) + (size_t)v * P.total
                           : nullptr;
-    host.assign(P.off_arrays, 0);
+    const size_t up = P.round_map_off.empty() ? P.off_arrays : P.round_map_off[0];
+    host.assign(up, 0);
     std::copy(reinterpret_cast<const char*>(P.qstart),
               reinterpret_cast<const char*>(P.qstart) + sizeof(P.qstart),
               host.begin() + P.off_counter + kQueues * sizeof(unsigned int));
@@ -189,9 +205,15 @@ int upload_plan(const UnevenPlan& P, const double* a, const double* b, const dou
         std::copy(reinterpret_cast<const char*>(P.rounds[r].data()),
                   reinterpret_cast<const char*>(P.rounds[r].data() + P.rounds[r].size()),
                   host.begin() + P.round_off[r]);
-    return hipMemcpyAsync(w, host.data(), host.size(), hipMemcpyHostToDevice, st) == hipSuccess
-               ? PCRP_OK
-               : PCRP_ERR_HIP;
+    if (hipMemcpyAsync(w, host.data(), host.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+        return PCRP_ERR_HIP;
+    for (size_t r = 0; r < P.rounds.size(); ++r) {
+        const int ntab = (int)P.rounds[r].size();
+        hipLaunchKernelGGL(pass_block_map_kernel, dim3((unsigned)(ntab < 4096 ? ntab : 4096)),
+                           dim3(256), 0, st, reinterpret_cast<const PassSys*>(w + P.round_off[r]),
+                           ntab, P.round_blocks[r], reinterpret_cast<int*>(w + P.round_map_off[r]));
+    }
+    return hipGetLastError() == hipSuccess ? PCRP_OK : PCRP_ERR_HIP;
 }
 
 // The pass rounds and the persistent LDS launch over uploaded tables.
@@ -206,8 +228,8 @@ int launch_plan(const UnevenPlan& P, double* y, void* ws, hipStream_t st)
         const int dst = 1 + (int)(r & 1);
         rc = launch_pass_uneven(P.round_shape[r], dsets, src, dst,
                                 reinterpret_cast<const PassSys*>(w + P.round_off[r]),
-                                (int)P.rounds[r].size(), P.round_blocks[r],
-                                (int)r * split_f64_get().k_fuse, st);
+                                reinterpret_cast<const int*>(w + P.round_map_off[r]),
+                                P.round_blocks[r], (int)r * split_f64_get().k_fuse, st);
         if (rc != 0) return PCRP_ERR_HIP;
     }
     // one persistent LDS launch over every job
