@@ -147,7 +147,7 @@ size_t uneven_f32r_workspace(const int64_t* h_off, const int* h_n, int m);
 size_t window_f32r_workspace(long long elems, int m, bool transposed);
 size_t window_f32r_workspace_uniform(int n, int m, int layout);
 // FP32 refinement tunables: levels a window of a longer system runs (halo
-// 2^L - 1 block rows per side; 2..7, default 5) and the batch size (unknowns)
+// 2^L - 1 block rows per side; 2..7, default 4) and the batch size (unknowns)
 // from which n in (1024, 2048] takes the window path instead of the
 // one-workgroup kernel (default 2^23; below, the read-back and the
 // transposes cost more than the windows save).

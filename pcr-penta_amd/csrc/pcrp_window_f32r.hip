@@ -56,7 +56,7 @@ namespace {
 constexpr int kWinNT = 256, kWinBR = 2;
 constexpr int kWinNB = kWinNT * kWinBR;    // block rows per window (LDS records)
 // Levels L a window of a longer system may run (tunable, f32r_tune: default
-// 5): halo H = 2^L - 1 block rows per side (the reach of L levels), interior
+// 4): halo H = 2^L - 1 block rows per side (the reach of L levels), interior
 // T = kWinNB - 2H block rows. kWinLMax bounds L (workspace sizes use it).
 constexpr int kWinLMax = 7;
 constexpr int kWinTMin = kWinNB - 2 * ((1 << kWinLMax) - 1);

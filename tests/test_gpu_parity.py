@@ -700,13 +700,14 @@ def test_refined_split_under_graph_capture(gpu, oracle_lib):
         assert oracle_lib.normwise_relerr(x, x_eager, n, m, I) <= 1e-12, k
 
 
-@pytest.mark.parametrize("n", [2049, 2700, 2701, 2702, 3601, 3603, 4500])
+@pytest.mark.parametrize("n", [2049, 2700, 2701, 2702, 3601, 3603, 4500, 2892, 2893, 2894,
+                               3856, 3857])
 def test_refined_window_edges(gpu, oracle_lib, n):
-    """Long systems cut into windows (at most 450 interior block rows,
-    balanced over the system's windows; 31 halo block rows per side): sizes
-    with one block row past a window edge, exact multiples, odd n; both
-    generators and layouts (the interleaved batch is transposed first), m not
-    a multiple of 32."""
+    """Long systems cut into windows (interior block rows balanced over the
+    system's windows, at most 512 - 2 (2^L - 1): 482 at the default L = 4,
+    450 at L = 5): sizes with one block row past a window edge for either
+    count, exact multiples, odd n; both generators and layouts (the
+    interleaved batch is transposed first), m not a multiple of 32."""
     m = 5
     for kind, layout in ((0, I), (1, C)):
         _say("refined-window", n, m, kind, layout)
