@@ -3,13 +3,13 @@
 // Replaces the reference's host layer: the per-size dispatch of main.cu:46-87
 // (n <= 1024 -> pcr_penta, else split stage 1 + stage 2) and the checkCuda
 // error convention (performanalysis.h:12-20), which exits the process; here
-// every failure is a returned status and nothing synchronises the device.
+// every failure is a returned status.
 #include "pcr_penta.h"
-
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
 #include <algorithm>
+#include <atomic>
 
 #include <stdio.h>
 
@@ -100,9 +100,13 @@ uint64_t host_splitmix64(uint64_t z)
 
 
 // ---- autotuner state: per precision, per size bucket ceil(log2 n) -> variant
+// (atomic entries holding variant + 1, 0 = not tuned: solves may read the
+// table while another thread tunes)
 constexpr int kBuckets = 12;   // n in [1, 2048]
-int g_tuned[2][kBuckets] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
-                            {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+std::atomic<int> g_tuned[2][kBuckets];
+
+int tuned_get(int p, int b) { return g_tuned[p][b].load(std::memory_order_relaxed) - 1; }
+void tuned_set(int p, int b, int id) { g_tuned[p][b].store(id < 0 ? 0 : id + 1, std::memory_order_relaxed); }
 
 int bucket_of(int n)
 {
@@ -117,9 +121,10 @@ namespace pcrp {
 int lds_tuned_variant(int prec, int n)
 {
     const int b = bucket_of(n);
-    if (b < kBuckets && g_tuned[prec][b] >= 0) {
-        const LdsVariantF64* v = lds_f64_variant(g_tuned[prec][b]);
-        if (v && v->max_n >= n) return g_tuned[prec][b];
+    const int id = b < kBuckets ? tuned_get(prec, b) : -1;
+    if (id >= 0) {
+        const LdsVariantF64* v = lds_f64_variant(id);
+        if (v && v->max_n >= n) return id;
     }
     return lds_f64_default_variant(n);
 }
@@ -127,7 +132,7 @@ int lds_tuned_variant(int prec, int n)
 int lds_tuned_variant(int prec, int n, int m)
 {
     const int b = bucket_of(n);
-    if (b < kBuckets && g_tuned[prec][b] >= 0) return lds_tuned_variant(prec, n);
+    if (b < kBuckets && tuned_get(prec, b) >= 0) return lds_tuned_variant(prec, n);
     return lds_f64_default_variant_for_batch(n, m);
 }
 }  // namespace pcrp
@@ -239,7 +244,7 @@ int pcrp_tune_set(int precision, int n, int variant_id)
         const LdsVariantF64* v = lds_f64_variant(variant_id);
         if (!v || v->max_n < n) return PCRP_ERR_ARG;
     }
-    g_tuned[precision][b] = variant_id < 0 ? -1 : variant_id;
+    tuned_set(precision, b, variant_id);
     return PCRP_OK;
 }
 
@@ -291,7 +296,7 @@ int pcrp_autotune(int precision, int n, int m, int layout, const double* a, cons
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != PCRP_OK) return st;
-    g_tuned[precision][bucket_of(n)] = best;
+    tuned_set(precision, bucket_of(n), best);
     if (best_id) *best_id = best;
     if (best_ms) *best_ms = bt;
     return PCRP_OK;
@@ -465,8 +470,9 @@ int pcrp_tune_save(const char* path)
     fprintf(f, "f32r %d %lld\n", fr.win_levels, fr.window_min_unknowns);
     for (int p = 0; p < 2; ++p)
         for (int b = 0; b < kBuckets; ++b) {
-            const LdsVariantF64* v = g_tuned[p][b] >= 0 ? lds_f64_variant(g_tuned[p][b]) : nullptr;
-            if (v) fprintf(f, "%d %d %d %d %d %d\n", p, b, g_tuned[p][b], v->kind, v->nt, v->rpt);
+            const int id = tuned_get(p, b);
+            const LdsVariantF64* v = id >= 0 ? lds_f64_variant(id) : nullptr;
+            if (v) fprintf(f, "%d %d %d %d %d %d\n", p, b, id, v->kind, v->nt, v->rpt);
         }
     fclose(f);
     return PCRP_OK;
@@ -500,7 +506,7 @@ int pcrp_tune_load(const char* path)
             st = PCRP_ERR_ARG;   // table from another build: ignore that entry
             continue;
         }
-        g_tuned[p][b] = id;
+        tuned_set(p, b, id);
     }
     fclose(f);
     return st;

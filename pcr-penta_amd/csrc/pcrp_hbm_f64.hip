@@ -29,6 +29,8 @@
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 
+#include <mutex>
+
 namespace pcrp {
 
 // One tile of one HBM pass: rows l = tile*T - H + lr of current sub-system q
@@ -286,9 +288,15 @@ int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, c
 
 // Tunables of the split path: LDS/HBM threshold (sub-systems at most this
 // long go to the LDS kernel), steps fused per HBM pass, pass tile shape.
+// (a mutex: solves read them while another thread may tune)
 static SplitTune g_split_f64 = {1024, 4, 0};
+static std::mutex g_split_mu;
 
-SplitTune split_f64_get() { return g_split_f64; }
+SplitTune split_f64_get()
+{
+    std::lock_guard<std::mutex> lk(g_split_mu);
+    return g_split_f64;
+}
 
 int split_f64_set(const SplitTune& t)
 {
@@ -298,6 +306,7 @@ int split_f64_set(const SplitTune& t)
     if (t.k_fuse < 1 || t.k_fuse > 6 || t.shape < 0 || t.shape >= nshapes) return -1;
     const int H = 2 * ((1 << t.k_fuse) - 1);
     if (kPassShapes[t.shape].tb - 2 * H < 1) return -1;
+    std::lock_guard<std::mutex> lk(g_split_mu);
     g_split_f64 = t;
     return 0;
 }
@@ -313,7 +322,7 @@ int split_f64_levels(int n, int lds_max)
 
 size_t split_f64_workspace(int n, int m)
 {
-    const int S = split_f64_levels(n, g_split_f64.lds_max);
+    const int S = split_f64_levels(n, split_f64_get().lds_max);
     if (S == 0) return 0;
     // two ping-pong sets of six arrays of m * 2^S * ceil(n / 2^S) doubles
     const size_t per = (size_t)m * ((size_t)((n + (1 << S) - 1) >> S) << S);
@@ -323,7 +332,7 @@ size_t split_f64_workspace(int n, int m)
 int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes, hipStream_t st)
 {
     const int n0 = A.n, m0 = A.m;
-    const SplitTune tune = g_split_f64;
+    const SplitTune tune = split_f64_get();
     const int S = split_f64_levels(n0, tune.lds_max);
     if (S == 0) return -2;
     if (ws_bytes < split_f64_workspace(n0, m0) || ws == nullptr) return -3;

@@ -47,6 +47,7 @@
 #include "pcrp_internal.h"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace pcrp {
@@ -736,13 +737,20 @@ int window_plan_execute(const WindowPlan& P, const double* a, const double* b, c
 
 }  // namespace
 
+// (a mutex: solves read them while another thread may tune)
 static F32RTune g_f32r = {4, 1LL << 23};
+static std::mutex g_f32r_mu;
 
-F32RTune f32r_tune_get() { return g_f32r; }
+F32RTune f32r_tune_get()
+{
+    std::lock_guard<std::mutex> lk(g_f32r_mu);
+    return g_f32r;
+}
 
 int f32r_tune_set(const F32RTune& t)
 {
     if (t.win_levels < 2 || t.win_levels > kWinLMax || t.window_min_unknowns < 0) return -1;
+    std::lock_guard<std::mutex> lk(g_f32r_mu);
     g_f32r = t;
     return 0;
 }

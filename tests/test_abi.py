@@ -186,3 +186,55 @@ def test_factor_argument_checks(lib):
     assert f(F, *p, y, 100, 4, 1, 0, -1, 1e-12, None, None) == 6
     assert f(F, *p, None, 100, 4, 1, 0, 4, 1e-12, None, None) == 2
     assert f(F, *p, y, 100, 4, 1, 0, 4, float("nan"), None, None) == 6
+
+
+def test_tuning_tables_are_consistent_under_concurrent_use(lib):
+    """Writers alternate two complete settings of the split and FP32 tunables
+    while readers run: every read sees one setting or the other, never a mix
+    (the tables are read by every solve, possibly while another thread
+    tunes). No GPU: the tunables are host state."""
+    import ctypes
+    import threading
+    import pcrpenta
+    split0, f32r0 = pcrpenta.split_tune_get()[:3], pcrpenta.f32r_tune_get()
+    split_ok = {(1024, 4, 0), (512, 3, 2)}
+    f32r_ok = {(4, 1 << 23), (6, 12345)}
+    stop = threading.Event()
+    bad = []
+
+    def writer():
+        k = 0
+        while not stop.is_set():
+            lm, kf, sh = sorted(split_ok)[k % 2]
+            lib.pcrp_split_tune_set(0, lm, kf, sh)
+            wl, wm = sorted(f32r_ok)[k % 2]
+            lib.pcrp_f32r_tune_set(wl, wm)
+            k += 1
+
+    def reader():
+        vals = [ctypes.c_int() for _ in range(4)]
+        wl, wm = ctypes.c_int(), ctypes.c_longlong()
+        for _ in range(20000):
+            lib.pcrp_split_tune_get(0, *[ctypes.byref(v) for v in vals])
+            if tuple(v.value for v in vals[:3]) not in split_ok:
+                bad.append(tuple(v.value for v in vals[:3]))
+            lib.pcrp_f32r_tune_get(ctypes.byref(wl), ctypes.byref(wm))
+            if (wl.value, wm.value) not in f32r_ok:
+                bad.append((wl.value, wm.value))
+
+    try:
+        pcrpenta.split_tune_set(1024, 4, 0)
+        pcrpenta.f32r_tune_set(4, 1 << 23)
+        ws = [threading.Thread(target=writer) for _ in range(2)]
+        rs = [threading.Thread(target=reader) for _ in range(4)]
+        for t in ws + rs:
+            t.start()
+        for t in rs:
+            t.join()
+        stop.set()
+        for t in ws:
+            t.join()
+    finally:
+        pcrpenta.split_tune_set(*split0)
+        pcrpenta.f32r_tune_set(*f32r0)
+    assert not bad, bad[:5]
