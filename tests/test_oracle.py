@@ -103,7 +103,7 @@ def test_pcr64_within_rounding_of_dgbsv(oracle_lib, golden):
 
 
 def test_gbsv_oracle_matches_scipy(oracle_lib, golden):
-    """The C banded LU used on the GPU box as the FP64 reference solution."""
+    """The C banded LU that the GPU tests use as their FP64 reference solution."""
     for key, arrs, x in _cases(golden):
         assert _relerr(oracle_lib.gbsv(*arrs), x) < 1e-14, key
         assert _relerr(oracle_lib.gbsv(*arrs, extended=True), x) < 1e-14, key
