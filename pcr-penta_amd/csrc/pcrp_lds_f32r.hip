@@ -147,13 +147,17 @@ __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
         return v;
     };
     if constexpr (G > 1) {
-        // rows 0..n-1 of the six arrays, G neighbouring values at i*rs + j0
+        // rows 0..n-1 of the six arrays, G neighbouring values at i*rs + j0;
+        // consecutive lanes take consecutive LDS positions p (even rows, then
+        // odd rows): walking rows i instead put the even and the odd lanes of
+        // a wave on the same banks (2.1e5 conflict cycles per v-line launch)
         const long long j0 = (long long)grp * G;
         const double* const arr[6] = {ga, gb, gc, gd, ge, gy};
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
-            for (int i = (int)threadIdx.x; i < n; i += NT * G) {
-                const int p = lds_pos(i);
+            for (int p = (int)threadIdx.x; p < NMAX; p += NT * G) {
+                const int i = p < NB ? 2 * p : 2 * (p - NB) + 1;   // lds_pos(i) == p
+                if (i >= n) continue;
                 if (f < 5 && ass == 0) {   // one shared matrix row for the G systems
                     const double w = band(f, i, arr[f][(long long)i * ars]);
 #pragma unroll
