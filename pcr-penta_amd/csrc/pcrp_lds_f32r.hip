@@ -179,10 +179,45 @@ __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
     double X[BR][2];
     float v[BR][2];   // FP32 right-hand side of the next inner solve (first: y)
     M2 L[BR], D[BR], U[BR];
+    // contiguous rows (unit stride, 16-byte aligned): a block row's two rows
+    // are one 16-byte load per array
+    const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const bool vec2 = G == 1 && rs == 1 && ars == 1 && ((base | abase) & 1) == 0 && al16(ga) &&
+                      al16(gb) && al16(gc) && al16(gd) && al16(ge) && al16(gy);
 #pragma unroll
     for (int q = 0; q < BR; ++q) {
         const int k = t + NT * q;
         double rw[2][5];
+        if constexpr (G == 1) {
+            if (vec2 && 2 * k + 1 < n) {
+                const long long g = abase + 2LL * k;
+                const double2 av = *reinterpret_cast<const double2*>(ga + g);
+                const double2 bv = *reinterpret_cast<const double2*>(gb + g);
+                const double2 cv = *reinterpret_cast<const double2*>(gc + g);
+                const double2 dv = *reinterpret_cast<const double2*>(gd + g);
+                const double2 ev = *reinterpret_cast<const double2*>(ge + g);
+                const double2 yv = MODE == kFactor ? make_double2(0.0, 0.0)
+                                                   : *reinterpret_cast<const double2*>(gy + base + 2LL * k);
+                const double ra[2] = {av.x, av.y}, rb[2] = {bv.x, bv.y}, rc[2] = {cv.x, cv.y};
+                const double rd[2] = {dv.x, dv.y}, re[2] = {ev.x, ev.y}, ry[2] = {yv.x, yv.y};
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int i = 2 * k + h;
+                    const double a = band(0, i, ra[h]), b = band(1, i, rb[h]), c = rc[h];
+                    const double d = band(3, i, rd[h]), e = band(4, i, re[h]), y = ry[h];
+                    const int p = lds_pos(i);
+                    A[0][p] = a; A[1][p] = b; A[2][p] = c; A[3][p] = d; A[4][p] = e;
+                    A[5][p] = y;
+                    rw[h][0] = a; rw[h][1] = b; rw[h][2] = c; rw[h][3] = d; rw[h][4] = e;
+                    X[q][h] = 0.0;
+                    v[q][h] = (float)y;
+                }
+                L[q] = M2{(float)rw[0][0], (float)rw[0][1], 0.f, (float)rw[1][0]};
+                D[q] = M2{(float)rw[0][2], (float)rw[0][3], (float)rw[1][1], (float)rw[1][2]};
+                U[q] = M2{(float)rw[0][4], 0.f, (float)rw[1][3], (float)rw[1][4]};
+                continue;
+            }
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int i = 2 * k + h;
