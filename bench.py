@@ -133,6 +133,14 @@ def measured_traffic(workload):
         return json.load(f).get(workload, {}).get("bytes_per_step")
 
 
+def _traffic_entry(workload):
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(tpath):
+        return {}
+    with open(tpath) as f:
+        return json.load(f).get(workload, {})
+
+
 def measured_valu(workload):
     """VALU wave-instructions of one solve (SQ_INSTS_VALU summed over every
     solver kernel of one step, rocprofv3); None for workloads not profiled."""
@@ -148,6 +156,15 @@ def roofline(unknowns, solve_s, workload, note):
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
            "frac": achieved / HBM_PEAK, "traffic": measured_traffic(workload),
            "solve_ms": solve_s * 1e3, "note": note}
+    dom = _traffic_entry(workload).get("dominant_kernel")
+    if dom:
+        # the kernel moving the most HBM bytes of one solve, from its rocprofv3
+        # summary (profiles/): its average duration and share of the solve
+        out["dominant_kernel"] = {"name": dom["name"], "profiled_avg_ms": dom["avg_us"] / 1e3,
+                                  "hbm_bytes_per_dispatch": dom["hbm_bytes_per_dispatch"],
+                                  "dispatches_per_solve": dom["dispatches_per_step"],
+                                  "share_of_solve": dom["avg_us"] * dom["dispatches_per_step"]
+                                  / 1e3 / (solve_s * 1e3)}
     insts = measured_valu(workload)
     if insts:
         # issue bound: one wave64 VALU instruction per CU per cycle (4 SIMDs,
