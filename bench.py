@@ -313,7 +313,7 @@ def tune_uneven_f32r(B):
     return {"win_levels": best[1], "ms": best[0]}
 
 
-def plan_execute_ms(B, prec, K, win_levels=None):
+def plan_execute_ms(B, prec, K, win_levels=None, keep_x=False):
     """Mean event-timed execute of a plan (pcrp_plan_create once: routing
     tables, window levels; executes launch kernels only) over K pristine
     right-hand sides. win_levels: FP32 window level count for this plan
@@ -336,6 +336,7 @@ def plan_execute_ms(B, prec, K, win_levels=None):
     sweeps = torch.zeros(m, dtype=torch.int32, device=y0.device)
     stream = torch.cuda.current_stream()
     plan.execute(ys[0], max_sweeps=MAX_SWEEPS, tol=TOL, sweeps=sweeps)
+    x = ys[0].clone() if keep_x else None
     ys[0].copy_(y0)
     torch.cuda.synchronize()
     tot = 0.0
@@ -352,6 +353,8 @@ def plan_execute_ms(B, prec, K, win_levels=None):
     out = {"ms": tot / K, "route": ["lds", "split", "window", "uneven-queue", "uneven-stage"][route]}
     if prec != "f64":
         out["sweeps_hist"] = torch.bincount(sweeps.long()).tolist()
+    if keep_x:
+        out["x"] = x
     return out
 
 
@@ -408,7 +411,8 @@ def bench_headline(args):
     f64 = run_uneven(B, "f64", K, W)
     f32 = run_uneven(B, "f32r", K, W)
     plan32 = plan_execute_ms(B, "f32r", 5)
-    stage32 = plan_execute_ms(B, "f32r", 3, win_levels=2)
+    stage32 = plan_execute_ms(B, "f32r", 3, win_levels=2, keep_x=not args.no_cpu)
+    x_stage = stage32.pop("x", None)
     ratio = f32["value"] / f64["value"]
     out = {
         "metric": METRIC, "value": f64["value"], "unit": "unknowns/s", "n_gpus": 1,
@@ -458,6 +462,11 @@ def bench_headline(args):
             "f32r_systems_above_1e-12": int((e32 > 1e-12).sum()),
         }
         out["f32r"]["max_relerr_vs_dgbsv"] = float(e32.max())
+        if x_stage is not None:
+            es = oracle.uneven_relerr(x_stage.cpu().numpy(), ref, off, nn)
+            out["f32r"]["stage_path"]["max_relerr_vs_dgbsv"] = float(es.max())
+            out["accuracy"]["f32r_stage_path_max_relerr"] = float(es.max())
+            del x_stage
         tmed, runs, xc = median_runs(lambda: oracle.pcr_uneven(*ins, off, nn), args.cpu_runs)
         out["cpu_baseline"] = {
             "value": total / tmed, "unit": "unknowns/s",
