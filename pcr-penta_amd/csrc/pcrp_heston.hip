@@ -63,7 +63,7 @@ __device__ inline double node(const Grid& g, const double* U, int i, int j, doub
     if (i == 0) return 0.0;
     if (i == g.ns + 1) return bS;
     if (j == g.nv) return S_at(g, i);
-    return U[(long long)j * g.ns + (i - 1)];
+    return U[j * g.ns + (i - 1)];   // ns * nv < 2^31 (checked at entry)
 }
 
 // stencil coefficients of A1 at row i (offsets -2..2)
@@ -124,7 +124,7 @@ __host__ __device__ inline void a2_coef(const Grid& g, int j, double w[5])
 // entries that would couple to a boundary node or fall outside are zero.
 __global__ void setup_lines_kernel(Grid g, double* sa, double* sb, double* sc, double* sd,
                                    double* se, double* va, double* vb, double* vc, double* vd,
-                                   double* ve)
+                                   double* ve, double* vm)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)g.ns * g.nv) return;
@@ -145,6 +145,13 @@ __global__ void setup_lines_kernel(Grid g, double* sa, double* sb, double* sc, d
     vc[idx] = 1.0 + f * w[2];
     vd[idx] = (j + 1 <= g.nv - 1) ? f * w[3] : 0.0;
     ve[idx] = (j + 2 <= g.nv - 1) ? f * w[4] : 0.0;
+    if (i == 1) {   // v-line 0 again, contiguous (row j at vm[k*nv + j]): every v-line's matrix
+        vm[j] = va[idx];
+        vm[g.nv + j] = vb[idx];
+        vm[2 * g.nv + j] = vc[idx];
+        vm[3 * g.nv + j] = vd[idx];
+        vm[4 * g.nv + j] = ve[idx];
+    }
 }
 
 __global__ void payoff_kernel(Grid g, double* U, int* step)
@@ -209,16 +216,12 @@ __device__ inline double apply_a0(const Grid& g, const double* U, int i, int j, 
     return c * x;
 }
 
-// explicit stage: R1 = RHS of the S-line systems, W = v-line correction
-// (step index: the device counter plus this step's offset within its chunk)
-__global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, const int* step,
-                                int off)
+// the explicit stage at unknown (i, j): R1 = RHS of the S-line system,
+// W = v-line correction (bS0, bS1: the far-S boundary at tau0, tau0 + dt)
+__device__ inline void explicit_node(const Grid& g, const double* U, double* R1, double* W, int i,
+                                     int j, double bS0, double bS1)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long long)g.ns * g.nv) return;
-    const int j = (int)(idx / g.ns), i = (int)(idx % g.ns) + 1;
-    const double tau0 = (*step + off) * g.dt, tau1 = tau0 + g.dt;
-    const double bS0 = far_s(g, tau0), bS1 = far_s(g, tau1);
+    const int idx = j * g.ns + (i - 1);
     const double u = U[idx];
     double w1[5], w2[5];   // each stencil's coefficients once per node
     a1_coef(g, i, j, w1);
@@ -231,6 +234,35 @@ __global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, 
     R1[idx] = y0 - th * l1 + th * bnd_a1(g, i, j, bS1, w1);
     W[idx] = -th * l2 + th * bnd_a2(g, i, j, bS1, w2);
 }
+
+// far-S boundary values of every step, once per run: bnd[2k] at
+// tau = k dt, bnd[2k+1] at tau = k dt + dt (the explicit stage's tau0, tau1;
+// the same expressions as computing them per node, without two FP64 exp per
+// node and step)
+__global__ void boundary_kernel(Grid g, double* bnd, int steps)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= steps) return;
+    const double tau0 = k * g.dt, tau1 = tau0 + g.dt;
+    bnd[2 * k] = far_s(g, tau0);
+    bnd[2 * k + 1] = far_s(g, tau1);
+}
+
+// explicit stage: R1 = RHS of the S-line systems, W = v-line correction
+// (step index: the device counter plus this step's offset within its chunk).
+// Grid: x over S (i - 1), y over v (j, strided by gridDim.y), so j is
+// uniform per workgroup.
+__global__ void explicit_kernel(Grid g, const double* U, double* R1, double* W, const int* step,
+                                int off, const double* __restrict__ bnd)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + 1;
+    if (i > g.ns) return;
+    const int k = *step + off;
+    const double bS0 = bnd[2 * k], bS1 = bnd[2 * k + 1];
+    for (int j = (int)blockIdx.y; j < g.nv; j += (int)gridDim.y)
+        explicit_node(g, U, R1, W, i, j, bS0, bS1);
+}
+
 
 // the next chunk's first step index, once its explicit kernels have read
 // this one (one thread: a completion counter across the explicit kernel's
@@ -268,6 +300,8 @@ Grid make_grid(const pcrp_heston_params* p)
 struct HestonWs {
     double *sa, *sb, *sc, *sd, *se, *va, *vb, *vc, *vd, *ve, *R1, *W;
     int* step;
+    double* bnd;   // far-S boundary values per step (boundary_kernel)
+    double* vm;    // the v-line matrix, contiguous: diagonal k at vm + k*nv
     void* solver_ws;
     size_t solver_bytes;
     void* fV;   // reuse_factor: the shared factor of the v-line matrix
@@ -295,10 +329,17 @@ bool reuse_supported(const pcrp_heston_params* p, int precision)
     return !p->reuse_factor || factor_v_bytes(p, precision) != 0;
 }
 
+// two boundary values per time step
+size_t bnd_bytes(const pcrp_heston_params* p) { return align256(2 * (size_t)p->steps * sizeof(double)); }
+
+// the contiguous copy of the v-line matrix (five diagonals of nv rows)
+size_t vm_bytes(const pcrp_heston_params* p) { return align256(5 * (size_t)p->nv * sizeof(double)); }
+
 size_t heston_ws_bytes(const pcrp_heston_params* p, int precision)
 {
     const size_t N = (size_t)p->ns * p->nv;
-    return align256(12 * N * sizeof(double) + 256) + solver_ws_bytes(p, precision) +
+    return align256(12 * N * sizeof(double) + 256) + bnd_bytes(p) + vm_bytes(p) +
+           solver_ws_bytes(p, precision) +
            factor_v_bytes(p, precision);
 }
 
@@ -312,6 +353,10 @@ HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
     for (int k = 0; k < 12; ++k) *slots[k] = d + (size_t)k * N;
     h.step = reinterpret_cast<int*>(d + 12 * N);
     char* c = static_cast<char*>(ws) + align256(12 * N * sizeof(double) + 256);
+    h.bnd = reinterpret_cast<double*>(c);
+    c += bnd_bytes(p);
+    h.vm = reinterpret_cast<double*>(c);
+    c += vm_bytes(p);
     h.solver_ws = c;
     h.solver_bytes = solver_ws_bytes(p, precision);
     c += h.solver_bytes;
@@ -323,16 +368,18 @@ HestonWs carve(const pcrp_heston_params* p, int precision, void* ws)
 int factor_lines(const Grid& g, int precision, const HestonWs& h, hipStream_t st)
 {
     if (!h.fV) return PCRP_OK;
-    // v-line 0's rows (element j at j*ns) are the matrix of every v-line
+    // the contiguous copy of v-line 0's rows is the matrix of every v-line
+    const double* const vm = h.vm;
+    const int nv = g.nv;
     if (precision == PCRP_PREC_F64)
-        return factor_f64(h.va, h.vb, h.vc, h.vd, h.ve, g.nv, 1, g.ns, 0,
+        return factor_f64(vm, vm + nv, vm + 2 * nv, vm + 3 * nv, vm + 4 * nv, nv, 1, 1, 0,
                           static_cast<double*>(h.fV), st) == 0
                    ? PCRP_OK
                    : PCRP_ERR_HIP;
-    SolveArgsF32R A{h.va, h.vb, h.vc, h.vd, h.ve, nullptr, g.nv, g.ns, g.ns, 1, pcr_steps(g.nv),
-                    0, 0.0, nullptr};
+    SolveArgsF32R A{vm, vm + nv, vm + 2 * nv, vm + 3 * nv, vm + 4 * nv, nullptr, nv, g.ns, g.ns, 1,
+                    pcr_steps(nv), 0, 0.0, nullptr};
     A.shared = true;
-    A.ars = g.ns;
+    A.ars = 1;
     return lds_f32r_factor(A, h.fV, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
 }
 
@@ -367,10 +414,13 @@ int solve_v_lines(const Grid& g, int precision, double* U, const HestonWs& h, hi
         return solve_factored_f64(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
     }
     if (h.fV) {
-        SolveArgsF32R A{h.va, h.vb, h.vc, h.vd, h.ve, U, n, m, m, 1, pcr_steps(n),
-                        PCRP_HESTON_MAX_SWEEPS, PCRP_HESTON_REFINE_TOL, nullptr};
+        // the shared matrix from its contiguous copy (coalesced rows, not v-line 0's
+        // one-double-per-line gather)
+        const double* const vm = h.vm;
+        SolveArgsF32R A{vm, vm + n, vm + 2 * n, vm + 3 * n, vm + 4 * n, U, n, m, m, 1,
+                        pcr_steps(n), PCRP_HESTON_MAX_SWEEPS, PCRP_HESTON_REFINE_TOL, nullptr};
         A.shared = true;
-        A.ars = m;
+        A.ars = 1;
         A.fct = h.fV;
         return lds_f32r_solve(A, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
     }
@@ -390,8 +440,9 @@ int enqueue_step(const Grid& g, int precision, double* U, const HestonWs& h, int
     const long long N = (long long)g.ns * g.nv;
     const int bs = 256;
     const unsigned blocks = (unsigned)((N + bs - 1) / bs);
-    hipLaunchKernelGGL(explicit_kernel, dim3(blocks), dim3(bs), 0, st, g, U, h.R1, h.W, h.step,
-                       off);
+    hipLaunchKernelGGL(explicit_kernel, dim3((g.ns + bs - 1) / bs, g.nv < 65535 ? g.nv : 65535),
+                       dim3(bs), 0, st, g, U,
+                       h.R1, h.W, h.step, off, h.bnd);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     int rc = solve_s_lines(g, precision, U, h, st);
     if (rc == PCRP_ERR_UNSUPPORTED) {
@@ -477,8 +528,11 @@ int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph,
     const int bs = 256;
     const unsigned blocks = (unsigned)((N + bs - 1) / bs);
     hipLaunchKernelGGL(setup_lines_kernel, dim3(blocks), dim3(bs), 0, st, g, h.sa, h.sb, h.sc,
-                       h.sd, h.se, h.va, h.vb, h.vc, h.vd, h.ve);
+                       h.sd, h.se, h.va, h.vb, h.vc, h.vd, h.ve, h.vm);
     hipLaunchKernelGGL(payoff_kernel, dim3(blocks), dim3(bs), 0, st, g, u, h.step);
+    if (p->steps > 0)
+        hipLaunchKernelGGL(boundary_kernel, dim3((p->steps + bs - 1) / bs), dim3(bs), 0, st, g,
+                           h.bnd, p->steps);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     const int frc = factor_lines(g, precision, h, st);
     if (frc != PCRP_OK) return frc;

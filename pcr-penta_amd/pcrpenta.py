@@ -31,7 +31,9 @@ STATUS = {
 class PCRPError(RuntimeError):
     def __init__(self, status: int, what: str):
         self.status = status
-        super().__init__(f"{what}: {STATUS.get(status, status)} ({strerror(status)})")
+        # the library's own text when it is loaded (never load it from here)
+        text = strerror(status) if _lib is not None else STATUS.get(status, str(status))
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({text})")
 
 
 _lib = None
@@ -136,7 +138,7 @@ def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise PCRPError(7, f"{LIB_PATH} missing (run __graft_entry__.build())")
+            raise FileNotFoundError(f"{LIB_PATH} missing (run __graft_entry__.build())")
         _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
 

@@ -238,3 +238,19 @@ def test_tuning_tables_are_consistent_under_concurrent_use(lib):
         pcrpenta.split_tune_set(*split0)
         pcrpenta.f32r_tune_set(*f32r0)
     assert not bad, bad[:5]
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the built library the binding raises a plain error that names
+    the file (the product path never falls back to anything else)."""
+    import pcrpenta
+    saved_path, saved_lib = pcrpenta.LIB_PATH, pcrpenta._lib
+    try:
+        pcrpenta.LIB_PATH = str(tmp_path / "libpcrpenta.so")
+        pcrpenta._lib = None
+        with pytest.raises(FileNotFoundError, match="libpcrpenta.so missing"):
+            pcrpenta.load()
+        err = pcrpenta.PCRPError(5, "pcrp_solve_f64")   # no library: the status name only
+        assert "PCRP_ERR_HIP" in str(err)
+    finally:
+        pcrpenta.LIB_PATH, pcrpenta._lib = saved_path, saved_lib

@@ -142,3 +142,45 @@ def test_factored_f32r_shared_matrix(gpu, oracle_lib, n, m, layout):
     print("shared f32r", n, m, layout, "err", err, "sweeps", np.bincount(_np(sw)), flush=True)
     assert err <= 1e-12
     assert _np(sw).min() >= 1 and _np(sw).max() <= 3
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 64, 65, 100, 128, 129, 200, 256, 257, 333, 511, 512])
+def test_factored_f32r_shared_grouped_equals_single(gpu, oracle_lib, n):
+    """Interleaved right-hand sides of one shared matrix, m even and 16-byte
+    aligned (pairs of systems per workgroup, 16-byte loads): the same x and
+    sweep counts as one system per workgroup (the same y at an 8-byte offset,
+    which that kernel takes), bit for bit: with the default stopping rule,
+    with tol = 0 (systems run max_sweeps and are flagged), and with a NaN in
+    one system."""
+    import torch
+    m = 300   # more systems than CUs: the grouped loader's condition
+    one = oracle_lib.gen_batch(1, 30 + n, n, 1, C)
+    dev = _dev(one[:5])
+    F = gpu.factor(*dev, n, m, I, precision=gpu.PREC_F32_REFINE, shared=True)
+    for case, (tol, ms) in enumerate(((1e-12, 8), (0.0, 3), (1e-12, 8))):
+        _say("f32r-shared-grouped", n, m, case)
+        rhs = oracle_lib.gen_batch(0, 31 + n + case, n, m, I)[5].copy()
+        if case == 2:
+            rhs[(n // 2) * m + 5] = np.nan
+        out = []
+        for off in (0, 1):
+            buf = torch.zeros(n * m + 2, dtype=torch.float64, device="cuda")
+            y = buf[off:off + n * m]
+            y.copy_(torch.from_numpy(rhs))
+            assert (y.data_ptr() % 16 == 0) == (off == 0)
+            sw = torch.full((m,), -1, dtype=torch.int32, device="cuda")
+            gpu.solve_factored_refined(F, *dev, y, n, m, I, shared=True, max_sweeps=ms, tol=tol,
+                                       sweeps=sw)
+            torch.cuda.synchronize()
+            out.append((_np(y).copy(), _np(sw).copy()))
+        (x0, s0), (x1, s1) = out
+        assert np.array_equal(np.nan_to_num(x0, nan=7.0), np.nan_to_num(x1, nan=7.0)), (n, case)
+        assert np.array_equal(s0, s1), (n, case)
+        if case == 1:   # tol = 0: only an exactly zero correction stops a system early
+            assert (s0 <= ms + 1).all() and (n < 64 or (s0 == ms + 1).any())
+        if case == 2:
+            assert s0[5] == 9 and (np.delete(s0, 5) <= 3).all()
+        if case == 0:
+            rep = [np.repeat(v, m) for v in one[:5]]
+            ref = oracle_lib.gbsv_batch(*rep, rhs, n, m, I)
+            assert oracle_lib.normwise_relerr(x0, ref, n, m, I) <= 1e-12
