@@ -16,6 +16,7 @@
 #pragma once
 
 #include "pcrp_device.h"
+#include "pcrp_internal.h"
 
 namespace pcrp {
 namespace {
@@ -178,6 +179,14 @@ __host__ inline int resident_grid(K kernel, int block_threads)
         return 0;
     const int g = cus * (per > 0 ? per : 1) / 8 * 8;
     return g > 0 ? g : 8;
+}
+
+// resident_grid cached per device (one cache per call site).
+template <typename K>
+__host__ inline int resident_grid(std::atomic<int> (&cache)[kMaxDevices], K kernel,
+                                  int block_threads)
+{
+    return device_cached(cache, [&](int) { return resident_grid(kernel, block_threads); });
 }
 
 // Launch grid for `logical` blocks walked by at most `resident` workgroups.

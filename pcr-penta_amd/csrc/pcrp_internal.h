@@ -5,7 +5,40 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
+
 namespace pcrp {
+
+// Per-device launch constants (CU counts, resident grids), computed on the
+// first call made with that device current and kept per device: safe for
+// concurrent callers (two first calls compute the same value) and for
+// processes that use several devices. compute(dev) returns the value, or
+// a value <= 0 on failure (not cached, returned as is).
+constexpr int kMaxDevices = 64;
+template <typename F>
+inline int device_cached(std::atomic<int> (&cache)[kMaxDevices], F&& compute)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v <= 0) {
+        v = compute(dev);
+        if (v > 0) cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
+// Compute units of the current device (-1 if the query fails).
+inline int device_cu_count()
+{
+    static std::atomic<int> cache[kMaxDevices];
+    return device_cached(cache, [](int dev) {
+        int cus = 0;
+        return hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess
+                   ? cus
+                   : -1;
+    });
+}
 
 // One batched FP64 solve over a strided view: element i of system j at
 // a[j*ss + i*rs]; `steps` reduction steps, final delta = 2^steps.

@@ -539,13 +539,8 @@ static bool groupable(const SolveArgsF32R& A, int G)
 
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
 {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            return -1;
-    }
+    const int cus = device_cu_count();
+    if (cus <= 0) return -1;
     const int n = A.n;
     if (n <= 128) return launch_f32r<64, 1, 1>(A, st);
     if (n <= 256) return launch_f32r<128, 1, 1>(A, st);

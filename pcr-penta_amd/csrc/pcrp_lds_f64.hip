@@ -469,17 +469,18 @@ template <int KIND, int NT, int RPT>
 static int launch_queue(const double* const* sets, double* x, const LdsJob* jobs, int njobs,
                         const int* qstart, unsigned int* counters, hipStream_t st)
 {
-    static int grid = 0;
-    if (grid == 0) {
-        int dev = 0, cus = 0, per = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+    static std::atomic<int> grids[kMaxDevices];
+    const int grid = device_cached(grids, [](int) {
+        const int cus = device_cu_count();
+        int per = 0;
+        if (cus <= 0 ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per, reinterpret_cast<const void*>(&lds_f64_queue_kernel<KIND, NT, RPT>), NT, 0) !=
                 hipSuccess)
             return -1;
-        grid = cus * (per > 0 ? per : 1);
-    }
+        return cus * (per > 0 ? per : 1);
+    });
+    if (grid <= 0) return -1;
     const int g = njobs < grid ? njobs : grid;
     if (g <= 0) return 0;
     hipLaunchKernelGGL((lds_f64_queue_kernel<KIND, NT, RPT>), dim3(g), dim3(NT), 0, st, sets, x,
@@ -505,19 +506,18 @@ static int resident_blocks_per_cu(int nt, int vgprs, int lds_bytes)
 template <int KIND, int NT, int RPT>
 static int launch_persist(const SolveArgsF64& A, hipStream_t st)
 {
-    static int grid = 0;
-    if (grid == 0) {
-        int dev = 0, cus = 0;
+    static std::atomic<int> grids[kMaxDevices];
+    const int grid = device_cached(grids, [](int) {
+        const int cus = device_cu_count();
         hipFuncAttributes fa;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(
-                                          &lds_f64_persist_kernel<KIND, NT, RPT>)) != hipSuccess)
+        if (cus <= 0 || hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(
+                                                      &lds_f64_persist_kernel<KIND, NT, RPT>)) !=
+                            hipSuccess)
             return -1;
-        grid = cus * resident_blocks_per_cu(NT, fa.numRegs, (int)fa.sharedSizeBytes);
-        grid = (grid / 8) * 8;
-        if (grid < 8) grid = 8;
-    }
+        const int g = (cus * resident_blocks_per_cu(NT, fa.numRegs, (int)fa.sharedSizeBytes) / 8) * 8;
+        return g < 8 ? 8 : g;
+    });
+    if (grid <= 0) return -1;
     // split views (SplitInfo) use the batch form; this form covers plain batches
     if (A.split.S > 0) return launch<KIND, NT, RPT>(A, st);
     const int g = A.m < grid ? A.m : grid;
@@ -595,13 +595,7 @@ int lds_f64_default_variant(int n)
 // up to m = 512, 2-10% slower from m = 1024 on than 256 x 4).
 int lds_f64_default_variant_for_batch(int n, int m)
 {
-    static int cus = 0;
-    if (cus == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            cus = -1;
-    }
+    const int cus = device_cu_count();
     if (n > 512 && n <= 1024 && cus > 0 && m <= 2 * cus) {
         for (int id = 0; id < lds_f64_variant_count(); ++id) {
             const LdsVariantF64& v = kVariants[id];

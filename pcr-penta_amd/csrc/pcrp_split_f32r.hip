@@ -237,9 +237,12 @@ int split_f32r_passes(const SolveArgsF32R& A, void* ws, size_t ws_bytes, hipStre
     for (int r = 0; r < P.rounds; ++r) p = carve_brows(p, P.per, sets[r]);
     if (S < 1) return -2;   // the split path always runs at least one pass
     constexpr int kPassThreads = kPassTB * kPassG / kPassRPT;
-    static const int gPass = resident_grid(bpass_kernel<kPassTB, kPassG, kPassRPT>, kPassThreads);
-    static const int gStage = resident_grid(bstage_kernel<kStageNT, kStageBR>, kStageNT);
-    static const int gRes = resident_grid(residual_kernel, 256);
+    static std::atomic<int> c_gPass[kMaxDevices];
+    const int gPass = resident_grid(c_gPass, bpass_kernel<kPassTB, kPassG, kPassRPT>, kPassThreads);
+    static std::atomic<int> c_gStage[kMaxDevices];
+    const int gStage = resident_grid(c_gStage, bstage_kernel<kStageNT, kStageBR>, kStageNT);
+    static std::atomic<int> c_gRes[kMaxDevices];
+    const int gRes = resident_grid(c_gRes, residual_kernel, 256);
     const long long Lst = (P.nb0 + (1LL << S) - 1) >> S;   // block rows per stage sub-system
     const unsigned bblocks = (unsigned)((NB + 255) / 256);
     const unsigned sblocks = (unsigned)((m0 + 255) / 256);

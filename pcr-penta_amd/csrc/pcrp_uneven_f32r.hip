@@ -392,10 +392,14 @@ int launch_pass_stage(const Plan& P, const double* a, const double* b, const dou
         p = carve_brows(p, (size_t)P.nbt_pad, sets.s[r + 1]);
     const unsigned bblocks = (unsigned)((P.nbt + 255) / 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
-    static const int gPass0 = resident_grid(upass_kernel<kTB0, kG0, kRPT>, kTB0 * kG0 / kRPT);
-    static const int gPass1 = resident_grid(upass_kernel<kTB1, kG1, kRPT>, kTB1 * kG1 / kRPT);
-    static const int gStage = resident_grid(ustage_kernel<kStageNT, kStageBR>, kStageNT);
-    static const int gRes = resident_grid(uresidual_kernel, 256);
+    static std::atomic<int> c_gPass0[kMaxDevices];
+    const int gPass0 = resident_grid(c_gPass0, upass_kernel<kTB0, kG0, kRPT>, kTB0 * kG0 / kRPT);
+    static std::atomic<int> c_gPass1[kMaxDevices];
+    const int gPass1 = resident_grid(c_gPass1, upass_kernel<kTB1, kG1, kRPT>, kTB1 * kG1 / kRPT);
+    static std::atomic<int> c_gStage[kMaxDevices];
+    const int gStage = resident_grid(c_gStage, ustage_kernel<kStageNT, kStageBR>, kStageNT);
+    static std::atomic<int> c_gRes[kMaxDevices];
+    const int gRes = resident_grid(c_gRes, uresidual_kernel, 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
     hipLaunchKernelGGL(uprep_kernel, dim3(bblocks), dim3(256), 0, st, a, b, c, d, e, y, sets.s[0],
                        y0, dsys, m, P.nbt);

@@ -513,8 +513,10 @@ int window_run(const double* a, const double* b, const double* c, const double* 
             hipSuccess ||
         hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess)
         return PCRP_ERR_HIP;
-    static const int gWin = resident_grid(wsweep_kernel<kWinNT, kWinBR, false>, kWinNT);
-    static const int gCopy = resident_grid(wcopy_kernel, 256);
+    static std::atomic<int> c_gWin[kMaxDevices];
+    const int gWin = resident_grid(c_gWin, wsweep_kernel<kWinNT, kWinBR, false>, kWinNT);
+    static std::atomic<int> c_gCopy[kMaxDevices];
+    const int gCopy = resident_grid(c_gCopy, wcopy_kernel, 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
     const dim3 tgrid((unsigned)((n + kTrRows - 1) / kTrRows), (unsigned)((m + kTrSys - 1) / kTrSys));
@@ -693,8 +695,10 @@ int window_plan_execute(const WindowPlan& P, const double* a, const double* b, c
     const bool transposed = P.transposed;
     const long long N = P.elems;
     const int m = P.m, n = P.n;
-    static const int gWin = resident_grid(wsweep_kernel<kWinNT, kWinBR, false>, kWinNT);
-    static const int gCopy = resident_grid(wcopy_kernel, 256);
+    static std::atomic<int> c_gWin[kMaxDevices];
+    const int gWin = resident_grid(c_gWin, wsweep_kernel<kWinNT, kWinBR, false>, kWinNT);
+    static std::atomic<int> c_gCopy[kMaxDevices];
+    const int gCopy = resident_grid(c_gCopy, wcopy_kernel, 256);
     const unsigned sblocks = (unsigned)((m + 255) / 256);
     hipLaunchKernelGGL(init_state_kernel, dim3(sblocks), dim3(256), 0, st, state, stop, m);
     const dim3 tgrid((unsigned)((n + kTrRows - 1) / kTrRows), (unsigned)((m + kTrSys - 1) / kTrSys));
