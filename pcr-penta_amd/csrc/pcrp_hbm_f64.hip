@@ -173,13 +173,15 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
         sR);
 }
 
-// Uneven batch: systems stored contiguously at their own offsets. For a single
-// system the residue-interleaved index of row l of sub-system q (q < 2^Sin)
-// is q + l*2^Sin, i.e. the row's own index, so each pass reads and writes
-// the same element positions (in the caller's arrays or a same-shaped
-// workspace). tab lists the systems that still need passes in this round;
-// bmap gives every block its entry of tab (a binary search over the entries,
-// 13 dependent loads at 6000 systems, delayed every block's first loads).
+// Uneven batch: systems stored contiguously at their own offsets. The first
+// round reads the caller's rows (off + l); every round writes its residue
+// sub-systems to the pass buffers in the grouped residue layout at the
+// system's own region (woff + res_slot), and later rounds read them there.
+// Row l of input residue q (Sin levels) goes to residue q + (l mod 2^k) 2^Sin,
+// element l >> k, of the Sin + k levels after the pass.
+// tab lists the systems that still need passes in this round; bmap gives
+// every block its entry of tab (a binary search over the entries, 13
+// dependent loads at 6000 systems, delayed every block's first loads).
 template <int TB, int G, int RPT>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     const double* const* __restrict__ sets, int src, int dst, const PassSys* __restrict__ tab,
@@ -204,8 +206,14 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     double* const out[6] = {const_cast<double*>(sets[6 * dst + 0]), const_cast<double*>(sets[6 * dst + 1]),
                             const_cast<double*>(sets[6 * dst + 2]), const_cast<double*>(sets[6 * dst + 3]),
                             const_cast<double*>(sets[6 * dst + 4]), const_cast<double*>(sets[6 * dst + 5])};
-    auto at = [&](int ll) { return P.off + q + ((long long)ll << Sin); };
-    pass_tile<double, TB, G, RPT>(in, out, q < Qin, l0, L, k, H, T, at, at, sR);
+    const int Sout = Sin + k;
+    auto in_at = [&](int ll) {
+        return Sin == 0 ? P.off + ll : P.woff + res_slot(q, ll, Sin, P.n);
+    };
+    auto out_at = [&](int ll) {
+        return P.woff + res_slot(q + ((ll & ((1 << k) - 1)) << Sin), ll >> k, Sout, P.n);
+    };
+    pass_tile<double, TB, G, RPT>(in, out, q < Qin, l0, L, k, H, T, in_at, out_at, sR);
 }
 
 template <typename T, int TB, int G, int RPT>

@@ -67,19 +67,52 @@ struct SolveArgsF64 {
 constexpr int kQueues = 8;
 
 // One job of the persistent LDS queue: rows base + (i << stride_log2),
-// i < n, of array set `src`; x written to the same indices of the output.
+// i < n, of array set `src`; x written to xbase + (i << xstride_log2) of the
+// output.
 struct LdsJob {
-    long long base;
-    int stride_log2, n, steps, src;
+    long long base, xbase;
+    int stride_log2, xstride_log2, n, steps, src, pad;
 };
 
-// One system of an uneven-batch HBM pass round: rows at off + i, i < n;
-// k steps this round; the round's blocks for it start at block0.
+// One system of an uneven-batch HBM pass round: rows at off + i, i < n, in
+// the caller's arrays, its residue sub-systems at woff in the pass buffers
+// (res_slot); k steps this round; the round's blocks for it start at block0.
 struct PassSys {
-    long long off;
+    long long off, woff;
     long long block0;
     int n, k, tiles, pad;
 };
+
+// Residue sub-systems of a split system in the pass buffers. After s levels
+// the system's rows form Q = 2^s residue sub-systems (residue q: rows
+// q + l*2^s, l < L_q = ceil((n - q) / Q)). Groups of G = min(kResGroup, Q)
+// consecutive residues are stored together, row-interleaved: element e of
+// residue q at (q / G) * G * Lmax + e * G + q % G, Lmax = ceil(n / Q). A pass
+// tile of G neighbouring residues then reads 8 G contiguous bytes per row
+// (a wave's rows are one contiguous run), and an LDS job reads its residue at
+// stride G: G of its rows per 8 G-double run, instead of one row per 128-byte
+// line when every residue of the system is interleaved (stride Q).
+// Slots of one level lie in [0, Q * Lmax) and are distinct; Q * Lmax grows
+// with s, so 2^S * ceil(n / 2^S) slots hold every level of an S-level split.
+constexpr int kResGroupLog2 = 2;
+
+__host__ __device__ inline int res_group_log2(int s)
+{
+    return s < kResGroupLog2 ? s : kResGroupLog2;
+}
+
+__host__ __device__ inline long long res_slot(int q, long long e, int s, int n)
+{
+    const int gl = res_group_log2(s);
+    const long long lmax = ((long long)n + (1LL << s) - 1) >> s;
+    return ((long long)(q >> gl) * lmax << gl) + (e << gl) + (q & ((1 << gl) - 1));
+}
+
+// Slots of the pass buffers one system of n rows split S levels takes.
+__host__ __device__ inline long long res_region(int n, int S)
+{
+    return (((long long)n + (1LL << S) - 1) >> S) << S;
+}
 
 // A compiled LDS-kernel variant: kind (0 shared pair equations, 1 redundant,
 // 2 in-place pair equations; +4 for the persistent prefetching batch form),

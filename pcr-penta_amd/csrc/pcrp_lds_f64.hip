@@ -421,10 +421,10 @@ __global__ __launch_bounds__(NT) void lds_f64_persist_kernel(
 // split systems) are split into one queue per XCD; a workgroup drains the
 // queue of the XCD it runs on (hardware register XCC_ID), then helps the
 // others, so every wave reaches the exit once all queues are empty. Residue
-// sub-systems of one system read interleaved rows that share cache lines, and
-// the host puts runs of them into the same queue, i.e. behind the same L2.
-// src picks the array set a job reads (the caller's arrays or a pass
-// workspace); x is written at the job's own indices.
+// sub-systems of one system read row-interleaved groups that share cache
+// lines, and the host puts runs of them into the same queue, i.e. behind the
+// same L2. src picks the array set a job reads (the caller's arrays or a pass
+// buffer); x goes to the job's output indices (the system's own rows).
 template <int KIND, int NT, int RPT>
 __global__ __launch_bounds__(NT) void lds_f64_queue_kernel(
     const double* const* __restrict__ sets, double* __restrict__ gx, const LdsJob* __restrict__ jobs,
@@ -446,9 +446,9 @@ __global__ __launch_bounds__(NT) void lds_f64_queue_kernel(
             if (idx >= len) break;
             const LdsJob J = jobs[begin + idx];
             const double* const* S = sets + 6 * J.src;
-            const long long rs = 1LL << J.stride_log2;
-            lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base, rs,
-                                         J.base, rs, J.n, J.steps, sR, sE);
+            lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base,
+                                         1LL << J.stride_log2, J.xbase, 1LL << J.xstride_log2,
+                                         J.n, J.steps, sR, sE);
             __syncthreads();
         }
     }

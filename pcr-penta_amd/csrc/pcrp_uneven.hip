@@ -10,11 +10,14 @@
 //   * n > lds_max: S = ceil(log2(n / lds_max)) reduction steps run as HBM
 //     passes (k_fuse steps per pass); all large systems share each pass round
 //     (one launch per round, a table maps blocks to systems); then every one of
-//     its 2^S residue sub-systems is one LDS job.
+//     its 2^S residue sub-systems is one LDS job. The passes write residue
+//     sub-systems to the system's own region of two pass buffers in the
+//     grouped residue layout (res_slot: groups of 4 residues row-interleaved),
+//     which the next round and the LDS jobs read.
 // All LDS jobs go through one persistent kernel with one work queue per XCD:
 // a workgroup drains its own XCD's queue, then the others. Runs of 16
-// consecutive residue sub-systems of a system (whose rows share cache lines)
-// stay in one queue, behind one L2; runs go to the least-loaded queue,
+// consecutive residue sub-systems of a system (groups whose rows share cache
+// lines) stay in one queue, behind one L2; runs go to the least-loaded queue,
 // largest first (longest-processing-time order).
 // Results are bit-identical to solving each system alone (same arithmetic,
 // same guards; the pass rounds are the split path with one system per batch).
@@ -55,6 +58,8 @@ struct UnevenPlan {
     std::vector<long long> round_blocks;
     std::vector<int> round_shape;
     long long total = 0;                        // elements per array (max off + n)
+    long long wtotal = 0;                       // pass-buffer slots per array
+    std::vector<long long> woff;                // each split system's pass-buffer region
     bool any_split = false;
     // workspace byte offsets
     size_t off_counter = 0, off_sets = 0, off_jobs = 0, off_rounds = 0, off_arrays = 0, bytes = 0;
@@ -76,6 +81,13 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
         if (S > 0) P.any_split = true;
     }
     if (P.total > 0x7fffffffLL) return PCRP_ERR_INVALID_SIZE;
+    // pass-buffer regions of the split systems (grouped residue layout)
+    P.woff.assign(m, -1);
+    for (const Sys& s : order)
+        if (s.S > 0) {
+            P.woff[s.j] = P.wtotal;
+            P.wtotal += res_region(h_n[s.j], s.S);
+        }
     std::stable_sort(order.begin(), order.end(),
                      [](const Sys& x, const Sys& y) { return x.nsub > y.nsub; });
     // LDS jobs, in chunks: a small system alone, or up to 16 consecutive
@@ -124,14 +136,17 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
         const int steps = pcr_steps(n);
         int& at = pos[qof[c]];
         if (ch.q0 < 0) {
-            P.jobs[at++] = LdsJob{h_off[ch.sys], 0, n, steps, 0};
+            P.jobs[at++] = LdsJob{h_off[ch.sys], h_off[ch.sys], 0, 0, n, steps, 0, 0};
             continue;
         }
         const int S = split_f64_levels(n, tune.lds_max);
         const int rounds = (S + tune.k_fuse - 1) / tune.k_fuse;
         const int src = 1 + ((rounds - 1) & 1);
+        // residue q: read from its group in the pass buffer, x to rows q + i 2^S
         for (int q = ch.q0; q < ch.q1; ++q)
-            P.jobs[at++] = LdsJob{h_off[ch.sys] + q, S, (n - q + (1 << S) - 1) >> S, steps - S, src};
+            P.jobs[at++] = LdsJob{P.woff[ch.sys] + res_slot(q, 0, S, n), h_off[ch.sys] + q,
+                                  res_group_log2(S), S, (n - q + (1 << S) - 1) >> S, steps - S,
+                                  src, 0};
     }
     P.qstart[kQueues] = (int)P.jobs.size();
     // HBM pass rounds: round r covers steps [r*k_fuse, min(S, (r+1)*k_fuse))
@@ -152,7 +167,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
             const int L = (h_n[j] + (1 << Sin) - 1) >> Sin;
             const int tiles = (L + T - 1) / T;
             const int groups = ((1 << Sin) + G - 1) / G;
-            tab.push_back({h_off[j], blocks, h_n[j], k, tiles, 0});
+            tab.push_back({h_off[j], P.woff[j], blocks, h_n[j], k, tiles, 0});
             blocks += (long long)groups * ((tiles + 7) / 8) * 8;   // XCD-striped tiles
         }
         if (tab.empty()) break;
@@ -174,7 +189,7 @@ int plan_uneven(const int64_t* h_off, const int* h_n, int m, UnevenPlan& P)
         o = align_up(o + (size_t)blocks * sizeof(int));
     }
     P.off_arrays = o;
-    if (P.any_split) o += 2 * 6 * (size_t)P.total * sizeof(double);
+    if (P.any_split) o += 2 * 6 * (size_t)P.wtotal * sizeof(double);
     P.bytes = o;
     return PCRP_OK;
 }
@@ -208,7 +223,7 @@ int upload_plan(const UnevenPlan& P, const double* a, const double* b, const dou
     const double* sets[18] = {a, b, c, d, e, y};
     for (int v = 0; v < 12; ++v)
         sets[6 + v] = P.any_split
-                          ? reinterpret_cast<double*>(w + P.off_arrays) + (size_t)v * P.total
+                          ? reinterpret_cast<double*>(w + P.off_arrays) + (size_t)v * P.wtotal
                           : nullptr;
     const size_t up = P.round_map_off.empty() ? P.off_arrays : P.round_map_off[0];
     host.assign(up, 0);
