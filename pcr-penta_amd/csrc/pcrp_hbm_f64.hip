@@ -13,13 +13,15 @@
 // side) lies in the tile. Tiles run in parallel (no sequential tile walk and
 // no shared-memory cache of the previous tile, so no write-after-read race).
 // The output goes to a workspace, already split by residue: row l of current
-// sub-system q lands in sub-system q' = (l mod 2^k)*Q + q at index l >> k,
-// interleaved (element l' of q' at l'*Q' + q'). With the original systems
-// numbered j, this makes sub-system (residue R, system j) land at q = R*m + j
-// after any sequence of passes, so the next pass and the final LDS stage read
-// coalesced, and the final stage writes x back through the same index
-// pattern in the caller's interleaved layout. a..e and y are never written
-// by the passes.
+// sub-system q lands in sub-system q' = (l mod 2^k)*Q + q at index l >> k.
+// With the original systems numbered j, this makes sub-system (residue R,
+// system j) land at q = R*m + j after any sequence of passes. The workspace
+// holds the sub-systems in the grouped layout (split_slot): groups of 4
+// consecutive sub-systems row-interleaved, so a tile of 4 sub-systems reads
+// one contiguous run per wave and the final LDS stage reads each sub-system at
+// stride 4 (a layout interleaving all Q sub-systems puts each of a stage
+// workgroup's rows in its own 128-byte line). The final stage writes x back
+// to the caller's rows. a..e and y are never written by the passes.
 //
 // Per-row arithmetic is make_pair/reduce_row, as in the LDS kernel, with the
 // guards evaluated in the current sub-system's coordinates. Those guards map
@@ -162,13 +164,18 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const int R = q / m0;
     const int L = (n0 - R + (1 << Sin) - 1) >> Sin;
     const long long Qout = (long long)Qin << k;
+    const long long Lin = ((long long)n0 + (1LL << Sin) - 1) >> Sin;
+    const long long Lout = ((long long)n0 + (1LL << (Sin + k)) - 1) >> (Sin + k);
     const T* const in[6] = {ia, ib, ic, id, ie, iy};
     T* const out[6] = {oa, ob, oc, od, oe, oy};
     pass_tile<T, TB, G, RPT>(
         in, out, q < Qin, l0, L, k, H, TI,
-        [&](int ll) { return (long long)q * in_ss + (long long)ll * in_rs; },
         [&](int ll) {
-            return (long long)(ll >> k) * Qout + (long long)(ll & ((1 << k) - 1)) * Qin + q;
+            return Sin == 0 ? (long long)q * in_ss + (long long)ll * in_rs
+                            : split_slot(q, ll, Qin, Lin);
+        },
+        [&](int ll) {
+            return split_slot((long long)(ll & ((1 << k) - 1)) * Qin + q, ll >> k, Qout, Lout);
         },
         sR);
 }

@@ -47,7 +47,8 @@ inline int device_cu_count()
 // workspace (sub-system q = R*m0 + j holds rows R + 2^S*l of system j, element
 // l at q + l*m); x goes back to the caller's y in the caller's layout.
 struct SplitInfo {
-    int S;            // steps already done in HBM passes (0: not a split solve)
+    int S;            // steps already done in HBM passes (0: not a split solve);
+                      // S > 0: the sub-systems sit in the grouped layout (split_slot)
     int n0, m0;       // original system size and count
     int out_layout;   // caller's layout (PCRP_LAYOUT_*) for x
     double* x;        // caller's y (receives x); S == 0: x's array (y's indexing)
@@ -106,6 +107,23 @@ __host__ __device__ inline long long res_slot(int q, long long e, int s, int n)
     const int gl = res_group_log2(s);
     const long long lmax = ((long long)n + (1LL << s) - 1) >> s;
     return ((long long)(q >> gl) * lmax << gl) + (e << gl) + (q & ((1 << gl) - 1));
+}
+
+// The same grouping for the uniform split path: after s levels, Q = m0 2^s
+// sub-systems (q = R m0 + j: residue R of system j) of at most lmax rows;
+// groups of G consecutive sub-systems (G = 4, or 2 or 1 when Q is not a
+// multiple of 4) row-interleaved: element e of q at (q / G) G lmax + e G + q % G.
+// Slots lie in [0, Q lmax), which grows with s.
+__host__ __device__ inline long long split_slot(long long q, long long e, long long Q,
+                                                long long lmax)
+{
+    const int gl = (Q & 3) == 0 ? 2 : ((Q & 1) == 0 ? 1 : 0);
+    return ((q >> gl) * lmax << gl) + (e << gl) + (q & ((1 << gl) - 1));
+}
+
+__host__ __device__ inline int split_group_log2(long long Q)
+{
+    return (Q & 3) == 0 ? 2 : ((Q & 1) == 0 ? 1 : 0);
 }
 
 // Slots of the pass buffers one system of n rows split S levels takes.

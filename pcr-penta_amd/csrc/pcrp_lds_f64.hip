@@ -361,19 +361,25 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
     __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
 
     const int sys = xcd_remap(blockIdx.x, m);
-    const long long base = (long long)sys * ss;
+    long long base = (long long)sys * ss;
     // where x goes: in place (or to split.x at y's element), or (split path)
     // back to the caller's layout
     double* gx = split.x ? split.x : gy;
     long long xbase = base, xrs = rs;
     if (split.S > 0) {
-        // sub-system sys = R*m0 + j: rows R + 2^S*l of original system j
+        // sub-system sys = R*m0 + j: rows R + 2^S*l of original system j, in
+        // the grouped layout of the pass buffers (n: the longest sub-system)
         const int R = sys / split.m0, j = sys - R * split.m0;
+        base = split_slot(sys, 0, m, n);
+        rs = 1LL << split_group_log2(m);
         n = (split.n0 - R + (1 << split.S) - 1) >> split.S;
         gx = split.x;
         if (split.out_layout == 0) {   // contiguous j*n0 + i
             xbase = (long long)j * split.n0 + R;
             xrs = 1LL << split.S;
+        } else {                       // interleaved i*m0 + j = sys + l*m
+            xbase = sys;
+            xrs = m;
         }
     }
     lds_solve_one<KIND, NT, RPT>(ga, gb, gc, gd, ge, gy, gx, base, rs, xbase, xrs, n, steps, sR, sE,
