@@ -463,6 +463,26 @@ def test_uneven_deep_splits_bit_exact(gpu, oracle_lib):
         gpu.split_tune_set(*saved[:3])
 
 
+def test_uneven_grouped_residue_layout_bit_exact(gpu, oracle_lib):
+    """Deep splits at the production thresholds: systems around powers of two
+    up to 2^16 (S up to 7 at lds_max 512), so the pass buffers hold residue
+    groups of 2 and 4 with short last residues; several steps per pass and
+    both tile families (4 sub-systems per tile, 8 per tile). Bit-identical to
+    the oracle in every setting."""
+    saved = gpu.split_tune_get()
+    sizes = [1025, 1027, 2047, 2049, 3071, 4097, 8191, 8193, 12289, 16385, 32767, 40001, 65537]
+    sizes += [5, 700, 1024]
+    try:
+        for lds_max, k, shape in ((1024, 3, 0), (1024, 5, 2), (512, 3, 4), (512, 5, 6),
+                                  (256, 4, 0)):
+            gpu.split_tune_set(lds_max, k, shape)
+            _say("uneven-grouped", lds_max, k, shape)
+            x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 0, 60 + k)
+            assert np.array_equal(x, ref), (lds_max, k, shape)
+    finally:
+        gpu.split_tune_set(*saved[:3])
+
+
 def test_uneven_log_uniform_sample(gpu, oracle_lib):
     """First 400 systems of the uneven size draw (n log-uniform in [64, 131072], seed 1)."""
     sizes = gpu.uneven_log_uniform_sizes()[:400]
