@@ -200,14 +200,13 @@ __device__ __forceinline__ void lds_solve_one(
     lds_solve_regs<KIND, NT, RPT>(r, gx, xbase, xrs, n, steps, sR, sE, add);
 }
 
-// The solve proper, from rows already in registers. add (optional): x +
-// add[] is stored, at x's own element.
-template <int KIND, int NT, int RPT>
-__device__ __forceinline__ void lds_solve_regs(
-    Row<double> (&r)[RPT], double* gx, long long xbase, long long xrs, int n, int steps,
-    double (&sR)[6][NT * RPT],
-    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1],
-    const double* __restrict__ add)
+// The solve proper, from rows already in registers. store(i, x) receives the
+// solution of row i (called for every row i < n exactly once, by some
+// thread: a row's x and its final-stage partner's come from one thread).
+template <int KIND, int NT, int RPT, typename Store>
+__device__ __forceinline__ void lds_solve_regs_f(
+    Row<double> (&r)[RPT], int n, int steps, double (&sR)[6][NT * RPT],
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1], Store&& store)
 {
     const int t = threadIdx.x;
 
@@ -319,7 +318,6 @@ __device__ __forceinline__ void lds_solve_regs(
     for (int k = 0; k < RPT; ++k) {
         const int i = t + NT * k;
         if (i < n) {
-            const long long g = xbase + (long long)i * xrs;
             if (i < n - delta) {
                 double cii, bii, yii;
                 if (delta >= NT) {
@@ -339,15 +337,28 @@ __device__ __forceinline__ void lds_solve_regs(
                 const double deno = __builtin_fma(r[k].c, cii, -(bii * r[k].d));
                 const double x0 = __builtin_fma(cii, r[k].y, -(r[k].d * yii)) / deno;
                 const double x1 = __builtin_fma(r[k].c, yii, -(bii * r[k].y)) / deno;
-                const long long g1 = g + (long long)delta * xrs;
-                gx[g] = add ? x0 + add[g] : x0;
-                gx[g1] = add ? x1 + add[g1] : x1;
+                store(i, x0);
+                store(i + delta, x1);
             } else if (i < delta) {
-                const double x0 = r[k].y / r[k].c;
-                gx[g] = add ? x0 + add[g] : x0;
+                store(i, r[k].y / r[k].c);
             }
         }
     }
+}
+
+// x to gx[xbase + i*xrs]; add (optional): x + add[] is stored, at x's own
+// element.
+template <int KIND, int NT, int RPT>
+__device__ __forceinline__ void lds_solve_regs(
+    Row<double> (&r)[RPT], double* gx, long long xbase, long long xrs, int n, int steps,
+    double (&sR)[6][NT * RPT],
+    double (&sE)[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NT * RPT : 1],
+    const double* __restrict__ add)
+{
+    lds_solve_regs_f<KIND, NT, RPT>(r, n, steps, sR, sE, [&](int i, double x) {
+        const long long g = xbase + (long long)i * xrs;
+        gx[g] = add ? x + add[g] : x;
+    });
 }
 
 template <int KIND, int NT, int RPT>
@@ -460,6 +471,62 @@ __global__ __launch_bounds__(NT) void lds_f64_queue_kernel(
     }
 }
 
+// Two neighbouring systems j0, j0 + 1 of an interleaved batch per workgroup
+// (in place, m even, 16-byte aligned arrays). In the interleaved layout one
+// system's rows sit one per 128-byte line, so the load phase runs at the L1
+// line rate; here every line fetched serves both systems (16-byte loads and
+// stores). The workgroup solves them one after the other: the second system's
+// rows wait in registers, and each x goes through LDS column 0 (the final
+// stage writes partner rows' x from other threads and reads only columns 1, 2
+// and 5) to the thread that stores the pair.
+template <int KIND, int NT, int RPT>
+__global__ __launch_bounds__(NT) void lds_f64_pair_kernel(
+    const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
+    const double* __restrict__ gd, const double* __restrict__ ge, double* gy, int n, int m,
+    int steps)
+{
+    constexpr int NMAX = NT * RPT;
+    __shared__ double sR[6][NMAX];
+    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
+    const int t = threadIdx.x;
+    // 8 pairs = 16 systems = one 128-byte line of each row, kept on one XCD
+    const long long j0 = 2LL * xcd_remap_c<8>((int)blockIdx.x, m >> 1);
+    Row<double> rA[RPT], rB[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = t + NT * k;
+        if (i < n) {
+            const long long g = (long long)i * m + j0;
+            const double2 va = *reinterpret_cast<const double2*>(ga + g);
+            const double2 vb = *reinterpret_cast<const double2*>(gb + g);
+            const double2 vc = *reinterpret_cast<const double2*>(gc + g);
+            const double2 vd = *reinterpret_cast<const double2*>(gd + g);
+            const double2 ve = *reinterpret_cast<const double2*>(ge + g);
+            const double2 vy = *reinterpret_cast<const double2*>(gy + g);
+            rA[k] = Row<double>{va.x, vb.x, vc.x, vd.x, ve.x, vy.x};
+            rB[k] = Row<double>{va.y, vb.y, vc.y, vd.y, ve.y, vy.y};
+        } else {
+            rA[k] = Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+            rB[k] = rA[k];
+        }
+    }
+    const auto to_lds = [&](int i, double x) { sR[0][i] = x; };
+    lds_solve_regs_f<KIND, NT, RPT>(rA, n, steps, sR, sE, to_lds);
+    __syncthreads();
+    double xA[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) xA[k] = t + NT * k < n ? sR[0][t + NT * k] : 0.0;
+    __syncthreads();   // every xA read before the second system's rows go to LDS
+    lds_solve_regs_f<KIND, NT, RPT>(rB, n, steps, sR, sE, to_lds);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = t + NT * k;
+        if (i < n)
+            *reinterpret_cast<double2*>(gy + (long long)i * m + j0) = make_double2(xA[k], sR[0][i]);
+    }
+}
+
 // ---- variant table ---------------------------------------------------------
 
 template <int KIND, int NT, int RPT>
@@ -532,6 +599,30 @@ static int launch_persist(const SolveArgsF64& A, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int lds_f64_plain_variant_for_batch(int n, int m);
+
+// Paired form: interleaved in-place batches of even m with 16-byte aligned
+// arrays; every other batch (and split views) takes the one-system variant
+// this batch would get by default.
+template <int KIND, int NT, int RPT>
+static int launch_pair(const SolveArgsF64& A, hipStream_t st)
+{
+    const auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const bool pairable = A.split.S == 0 && A.split.x == A.y && A.split.add == nullptr &&
+                          A.ss == 1 && A.rs == A.m && (A.m & 1) == 0 && al16(A.a) && al16(A.b) &&
+                          al16(A.c) && al16(A.d) && al16(A.e) && al16(A.y);
+    if (!pairable) {
+        const LdsVariantF64* v = lds_f64_variant(lds_f64_plain_variant_for_batch(A.n, A.m));
+        return v ? v->launch(A, st) : -1;
+    }
+    hipLaunchKernelGGL((lds_f64_pair_kernel<KIND, NT, RPT>), dim3(A.m / 2), dim3(NT), 0, st, A.a,
+                       A.b, A.c, A.d, A.e, A.y, A.n, A.m, A.steps);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+#define PCRP_2(KIND, NT, RPT) \
+    {KIND + 8, NT, RPT, NT * RPT, &launch_pair<KIND, NT, RPT>, &launch_queue<KIND, NT, RPT>}
+
 #define PCRP_P(KIND, NT, RPT) \
     {KIND + 4, NT, RPT, NT * RPT, &launch_persist<KIND, NT, RPT>, &launch_queue<KIND, NT, RPT>}
 
@@ -560,9 +651,13 @@ static const LdsVariantF64 kVariants[] = {
     // persistent batch form with next-system prefetch (kind code + 4)
     PCRP_P(KIND_REDUNDANT, 256, 4), PCRP_P(KIND_REDUNDANT, 1024, 1), PCRP_P(KIND_REDUNDANT, 256, 2),
     PCRP_P(KIND_REDUNDANT, 256, 1), PCRP_P(KIND_REDUNDANT, 128, 4), PCRP_P(KIND_SHARED, 512, 1),
+    // two interleaved systems per workgroup (kind code + 8)
+    PCRP_2(KIND_REDUNDANT, 256, 4), PCRP_2(KIND_REDUNDANT, 256, 2), PCRP_2(KIND_REDUNDANT, 128, 4),
 };
 
 #undef PCRP_V
+#undef PCRP_P
+#undef PCRP_2
 
 int lds_f64_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 
@@ -595,20 +690,43 @@ int lds_f64_default_variant(int n)
     return best;
 }
 
-// Default for a whole batch of m systems: at 512 < n <= 1024 a batch of at
-// most two workgroups per CU is latency-bound, and one row per thread
-// (1024 threads) finishes each system sooner (MI355X, n = 1024: 5-10% faster
-// up to m = 512, 2-10% slower from m = 1024 on than 256 x 4).
-int lds_f64_default_variant_for_batch(int n, int m)
+static int find_variant(int kind, int nt, int rpt)
+{
+    for (int id = 0; id < lds_f64_variant_count(); ++id) {
+        const LdsVariantF64& v = kVariants[id];
+        if (v.kind == kind && v.nt == nt && v.rpt == rpt) return id;
+    }
+    return -1;
+}
+
+// One-system default for a whole batch of m systems: at 512 < n <= 1024 a
+// batch of at most two workgroups per CU is latency-bound, and one row per
+// thread (1024 threads) finishes each system sooner (MI355X, n = 1024: 5-10%
+// faster up to m = 512, 2-10% slower from m = 1024 on than 256 x 4).
+int lds_f64_plain_variant_for_batch(int n, int m)
 {
     const int cus = device_cu_count();
     if (n > 512 && n <= 1024 && cus > 0 && m <= 2 * cus) {
-        for (int id = 0; id < lds_f64_variant_count(); ++id) {
-            const LdsVariantF64& v = kVariants[id];
-            if (v.kind == KIND_REDUNDANT && v.nt == 1024 && v.rpt == 1) return id;
-        }
+        const int id = find_variant(KIND_REDUNDANT, 1024, 1);
+        if (id >= 0) return id;
     }
     return lds_f64_default_variant(n);
+}
+
+// Default for a whole batch: large batches of 256 < n <= 1024 take the paired
+// form (which runs the one-system default above when the batch is not an
+// even, aligned interleaved one). MI355X, m = 16384, interleaved: n = 1024
+// 0.957 -> 0.902 ms (256 x 4), n = 512 0.466 -> 0.403 ms (128 x 4); smaller
+// batches (m <= 1024) and n = 256 are slower paired (half the workgroups).
+int lds_f64_default_variant_for_batch(int n, int m)
+{
+    const int cus = device_cu_count();
+    if (cus > 0 && m >= 16 * cus && n > 256 && n <= 1024) {
+        const int id = n > 512 ? find_variant(KIND_REDUNDANT + 8, 256, 4)
+                               : find_variant(KIND_REDUNDANT + 8, 128, 4);
+        if (id >= 0) return id;
+    }
+    return lds_f64_plain_variant_for_batch(n, m);
 }
 
 }  // namespace pcrp

@@ -364,7 +364,8 @@ def variants(precision: int = PREC_F64):
         _check(lib.pcrp_lds_variant_info(precision, vid, *[ctypes.byref(v) for v in vals]),
                "pcrp_lds_variant_info")
         kind, nt, rpt, max_n, lds = (v.value for v in vals)
-        out.append({"id": vid, "kind": (("persistent-" if kind >= 4 else "") + ("shared", "redundant", "inplace")[kind & 3]), "nt": nt, "rpt": rpt,
+        prefix = "paired-" if kind >= 8 else ("persistent-" if kind >= 4 else "")
+        out.append({"id": vid, "kind": prefix + ("shared", "redundant", "inplace")[kind & 3], "nt": nt, "rpt": rpt,
                     "max_n": max_n, "lds_bytes": lds})
     return out
 

@@ -142,6 +142,37 @@ def test_every_lds_variant_bit_exact(gpu, oracle_lib):
             gpu.tune_set(n, -1)
 
 
+def test_paired_lds_variants_bit_exact(gpu, oracle_lib):
+    """The paired variants (two neighbouring interleaved systems per workgroup,
+    16-byte loads and stores) give the oracle's bits: even m on interleaved
+    batches (the paired kernel), and odd m, the contiguous layout and a
+    misaligned right-hand side (the one-system kernel they fall back to)."""
+    import torch
+    pv = [v for v in gpu.variants() if v["kind"].startswith("paired-")]
+    assert pv
+    sizes = (1, 2, 3, 100, 255, 256, 257, 511, 512, 513, 777, 1023, 1024)
+    try:
+        for n in sizes:
+            for m, layout, off in ((22, I, 0), (64, I, 0), (21, I, 0), (22, C, 0), (22, I, 1)):
+                _say("paired", n, m, layout, off)
+                dev = gpu.generate(0, 300 + n + m, n, m, layout)
+                ins = [_np(t).copy() for t in dev]
+                ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+                for v in pv:
+                    if v["max_n"] < n:
+                        continue
+                    gpu.tune_set(n, v["id"])
+                    buf = torch.zeros(n * m + 2, dtype=torch.float64, device="cuda")
+                    y = buf[off:off + n * m]
+                    y.copy_(torch.from_numpy(ins[5]))
+                    gpu.solve(*dev[:5], y, n, m, layout)
+                    torch.cuda.synchronize()
+                    assert np.array_equal(_np(y), ref), (n, m, layout, off, v)
+    finally:
+        for n in sizes:
+            gpu.tune_set(n, -1)
+
+
 def test_autotune_picks_a_fitting_variant(gpu, oracle_lib, tmp_path):
     import torch
     n, m = 300, 64
