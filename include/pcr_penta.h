@@ -346,8 +346,10 @@ int pcrp_init_t2(int n, int m, int layout,
  * ceil(log2 n) maps to one compiled LDS-kernel variant (kind, threads NT, rows
  * per thread RPT); the table starts at a built-in default and is set by
  * timing every variant on the device. kind: 0 shared pair equations, 1
- * redundant (reference form), 2 in-place pair equations; +4 = persistent
- * batch form that prefetches the next system. */
+ * redundant (reference form), 2 in-place pair equations, 3 spaced rows (each
+ * pair equation serves two of a thread's rows); +4 = persistent batch form
+ * that prefetches the next system; +8 = two interleaved systems per
+ * workgroup. */
 int pcrp_lds_variant_count(int precision);
 int pcrp_lds_variant_info(int precision, int id, int *kind, int *nt, int *rpt,
                           int *max_n, int *lds_bytes);

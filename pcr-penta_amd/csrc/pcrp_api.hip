@@ -135,6 +135,13 @@ int lds_tuned_variant(int prec, int n, int m)
     if (b < kBuckets && tuned_get(prec, b) >= 0) return lds_tuned_variant(prec, n);
     return lds_f64_default_variant_for_batch(n, m);
 }
+
+int lds_tuned_queue_variant(int prec, int n)
+{
+    const int b = bucket_of(n);
+    if (b < kBuckets && tuned_get(prec, b) >= 0) return lds_tuned_variant(prec, n);
+    return lds_f64_queue_default_variant(n);
+}
 }  // namespace pcrp
 
 extern "C" {

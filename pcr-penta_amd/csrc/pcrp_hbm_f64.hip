@@ -379,7 +379,7 @@ int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes
     SolveArgsF64 F{in[0], in[1], in[2], in[3], in[4], const_cast<double*>(in[5]),
                    (n0 + (1 << S) - 1) >> S, Qin, in_rs, in_ss, A.steps - S,
                    SplitInfo{S, n0, m0, layout, A.y}};
-    const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, F.n));
+    const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, F.n, F.m));
     return v->launch(F, st);
 }
 

@@ -133,7 +133,8 @@ __host__ __device__ inline long long res_region(int n, int S)
 }
 
 // A compiled LDS-kernel variant: kind (0 shared pair equations, 1 redundant,
-// 2 in-place pair equations; +4 for the persistent prefetching batch form),
+// 2 in-place pair equations, 3 spaced rows; +4 for the persistent prefetching
+// batch form, +8 for two interleaved systems per workgroup),
 // NT threads x RPT rows per thread, capacity max_n = NT*RPT rows.
 struct LdsVariantF64 {
     int kind, nt, rpt, max_n;
@@ -148,6 +149,7 @@ int lds_f64_variant_count();
 const LdsVariantF64* lds_f64_variant(int id);
 int lds_f64_default_variant(int n);
 int lds_f64_default_variant_for_batch(int n, int m);
+int lds_f64_queue_default_variant(int n);
 int lds_f64_lds_bytes(const LdsVariantF64& v);
 int lds_f64_max_n();
 
@@ -173,6 +175,8 @@ struct SolveArgsF32R {
 int lds_tuned_variant(int prec, int n);
 // The same for a whole batch: without a tuned entry, the batch-size-aware default.
 int lds_tuned_variant(int prec, int n, int m);
+// The same for the uneven work queue (jobs of every size up to n).
+int lds_tuned_queue_variant(int prec, int n);
 
 // Split path tunables: sub-systems up to lds_max rows go to the LDS kernel;
 // HBM passes fuse up to k_fuse steps; shape indexes the pass tile shapes.

@@ -27,6 +27,10 @@
 //             (detected block-wide) runs as a redundant step instead, whose
 //             rows are still intact, because the reference's zero-pivot rule
 //             needs the neighbours' multipliers.
+//   SPACED  : at each LDS stride a thread owns RPT rows two strides apart, so
+//             its RPT + 1 pair equations serve all its rows (4.5 divisions per
+//             row-step at RPT = 4) and it reads 2*RPT + 3 rows; the rows stay
+//             in LDS between steps in a per-stride layout (spaced_step).
 // All give the same bits (the pair equation is the same expression on the
 // same operands either way).
 #include "pcrp_device.h"
@@ -34,7 +38,7 @@
 
 namespace pcrp {
 
-enum { KIND_SHARED = 0, KIND_REDUNDANT = 1, KIND_INPLACE = 2 };
+enum { KIND_SHARED = 0, KIND_REDUNDANT = 1, KIND_INPLACE = 2, KIND_SPACED = 3 };
 
 template <int KIND>
 struct LdsWords {
@@ -152,6 +156,113 @@ __device__ __forceinline__ bool inplace_zero_pivot_step(Row<double> (&r)[RPT], i
     return false;
 }
 
+// ---- SPACED steps ------------------------------------------------------------
+// At stride d = 2^S < NT the rows split into d chains (residue r = i mod d);
+// chain positions p (row r + d*p) are cut into blocks of 2*RPT, and a thread
+// owns the RPT positions of one parity in one block: p = P0 + 2k with
+// P0 = par + 2*RPT*b. Its rows' pair equations are centred on the odd
+// positions P0 - 1, P0 + 1, ..., P0 + 2*RPT - 1, and each of those RPT + 1
+// centres serves two of its rows (as E(i+d) of one and E(i-d) of the next),
+// so a row-step costs (RPT+1)/RPT pair equations instead of 2:
+// 2 + 2(RPT+1)/RPT divisions (4.5 at RPT = 4) and 2*RPT + 3 rows read from
+// LDS per RPT rows. The same pair-equation and reduction expressions run on
+// the same operands, so the bits are those of the other forms.
+//
+// Layout (per stride): thread t = r + d*(par + 2b) owns LDS slot t, and its
+// row k sits at sR[c][k*NT + t]. The partner thread (other parity, same
+// block) is slot t ^ d; the outer rows are rows of slots t -+ 2d; so every
+// lane reads one slot per instruction and the 64 lanes of a wave read 64
+// different banks. A step writes its new rows straight into the layout of
+// stride 2d (or in natural order, row i at i, after the last LDS step).
+// Slots of rows >= n hold the identity row (0, 0, 1, 0, 0, 0), so every row
+// read is finite; outer neighbours past either end of a chain are replaced by
+// the thread's own first / last row, as the redundant form does.
+__device__ __forceinline__ Row<double> identity_row() { return Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0}; }
+
+// LDS address of row i in the spaced layout of stride 2^S (RPT rows per slot).
+template <int NT, int RPT>
+__device__ __forceinline__ int spaced_addr(int i, int S)
+{
+    const int d = 1 << S;
+    const int r = i & (d - 1), p = i >> S;
+    const int b = p / (2 * RPT), q = p & (2 * RPT - 1);
+    return (q >> 1) * NT + r + d * ((q & 1) + 2 * b);
+}
+
+template <int NT, int RPT>
+__device__ __forceinline__ void spaced_step(int n, int S, bool natural, double (&sR)[6][NT * RPT])
+{
+    static_assert(RPT >= 2 && RPT % 2 == 0, "spaced steps need an even number of rows per thread");
+    const int t = threadIdx.x;
+    const int d = 1 << S;
+    const int r = t & (d - 1), par = (t >> S) & 1, b = t >> (S + 1);
+    const int NB = NT >> (S + 1);
+    const int P0 = par + 2 * RPT * b;
+    const int Lr = (n - r + d - 1) >> S;   // positions of chain r holding rows < n
+    auto rd = [&](int a) {
+        Row<double> q;
+        q.a = sR[0][a]; q.b = sR[1][a]; q.c = sR[2][a];
+        q.d = sR[3][a]; q.e = sR[4][a]; q.y = sR[5][a];
+        return q;
+    };
+    // odd positions P0 + 2m - 1: m = 1..RPT-1 from the partner at bo + m*NT;
+    // m = 0 and m = RPT may lie in the neighbouring block's partner slot
+    const int bo = par ? t - d : t + d - NT;
+    const int a_o0 = par ? t - d : (b > 0 ? (RPT - 1) * NT + t - d : t);
+    const int a_on = par ? (b < NB - 1 ? t + d : (RPT - 1) * NT + t) : (RPT - 1) * NT + t + d;
+    const int a_l2 = b > 0 ? (RPT - 1) * NT + t - 2 * d : t;        // position P0 - 2
+    const int a_r2 = b < NB - 1 ? t + 2 * d : (RPT - 1) * NT + t;   // position P0 + 2*RPT
+    Row<double> N[RPT];
+    if (P0 < Lr) {   // a thread whose rows are all >= n only writes identity rows
+        Pair<double> Ep;
+        Row<double> L = rd(a_l2);
+#pragma unroll
+        for (int m = 0; m <= RPT; ++m) {
+            const Row<double> M = rd(m == 0 ? a_o0 : (m == RPT ? a_on : bo + m * NT));
+            const Row<double> R = rd(m == RPT ? a_r2 : m * NT + t);
+            const int pc = P0 + 2 * m - 1;
+            const Pair<double> E = make_pair(L, M, R, pc >= 1, pc >= 2, pc + 1 < Lr, pc + 2 < Lr);
+            if (m > 0) {
+                const int po = pc - 1;   // own row m - 1, held in L
+                N[m - 1] = reduce_row(L, Ep, E, po >= 1, po + 1 < Lr, po >= 2, po + 2 < Lr,
+                                      [&](double& alL, double& arL, double& alR, double& arR) {
+                                          alL = Ep.al; arL = Ep.ar; alR = E.al; arR = E.ar;
+                                      });
+            }
+            Ep = E;
+            set_row(L, R);
+        }
+    }
+    __syncthreads();   // every read of this step done
+    auto wr = [&](int a, const Row<double>& q) {
+        sR[0][a] = q.a; sR[1][a] = q.b; sR[2][a] = q.c;
+        sR[3][a] = q.d; sR[4][a] = q.e; sR[5][a] = q.y;
+    };
+    if (natural) {
+        const int base = r + d * P0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (P0 + 2 * k < Lr) wr(base + 2 * d * k, N[k]);
+    } else {
+        // row r + d*(P0 + 2k) at stride 2d: residue r + d*par, block b/2,
+        // parity k&1, slot row (RPT/2)*(b&1) + k/2
+        const int w0 = (RPT / 2) * (b & 1) * NT + r + d * par + 4 * d * (b >> 1);
+        const int w1 = w0 + 2 * d;
+        if (__builtin_expect(P0 + 2 * (RPT - 1) < Lr, 1)) {   // all RPT rows < n
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) wr(((k & 1) ? w1 : w0) + (k >> 1) * NT, N[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int a = ((k & 1) ? w1 : w0) + (k >> 1) * NT;
+                if (P0 + 2 * k < Lr) wr(a, N[k]);
+                else wr(a, identity_row());
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Solve one system (or sub-system) held at ga..gy[base + i*rs], i < n, with
 // `steps` reduction steps; x goes to gx[xbase + i*xrs]. All threads of the
 // workgroup call this together (it contains barriers).
@@ -224,13 +335,30 @@ __device__ __forceinline__ void lds_solve_regs_f(
     // Steps with stride < NT exchange rows through LDS.
     int s = 0, delta = 1;
     const bool any_lds = steps > 0 && NT > 1;
-    if (any_lds) {
+    if constexpr (KIND == KIND_SPACED) {
+        if (any_lds) {
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int i = t + NT * k;
+                writeR(spaced_addr<NT, RPT>(i, 0), i < n ? r[k] : identity_row());
+            }
+            __syncthreads();
+            for (; s < steps && delta < NT; ++s, delta <<= 1)
+                spaced_step<NT, RPT>(n, s, s + 1 == steps || 2 * delta >= NT, sR);
+            // back to one thread per row i = t + NT*k (rows in natural order now)
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int i = t + NT * k;
+                r[k] = i < n ? readR(i) : identity_row();
+            }
+        }
+    } else if (any_lds) {
 #pragma unroll
         for (int k = 0; k < RPT; ++k)
             if (t + NT * k < n) writeR(t + NT * k, r[k]);
         __syncthreads();
     }
-    for (; s < steps && delta < NT; ++s, delta <<= 1) {
+    for (; KIND != KIND_SPACED && s < steps && delta < NT; ++s, delta <<= 1) {
         if constexpr (KIND == KIND_SHARED) {
             // Reduction-I/II once per centre j (kernel.cu:69-117).
 #pragma unroll
@@ -653,6 +781,10 @@ static const LdsVariantF64 kVariants[] = {
     PCRP_P(KIND_REDUNDANT, 256, 1), PCRP_P(KIND_REDUNDANT, 128, 4), PCRP_P(KIND_SHARED, 512, 1),
     // two interleaved systems per workgroup (kind code + 8)
     PCRP_2(KIND_REDUNDANT, 256, 4), PCRP_2(KIND_REDUNDANT, 256, 2), PCRP_2(KIND_REDUNDANT, 128, 4),
+    // rows two strides apart per thread, rows in LDS between steps
+    PCRP_V(KIND_SPACED, 256, 4),    PCRP_V(KIND_SPACED, 128, 8),    PCRP_V(KIND_SPACED, 512, 2),
+    PCRP_V(KIND_SPACED, 256, 2),    PCRP_V(KIND_SPACED, 128, 4),    PCRP_V(KIND_SPACED, 64, 4),
+    PCRP_2(KIND_SPACED, 256, 4),    PCRP_2(KIND_SPACED, 128, 4),
 };
 
 #undef PCRP_V
@@ -674,22 +806,6 @@ int lds_f64_lds_bytes(const LdsVariantF64& v)
                                                 : LdsWords<KIND_REDUNDANT>::value);   // INPLACE: 6 too
 }
 
-// Default choice before tuning (measured on MI355X at n = 1024): the
-// redundant form with 256 threads per workgroup and as many rows per thread as
-// n needs (up to 4); larger workgroups for n > 1024 never occur here.
-int lds_f64_default_variant(int n)
-{
-    int best = -1, best_score = 1 << 30;
-    for (int id = 0; id < lds_f64_variant_count(); ++id) {
-        const LdsVariantF64& v = kVariants[id];
-        if (v.kind != KIND_REDUNDANT || v.max_n < n) continue;
-        // tightest capacity first, then the workgroup size closest to 256
-        const int score = v.max_n * 4 + (v.nt > 256 ? v.nt / 256 : 256 / v.nt);
-        if (score < best_score) { best = id; best_score = score; }
-    }
-    return best;
-}
-
 static int find_variant(int kind, int nt, int rpt)
 {
     for (int id = 0; id < lds_f64_variant_count(); ++id) {
@@ -699,31 +815,80 @@ static int find_variant(int kind, int nt, int rpt)
     return -1;
 }
 
-// One-system default for a whole batch of m systems: at 512 < n <= 1024 a
-// batch of at most two workgroups per CU is latency-bound, and one row per
-// thread (1024 threads) finishes each system sooner (MI355X, n = 1024: 5-10%
-// faster up to m = 512, 2-10% slower from m = 1024 on than 256 x 4).
+// The redundant form with 256 threads per workgroup and as many rows per
+// thread as n needs (tightest capacity first, then the workgroup size closest
+// to 256). Default of the uneven work queue, whose jobs have every size up to
+// its capacity: the spaced form works on all its rows whatever n is, and on
+// the uneven batch of 10000 systems (MI355X) it is 2.5% slower.
+int lds_f64_queue_default_variant(int n)
+{
+    int best = -1, best_score = 1 << 30;
+    for (int id = 0; id < lds_f64_variant_count(); ++id) {
+        const LdsVariantF64& v = kVariants[id];
+        if (v.kind != KIND_REDUNDANT || v.max_n < n) continue;
+        const int score = v.max_n * 4 + (v.nt > 256 ? v.nt / 256 : 256 / v.nt);
+        if (score < best_score) { best = id; best_score = score; }
+    }
+    return best;
+}
+
+// Default for systems of n rows, batch size unknown: the spaced form when n
+// nearly fills 256 x 4 (its rows past n still cost steps), else the redundant
+// form above.
+int lds_f64_default_variant(int n)
+{
+    if (n > 832 && n <= 1024) {
+        const int id = find_variant(KIND_SPACED, 256, 4);
+        if (id >= 0) return id;
+    }
+    return lds_f64_queue_default_variant(n);
+}
+
+// One-system default for a whole batch of m systems (MI355X, FP64, every
+// candidate warmed and timed interleaved):
+//   * 832 < n <= 1024: spaced 256 x 4 from m = 2 x CUs on (n = 1024, m = 16384:
+//     0.846 vs 0.935 ms interleaved, 0.680 vs 0.814 ms contiguous; m = 512:
+//     0.039 vs 0.042 ms); smaller batches are latency-bound and take one row
+//     per thread (1024 threads);
+//   * 512 < n <= 832, m <= 2 x CUs: one row per thread as well (5-10% faster
+//     than 256 x 4 up to m = 512);
+//   * 448 < n <= 512: spaced 128 x 4 from m = 16 x CUs (contiguous m = 16384:
+//     0.313 vs 0.372 ms), spaced 256 x 2 below (m = 2048: 0.051-0.057 vs
+//     0.053-0.061 ms);
+//   * 224 < n <= 256, m >= 8 x CUs: spaced 64 x 4 (m = 16384: 0.137 vs 0.172
+//     ms contiguous, 0.169 vs 0.185 ms interleaved).
+// Sizes in between fill the spaced tiles too little to gain.
 int lds_f64_plain_variant_for_batch(int n, int m)
 {
     const int cus = device_cu_count();
-    if (n > 512 && n <= 1024 && cus > 0 && m <= 2 * cus) {
-        const int id = find_variant(KIND_REDUNDANT, 1024, 1);
-        if (id >= 0) return id;
+    int id = -1;
+    if (cus > 0) {
+        if (n > 832 && n <= 1024)
+            id = m >= 2 * cus ? find_variant(KIND_SPACED, 256, 4) : find_variant(KIND_REDUNDANT, 1024, 1);
+        else if (n > 512 && n <= 832 && m <= 2 * cus)
+            id = find_variant(KIND_REDUNDANT, 1024, 1);
+        else if (n > 448 && n <= 512)
+            id = m >= 16 * cus ? find_variant(KIND_SPACED, 128, 4) : find_variant(KIND_SPACED, 256, 2);
+        else if (n > 224 && n <= 256 && m >= 8 * cus)
+            id = find_variant(KIND_SPACED, 64, 4);
     }
-    return lds_f64_default_variant(n);
+    return id >= 0 ? id : lds_f64_default_variant(n);
 }
 
-// Default for a whole batch: large batches of 256 < n <= 1024 take the paired
-// form (which runs the one-system default above when the batch is not an
-// even, aligned interleaved one). MI355X, m = 16384, interleaved: n = 1024
-// 0.957 -> 0.902 ms (256 x 4), n = 512 0.466 -> 0.403 ms (128 x 4); smaller
-// batches (m <= 1024) and n = 256 are slower paired (half the workgroups).
+// Default for a whole batch: large interleaved batches of 256 < n <= 832 take
+// the redundant paired form and 448 < n <= 512 the spaced paired form (two
+// systems per workgroup share every 128-byte line of the load; a batch that
+// cannot pair runs the one-system default above). MI355X, m = 16384,
+// interleaved: n = 512 0.397 ms paired-spaced 128 x 4 vs 0.415 spaced 128 x 4;
+// n = 640-768 0.60-0.68 ms paired-redundant vs 0.62-0.71 one system; from
+// n = 896 the spaced one-system form is faster (0.77-0.85 vs 0.80-0.89 ms).
 int lds_f64_default_variant_for_batch(int n, int m)
 {
     const int cus = device_cu_count();
-    if (cus > 0 && m >= 16 * cus && n > 256 && n <= 1024) {
+    if (cus > 0 && m >= 16 * cus && n > 256 && n <= 832) {
         const int id = n > 512 ? find_variant(KIND_REDUNDANT + 8, 256, 4)
-                               : find_variant(KIND_REDUNDANT + 8, 128, 4);
+                               : (n > 448 ? find_variant(KIND_SPACED + 8, 128, 4)
+                                          : find_variant(KIND_REDUNDANT + 8, 128, 4));
         if (id >= 0) return id;
     }
     return lds_f64_plain_variant_for_batch(n, m);

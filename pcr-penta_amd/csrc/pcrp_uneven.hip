@@ -270,7 +270,7 @@ int launch_plan(const UnevenPlan& P, double* y, void* ws, hipStream_t st)
     const int* qstart = reinterpret_cast<const int*>(w + P.off_counter + kQueues * sizeof(unsigned int));
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, st, counters);
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
-    const LdsVariantF64* v = lds_f64_variant(lds_tuned_variant(PCRP_PREC_F64, split_f64_get().lds_max));
+    const LdsVariantF64* v = lds_f64_variant(lds_tuned_queue_variant(PCRP_PREC_F64, split_f64_get().lds_max));
     rc = v->launch_queue(dsets, y, reinterpret_cast<const LdsJob*>(w + P.off_jobs),
                          (int)P.jobs.size(), qstart, counters, st);
     return rc == 0 ? PCRP_OK : PCRP_ERR_HIP;

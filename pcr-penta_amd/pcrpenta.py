@@ -365,7 +365,7 @@ def variants(precision: int = PREC_F64):
                "pcrp_lds_variant_info")
         kind, nt, rpt, max_n, lds = (v.value for v in vals)
         prefix = "paired-" if kind >= 8 else ("persistent-" if kind >= 4 else "")
-        out.append({"id": vid, "kind": prefix + ("shared", "redundant", "inplace")[kind & 3], "nt": nt, "rpt": rpt,
+        out.append({"id": vid, "kind": prefix + ("shared", "redundant", "inplace", "spaced")[kind & 3], "nt": nt, "rpt": rpt,
                     "max_n": max_n, "lds_bytes": lds})
     return out
 

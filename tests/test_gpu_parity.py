@@ -142,6 +142,38 @@ def test_every_lds_variant_bit_exact(gpu, oracle_lib):
             gpu.tune_set(n, -1)
 
 
+def test_spaced_lds_variants_bit_exact(gpu, oracle_lib):
+    """The spaced variants (a thread's rows two strides apart, each pair
+    equation serving two of them, rows re-laid out in LDS every step) give the
+    oracle's bits at every size from 1 to 70 and around every power of two up
+    to 1024 (chains of every length, ragged last blocks, rows past n in the
+    last blocks), in both layouts and for even and odd batches (the paired
+    kernel and its one-system fallback)."""
+    import torch
+    sv = [v for v in gpu.variants() if v["kind"].endswith("spaced")]
+    assert len(sv) >= 2
+    sizes = sorted(set(list(range(1, 71)) + [95, 96, 97, 127, 128, 129, 191, 255, 256, 257, 383,
+                                               511, 512, 513, 640, 767, 768, 769, 1000, 1023, 1024]))
+    try:
+        for n in sizes:
+            for m, layout in ((10, I), (9, C)):
+                _say("spaced", n, m, layout)
+                dev = gpu.generate(0, 500 + n, n, m, layout)
+                ins = [_np(t).copy() for t in dev]
+                ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+                for v in sv:
+                    if v["max_n"] < n:
+                        continue
+                    gpu.tune_set(n, v["id"])
+                    y = torch.from_numpy(ins[5]).cuda()
+                    gpu.solve(*dev[:5], y, n, m, layout)
+                    torch.cuda.synchronize()
+                    assert np.array_equal(_np(y), ref), (n, m, layout, v)
+    finally:
+        for n in sizes:
+            gpu.tune_set(n, -1)
+
+
 def test_paired_lds_variants_bit_exact(gpu, oracle_lib):
     """The paired variants (two neighbouring interleaved systems per workgroup,
     16-byte loads and stores) give the oracle's bits: even m on interleaved
