@@ -218,6 +218,9 @@ __device__ __forceinline__ void spaced_step(int n, int S, bool natural, double (
         Row<double> L = rd(a_l2);
 #pragma unroll
         for (int m = 0; m <= RPT; ++m) {
+            // E(m) serves rows m - 1 and m; once row m - 1 is past n, so is
+            // every later row (their slots get identity rows below)
+            if (m > 0 && P0 + 2 * (m - 1) >= Lr) break;
             const Row<double> M = rd(m == 0 ? a_o0 : (m == RPT ? a_on : bo + m * NT));
             const Row<double> R = rd(m == RPT ? a_r2 : m * NT + t);
             const int pc = P0 + 2 * m - 1;
@@ -833,11 +836,11 @@ int lds_f64_queue_default_variant(int n)
 }
 
 // Default for systems of n rows, batch size unknown: the spaced form when n
-// nearly fills 256 x 4 (its rows past n still cost steps), else the redundant
-// form above.
+// fills most of 256 x 4 (past n it skips a thread's trailing rows, but every
+// workgroup still walks all strides), else the redundant form above.
 int lds_f64_default_variant(int n)
 {
-    if (n > 832 && n <= 1024) {
+    if (n > 736 && n <= 1024) {
         const int id = find_variant(KIND_SPACED, 256, 4);
         if (id >= 0) return id;
     }
@@ -846,11 +849,12 @@ int lds_f64_default_variant(int n)
 
 // One-system default for a whole batch of m systems (MI355X, FP64, every
 // candidate warmed and timed interleaved):
-//   * 832 < n <= 1024: spaced 256 x 4 from m = 2 x CUs on (n = 1024, m = 16384:
+//   * 736 < n <= 1024: spaced 256 x 4 from m = 2 x CUs on (n = 1024, m = 16384:
 //     0.846 vs 0.935 ms interleaved, 0.680 vs 0.814 ms contiguous; m = 512:
-//     0.039 vs 0.042 ms); smaller batches are latency-bound and take one row
-//     per thread (1024 threads);
-//   * 512 < n <= 832, m <= 2 x CUs: one row per thread as well (5-10% faster
+//     0.039 vs 0.042 ms; n = 768: 0.60-0.69 vs 0.61-0.71 ms; n = 700 is 2-3%
+//     slower); smaller batches are latency-bound and take one row per thread
+//     (1024 threads);
+//   * 512 < n <= 736, m <= 2 x CUs: one row per thread as well (5-10% faster
 //     than 256 x 4 up to m = 512);
 //   * 448 < n <= 512: spaced 128 x 4 from m = 16 x CUs (contiguous m = 16384:
 //     0.313 vs 0.372 ms), spaced 256 x 2 below (m = 2048: 0.051-0.057 vs
@@ -863,9 +867,9 @@ int lds_f64_plain_variant_for_batch(int n, int m)
     const int cus = device_cu_count();
     int id = -1;
     if (cus > 0) {
-        if (n > 832 && n <= 1024)
+        if (n > 736 && n <= 1024)
             id = m >= 2 * cus ? find_variant(KIND_SPACED, 256, 4) : find_variant(KIND_REDUNDANT, 1024, 1);
-        else if (n > 512 && n <= 832 && m <= 2 * cus)
+        else if (n > 512 && n <= 736 && m <= 2 * cus)
             id = find_variant(KIND_REDUNDANT, 1024, 1);
         else if (n > 448 && n <= 512)
             id = m >= 16 * cus ? find_variant(KIND_SPACED, 128, 4) : find_variant(KIND_SPACED, 256, 2);
