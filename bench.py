@@ -79,15 +79,17 @@ def parse():
     ap.add_argument("--tune-save", default=None, help="write the tuning table to this file")
     ap.add_argument("--tune-load", default=None,
                     help="load a tuning table (implies --no-tune)")
-    ap.add_argument("--split", default=None, metavar="K_FUSE,SHAPE",
-                    help="fix the split path's steps per pass and pass tile shape "
-                         "(implies --no-tune for the split path)")
+    ap.add_argument("--split", default=None, metavar="[LDS_MAX,]K_FUSE,SHAPE",
+                    help="fix the split path's steps per pass and pass tile shape, and "
+                         "optionally its LDS/HBM threshold (implies --no-tune for the split path)")
     ap.add_argument("--win-levels", type=int, default=None, metavar="L",
                     help="fix the FP32 refinement's window level count (2..7)")
     args = ap.parse_args()
     if args.split:
-        kf, sh = (int(v) for v in args.split.split(","))
-        args.split = (kf, sh)
+        vals = tuple(int(v) for v in args.split.split(","))
+        if len(vals) not in (2, 3):
+            ap.error("--split takes K_FUSE,SHAPE or LDS_MAX,K_FUSE,SHAPE")
+        args.split = vals
         args.no_tune = True
     if args.tune_load:
         args.no_tune = True
@@ -178,8 +180,10 @@ def roofline(unknowns, solve_s, workload, note):
 def apply_split(args):
     import pcrpenta
     if args.split:
-        lds_max = pcrpenta.split_tune_get()[0]
-        pcrpenta.split_tune_set(lds_max, *args.split)
+        if len(args.split) == 3:
+            pcrpenta.split_tune_set(*args.split)
+        else:
+            pcrpenta.split_tune_set(pcrpenta.split_tune_get()[0], *args.split)
     if args.win_levels is not None:
         pcrpenta.f32r_tune_set(args.win_levels, pcrpenta.f32r_tune_get()[1])
 

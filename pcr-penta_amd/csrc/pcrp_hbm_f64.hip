@@ -30,6 +30,7 @@
 #include "pcr_penta.h"
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
+#include "pcrp_spaced.h"
 
 #include <mutex>
 
@@ -42,7 +43,10 @@ namespace pcrp {
 // RPT rows are independent work for the same division latency.
 // in_at(l) / out_at(l) give the element index of row l of this sub-system in
 // the input arrays and of its destination in the output arrays.
-template <typename T, int TB, int G, int RPT, typename InAt, typename OutAt>
+// SP: the k steps in the spaced form (pcrp_spaced.h; FP64, RPT even, 2^k at
+// most the TB/RPT threads of a sub-system), else each row builds both its
+// pair equations (kernel.cu:66-140 as written).
+template <typename T, int TB, int G, int RPT, bool SP, typename InAt, typename OutAt>
 __device__ __forceinline__ void pass_tile(const T* const (&in)[6], T* const (&out)[6],
                                           bool qvalid, int l0, int L, int k, int H, int TI,
                                           InAt in_at, OutAt out_at, T (&sR)[6][TB * G])
@@ -73,10 +77,24 @@ __device__ __forceinline__ void pass_tile(const T* const (&in)[6], T* const (&ou
         sR[0][i] = w.a; sR[1][i] = w.b; sR[2][i] = w.c;
         sR[3][i] = w.d; sR[4][i] = w.e; sR[5][i] = w.y;
     };
+    if constexpr (SP) {
+        // rows to the spaced layout of stride 1 (invalid rows are identity rows)
 #pragma unroll
-    for (int kk = 0; kk < RPT; ++kk) writeR(tid + NTH * kk, r[kk]);
-    __syncthreads();
-    for (int s = 0; s < k; ++s) {
+        for (int kk = 0; kk < RPT; ++kk)
+            writeR(spaced_addr<RSTEP, RPT>(lr0 + RSTEP * kk, 0) * G + g, r[kk]);
+        __syncthreads();
+        const int lofs = l0 - lr0;   // sub-system row of tile row 0
+        const int Lq = qvalid ? L : 0;
+        for (int s = 0; s < k; ++s) spaced_step<RSTEP, RPT, G>(s, s + 1 == k, lofs, Lq, sR);
+        // (rows outside the sub-system are not stored: any value will do)
+#pragma unroll
+        for (int kk = 0; kk < RPT; ++kk) r[kk] = readR(tid + NTH * kk);
+    }
+#pragma unroll
+    for (int kk = 0; kk < RPT; ++kk)
+        if (!SP) writeR(tid + NTH * kk, r[kk]);
+    if (!SP) __syncthreads();
+    for (int s = 0; !SP && s < k; ++s) {
         const int delta = 1 << s;
         Row<T> N[RPT];
 #pragma unroll
@@ -144,7 +162,7 @@ __host__ __forceinline__ long long blocks_for(int groups, int tiles)
 }
 
 // Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
-template <typename T, int TB, int G, int RPT>
+template <typename T, int TB, int G, int RPT, bool SP>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const T* __restrict__ ia, const T* __restrict__ ib, const T* __restrict__ ic,
     const T* __restrict__ id, const T* __restrict__ ie, const T* __restrict__ iy,
@@ -168,7 +186,7 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const long long Lout = ((long long)n0 + (1LL << (Sin + k)) - 1) >> (Sin + k);
     const T* const in[6] = {ia, ib, ic, id, ie, iy};
     T* const out[6] = {oa, ob, oc, od, oe, oy};
-    pass_tile<T, TB, G, RPT>(
+    pass_tile<T, TB, G, RPT, SP>(
         in, out, q < Qin, l0, L, k, H, TI,
         [&](int ll) {
             return Sin == 0 ? (long long)q * in_ss + (long long)ll * in_rs
@@ -189,7 +207,7 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
 // tab lists the systems that still need passes in this round; bmap gives
 // every block its entry of tab (a binary search over the entries, 13
 // dependent loads at 6000 systems, delayed every block's first loads).
-template <int TB, int G, int RPT>
+template <int TB, int G, int RPT, bool SP>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     const double* const* __restrict__ sets, int src, int dst, const PassSys* __restrict__ tab,
     const int* __restrict__ bmap, int Sin)
@@ -220,10 +238,12 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     auto out_at = [&](int ll) {
         return P.woff + res_slot(q + ((ll & ((1 << k) - 1)) << Sin), ll >> k, Sout, P.n);
     };
-    pass_tile<double, TB, G, RPT>(in, out, q < Qin, l0, L, k, H, T, in_at, out_at, sR);
+    pass_tile<double, TB, G, RPT, SP>(in, out, q < Qin, l0, L, k, H, T, in_at, out_at, sR);
 }
 
-template <typename T, int TB, int G, int RPT>
+// SP shapes run the spaced form when 2^k fits the TB/RPT threads of a
+// sub-system, the redundant form of the same shape otherwise.
+template <typename T, int TB, int G, int RPT, bool SP>
 static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, int Sin,
                        int k, long long in_rs, long long in_ss, int Qin, hipStream_t st)
 {
@@ -235,20 +255,33 @@ static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, in
     const int groups = (Qin + G - 1) / G;
     const long long blocks = blocks_for(groups, tiles);
     if (blocks > 0x7fffffffLL) return -2;
-    hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT),
-                       0, st, in[0], in[1], in[2], in[3], in[4], in[5], out[0], out[1], out[2],
-                       out[3], out[4], out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles);
+    if (SP && (1 << k) <= TB / RPT)
+        hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT, SP>), dim3((unsigned)blocks),
+                           dim3(TB * G / RPT), 0, st, in[0], in[1], in[2], in[3], in[4], in[5],
+                           out[0], out[1], out[2], out[3], out[4], out[5], n0, m0, Sin, k, in_rs,
+                           in_ss, Qin, tiles);
+    else
+        hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT, false>), dim3((unsigned)blocks),
+                           dim3(TB * G / RPT), 0, st, in[0], in[1], in[2], in[3], in[4], in[5],
+                           out[0], out[1], out[2], out[3], out[4], out[5], n0, m0, Sin, k, in_rs,
+                           in_ss, Qin, tiles);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int TB, int G, int RPT>
+// kmax: the most steps any system of this round fuses (spaced form only
+// when 2^kmax fits the TB/RPT threads of a sub-system).
+template <int TB, int G, int RPT, bool SP>
 static int launch_pass_uneven_t(const double* const* sets, int src, int dst, const PassSys* tab,
-                                const int* bmap, long long blocks, int Sin, hipStream_t st)
+                                const int* bmap, long long blocks, int Sin, int kmax, hipStream_t st)
 {
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffffLL) return -2;
-    hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT>), dim3((unsigned)blocks), dim3(TB * G / RPT),
-                       0, st, sets, src, dst, tab, bmap, Sin);
+    if (SP && (1 << kmax) <= TB / RPT)
+        hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT, SP>), dim3((unsigned)blocks),
+                           dim3(TB * G / RPT), 0, st, sets, src, dst, tab, bmap, Sin);
+    else
+        hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT, false>), dim3((unsigned)blocks),
+                           dim3(TB * G / RPT), 0, st, sets, src, dst, tab, bmap, Sin);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -256,16 +289,19 @@ using PassFn = int (*)(const double* const*, double* const*, int, int, int, int,
                        long long, int, hipStream_t);
 
 using PassUnevenFn = int (*)(const double* const*, int, int, const PassSys*, const int*, long long,
-                             int, hipStream_t);
+                             int, int, hipStream_t);
 
 struct PassShape {
     int tb, g, rpt;
+    bool spaced;
     PassFn fn;
     PassUnevenFn uneven;
 };
 
 #define PCRP_PASS_SHAPE(TB, G, RPT) \
-    {TB, G, RPT, &launch_pass<double, TB, G, RPT>, &launch_pass_uneven_t<TB, G, RPT>}
+    {TB, G, RPT, false, &launch_pass<double, TB, G, RPT, false>, &launch_pass_uneven_t<TB, G, RPT, false>}
+#define PCRP_PASS_SHAPE_SP(TB, G, RPT) \
+    {TB, G, RPT, true, &launch_pass<double, TB, G, RPT, true>, &launch_pass_uneven_t<TB, G, RPT, true>}
 
 static const PassShape kPassShapes[] = {
     // one (row, sub-system) slot per thread
@@ -278,8 +314,14 @@ static const PassShape kPassShapes[] = {
     PCRP_PASS_SHAPE(512, 2, 4),
     PCRP_PASS_SHAPE(128, 8, 4),
     PCRP_PASS_SHAPE(1024, 1, 4),
+    // four slots per thread, spaced steps
+    PCRP_PASS_SHAPE_SP(256, 4, 4),
+    PCRP_PASS_SHAPE_SP(512, 2, 4),
+    PCRP_PASS_SHAPE_SP(128, 8, 4),
+    PCRP_PASS_SHAPE_SP(1024, 1, 4),
 };
 #undef PCRP_PASS_SHAPE
+#undef PCRP_PASS_SHAPE_SP
 
 int pass_shape_tb(int shape) { return kPassShapes[shape].tb; }
 int pass_shape_g(int shape) { return kPassShapes[shape].g; }
@@ -291,20 +333,25 @@ int pass_shape_single(int shape)
 {
     const int nshapes = (int)(sizeof(kPassShapes) / sizeof(kPassShapes[0]));
     for (int i = 0; i < nshapes; ++i)
-        if (kPassShapes[i].g == 1 && kPassShapes[i].rpt == kPassShapes[shape].rpt) return i;
+        if (kPassShapes[i].g == 1 && kPassShapes[i].rpt == kPassShapes[shape].rpt &&
+            kPassShapes[i].spaced == kPassShapes[shape].spaced)
+            return i;
     return shape;
 }
 
 int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
-                       const int* bmap, long long blocks, int Sin, hipStream_t st)
+                       const int* bmap, long long blocks, int Sin, int kmax, hipStream_t st)
 {
-    return kPassShapes[shape].uneven(sets, src, dst, tab, bmap, blocks, Sin, st);
+    return kPassShapes[shape].uneven(sets, src, dst, tab, bmap, blocks, Sin, kmax, st);
 }
 
 // Tunables of the split path: LDS/HBM threshold (sub-systems at most this
 // long go to the LDS kernel), steps fused per HBM pass, pass tile shape.
-// (a mutex: solves read them while another thread may tune)
-static SplitTune g_split_f64 = {1024, 4, 0};
+// (a mutex: solves read them while another thread may tune). Default: the
+// autotuner's choice on the uneven batch of 10000 systems (MI355X): five
+// steps per pass, spaced 256 x 4 tiles of 4 sub-systems (whole-row spaced
+// 1024 x 1 tiles in the first uneven round).
+static SplitTune g_split_f64 = {1024, 5, 8};
 static std::mutex g_split_mu;
 
 SplitTune split_f64_get()

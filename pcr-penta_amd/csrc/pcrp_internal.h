@@ -190,7 +190,7 @@ int split_f64_levels(int n, int lds_max);
 size_t split_f64_workspace(int n, int m);
 int split_f64_solve(const SolveArgsF64& A, int layout, void* ws, size_t ws_bytes, hipStream_t st);
 int launch_pass_uneven(int shape, const double* const* sets, int src, int dst, const PassSys* tab,
-                       const int* bmap, long long blocks, int Sin, hipStream_t st);
+                       const int* bmap, long long blocks, int Sin, int kmax, hipStream_t st);
 int pass_shape_tb(int shape);
 int pass_shape_g(int shape);
 int pass_shape_rpt(int shape);

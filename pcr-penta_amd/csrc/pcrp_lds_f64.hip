@@ -30,11 +30,12 @@
 //   SPACED  : at each LDS stride a thread owns RPT rows two strides apart, so
 //             its RPT + 1 pair equations serve all its rows (4.5 divisions per
 //             row-step at RPT = 4) and it reads 2*RPT + 3 rows; the rows stay
-//             in LDS between steps in a per-stride layout (spaced_step).
+//             in LDS between steps in a per-stride layout (pcrp_spaced.h).
 // All give the same bits (the pair equation is the same expression on the
 // same operands either way).
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
+#include "pcrp_spaced.h"
 
 namespace pcrp {
 
@@ -156,116 +157,6 @@ __device__ __forceinline__ bool inplace_zero_pivot_step(Row<double> (&r)[RPT], i
     return false;
 }
 
-// ---- SPACED steps ------------------------------------------------------------
-// At stride d = 2^S < NT the rows split into d chains (residue r = i mod d);
-// chain positions p (row r + d*p) are cut into blocks of 2*RPT, and a thread
-// owns the RPT positions of one parity in one block: p = P0 + 2k with
-// P0 = par + 2*RPT*b. Its rows' pair equations are centred on the odd
-// positions P0 - 1, P0 + 1, ..., P0 + 2*RPT - 1, and each of those RPT + 1
-// centres serves two of its rows (as E(i+d) of one and E(i-d) of the next),
-// so a row-step costs (RPT+1)/RPT pair equations instead of 2:
-// 2 + 2(RPT+1)/RPT divisions (4.5 at RPT = 4) and 2*RPT + 3 rows read from
-// LDS per RPT rows. The same pair-equation and reduction expressions run on
-// the same operands, so the bits are those of the other forms.
-//
-// Layout (per stride): thread t = r + d*(par + 2b) owns LDS slot t, and its
-// row k sits at sR[c][k*NT + t]. The partner thread (other parity, same
-// block) is slot t ^ d; the outer rows are rows of slots t -+ 2d; so every
-// lane reads one slot per instruction and the 64 lanes of a wave read 64
-// different banks. A step writes its new rows straight into the layout of
-// stride 2d (or in natural order, row i at i, after the last LDS step).
-// Slots of rows >= n hold the identity row (0, 0, 1, 0, 0, 0), so every row
-// read is finite; outer neighbours past either end of a chain are replaced by
-// the thread's own first / last row, as the redundant form does.
-__device__ __forceinline__ Row<double> identity_row() { return Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0}; }
-
-// LDS address of row i in the spaced layout of stride 2^S (RPT rows per slot).
-template <int NT, int RPT>
-__device__ __forceinline__ int spaced_addr(int i, int S)
-{
-    const int d = 1 << S;
-    const int r = i & (d - 1), p = i >> S;
-    const int b = p / (2 * RPT), q = p & (2 * RPT - 1);
-    return (q >> 1) * NT + r + d * ((q & 1) + 2 * b);
-}
-
-template <int NT, int RPT>
-__device__ __forceinline__ void spaced_step(int n, int S, bool natural, double (&sR)[6][NT * RPT])
-{
-    static_assert(RPT >= 2 && RPT % 2 == 0, "spaced steps need an even number of rows per thread");
-    const int t = threadIdx.x;
-    const int d = 1 << S;
-    const int r = t & (d - 1), par = (t >> S) & 1, b = t >> (S + 1);
-    const int NB = NT >> (S + 1);
-    const int P0 = par + 2 * RPT * b;
-    const int Lr = (n - r + d - 1) >> S;   // positions of chain r holding rows < n
-    auto rd = [&](int a) {
-        Row<double> q;
-        q.a = sR[0][a]; q.b = sR[1][a]; q.c = sR[2][a];
-        q.d = sR[3][a]; q.e = sR[4][a]; q.y = sR[5][a];
-        return q;
-    };
-    // odd positions P0 + 2m - 1: m = 1..RPT-1 from the partner at bo + m*NT;
-    // m = 0 and m = RPT may lie in the neighbouring block's partner slot
-    const int bo = par ? t - d : t + d - NT;
-    const int a_o0 = par ? t - d : (b > 0 ? (RPT - 1) * NT + t - d : t);
-    const int a_on = par ? (b < NB - 1 ? t + d : (RPT - 1) * NT + t) : (RPT - 1) * NT + t + d;
-    const int a_l2 = b > 0 ? (RPT - 1) * NT + t - 2 * d : t;        // position P0 - 2
-    const int a_r2 = b < NB - 1 ? t + 2 * d : (RPT - 1) * NT + t;   // position P0 + 2*RPT
-    Row<double> N[RPT];
-    if (P0 < Lr) {   // a thread whose rows are all >= n only writes identity rows
-        Pair<double> Ep;
-        Row<double> L = rd(a_l2);
-#pragma unroll
-        for (int m = 0; m <= RPT; ++m) {
-            // E(m) serves rows m - 1 and m; once row m - 1 is past n, so is
-            // every later row (their slots get identity rows below)
-            if (m > 0 && P0 + 2 * (m - 1) >= Lr) break;
-            const Row<double> M = rd(m == 0 ? a_o0 : (m == RPT ? a_on : bo + m * NT));
-            const Row<double> R = rd(m == RPT ? a_r2 : m * NT + t);
-            const int pc = P0 + 2 * m - 1;
-            const Pair<double> E = make_pair(L, M, R, pc >= 1, pc >= 2, pc + 1 < Lr, pc + 2 < Lr);
-            if (m > 0) {
-                const int po = pc - 1;   // own row m - 1, held in L
-                N[m - 1] = reduce_row(L, Ep, E, po >= 1, po + 1 < Lr, po >= 2, po + 2 < Lr,
-                                      [&](double& alL, double& arL, double& alR, double& arR) {
-                                          alL = Ep.al; arL = Ep.ar; alR = E.al; arR = E.ar;
-                                      });
-            }
-            Ep = E;
-            set_row(L, R);
-        }
-    }
-    __syncthreads();   // every read of this step done
-    auto wr = [&](int a, const Row<double>& q) {
-        sR[0][a] = q.a; sR[1][a] = q.b; sR[2][a] = q.c;
-        sR[3][a] = q.d; sR[4][a] = q.e; sR[5][a] = q.y;
-    };
-    if (natural) {
-        const int base = r + d * P0;
-#pragma unroll
-        for (int k = 0; k < RPT; ++k)
-            if (P0 + 2 * k < Lr) wr(base + 2 * d * k, N[k]);
-    } else {
-        // row r + d*(P0 + 2k) at stride 2d: residue r + d*par, block b/2,
-        // parity k&1, slot row (RPT/2)*(b&1) + k/2
-        const int w0 = (RPT / 2) * (b & 1) * NT + r + d * par + 4 * d * (b >> 1);
-        const int w1 = w0 + 2 * d;
-        if (__builtin_expect(P0 + 2 * (RPT - 1) < Lr, 1)) {   // all RPT rows < n
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) wr(((k & 1) ? w1 : w0) + (k >> 1) * NT, N[k]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int a = ((k & 1) ? w1 : w0) + (k >> 1) * NT;
-                if (P0 + 2 * k < Lr) wr(a, N[k]);
-                else wr(a, identity_row());
-            }
-        }
-    }
-    __syncthreads();
-}
-
 // Solve one system (or sub-system) held at ga..gy[base + i*rs], i < n, with
 // `steps` reduction steps; x goes to gx[xbase + i*xrs]. All threads of the
 // workgroup call this together (it contains barriers).
@@ -347,7 +238,7 @@ __device__ __forceinline__ void lds_solve_regs_f(
             }
             __syncthreads();
             for (; s < steps && delta < NT; ++s, delta <<= 1)
-                spaced_step<NT, RPT>(n, s, s + 1 == steps || 2 * delta >= NT, sR);
+                spaced_step<NT, RPT, 1>(s, s + 1 == steps || 2 * delta >= NT, 0, n, sR);
             // back to one thread per row i = t + NT*k (rows in natural order now)
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {

@@ -262,7 +262,8 @@ int launch_plan(const UnevenPlan& P, double* y, void* ws, hipStream_t st)
         rc = launch_pass_uneven(P.round_shape[r], dsets, src, dst,
                                 reinterpret_cast<const PassSys*>(w + P.round_off[r]),
                                 reinterpret_cast<const int*>(w + P.round_map_off[r]),
-                                P.round_blocks[r], (int)r * split_f64_get().k_fuse, st);
+                                P.round_blocks[r], (int)r * split_f64_get().k_fuse,
+                                split_f64_get().k_fuse, st);
         if (rc != 0) return PCRP_ERR_HIP;
     }
     // one persistent LDS launch over every job

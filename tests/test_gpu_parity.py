@@ -436,6 +436,35 @@ def test_split_tunables_bit_exact(gpu, oracle_lib):
         gpu.split_tune_set(*saved[:3])
 
 
+def test_split_spaced_pass_shapes_bit_exact(gpu, oracle_lib):
+    """The spaced pass shapes at the production LDS threshold: one pass round
+    (n = 2048), a ragged split (n = 5000) and two rounds (n = 2^16), both
+    layouts, several steps per pass. Bit-identical to the oracle."""
+    import torch
+    saved = gpu.split_tune_get()
+    spaced_shapes = (8, 9, 10, 11)
+    assert saved[3] >= 12
+    try:
+        for n, m in ((2048, 6), (5000, 5), (1 << 16, 2)):
+            for layout in (I, C):
+                dev = gpu.generate(0, 40 + n, n, m, layout)
+                ins = [_np(t).copy() for t in dev]
+                ref = oracle_lib.pcr_batch(*ins, n, m, layout)
+                for shape in spaced_shapes:
+                    for k in (3, 5):
+                        try:
+                            gpu.split_tune_set(1024, k, shape)
+                        except gpu.PCRPError:
+                            continue   # halo too wide for this tile shape
+                        _say("split-spaced", n, m, layout, k, shape)
+                        y = torch.from_numpy(ins[5]).cuda()
+                        gpu.solve(*dev[:5], y, n, m, layout)
+                        torch.cuda.synchronize()
+                        assert np.array_equal(_np(y), ref), (n, m, layout, k, shape)
+    finally:
+        gpu.split_tune_set(*saved[:3])
+
+
 def test_split_n2pow20_weak_sample(gpu, oracle_lib):
     """n = 2^20 systems (weakly dominant generator), 2 systems:
     bit-identical to the oracle and within FP64 PCR noise of dgbsv."""
@@ -530,14 +559,17 @@ def test_uneven_grouped_residue_layout_bit_exact(gpu, oracle_lib):
     """Deep splits at the production thresholds: systems around powers of two
     up to 2^16 (S up to 7 at lds_max 512), so the pass buffers hold residue
     groups of 2 and 4 with short last residues; several steps per pass and
-    both tile families (4 sub-systems per tile, 8 per tile). Bit-identical to
-    the oracle in every setting."""
+    both tile families (4 sub-systems per tile, 8 per tile), with the
+    redundant and the spaced step forms (shapes 8-11; 128 x 8 tiles at k = 5
+    give 32 threads per sub-system, the most k allows). Bit-identical to the
+    oracle in every setting."""
     saved = gpu.split_tune_get()
     sizes = [1025, 1027, 2047, 2049, 3071, 4097, 8191, 8193, 12289, 16385, 32767, 40001, 65537]
     sizes += [5, 700, 1024]
     try:
         for lds_max, k, shape in ((1024, 3, 0), (1024, 5, 2), (512, 3, 4), (512, 5, 6),
-                                  (256, 4, 0)):
+                                  (256, 4, 0), (1024, 4, 8), (512, 3, 9), (1024, 5, 10),
+                                  (256, 6, 11)):
             gpu.split_tune_set(lds_max, k, shape)
             _say("uneven-grouped", lds_max, k, shape)
             x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 0, 60 + k)
