@@ -90,6 +90,32 @@ __global__ void init_t2_kernel(int n, int m, long long rs, long long ss,
     a[g] = va; b[g] = vb; c[g] = vc; d[g] = vd; e[g] = ve; y[g] = 1.0;
 }
 
+// Reads a..e once (count elements each). The tuners run it between copying
+// the right-hand side into the work buffer and the timed launch: the copy
+// leaves that buffer in the 256 MB Infinity Cache, and a batch whose a..e
+// outgrow it evicts the buffer again, so the timed solve reads its right-hand
+// side from HBM, as a solve of a new right-hand side does (warm buffers rank
+// the variants differently). `never` is 0: the sum is stored only to keep the
+// loads.
+__global__ void stream_read_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                   const double* __restrict__ c, const double* __restrict__ d,
+                                   const double* __restrict__ e, size_t count, int never,
+                                   double* sink)
+{
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        acc += a[i] + b[i] + c[i] + d[i] + e[i];
+    if (never) sink[0] = acc;
+}
+
+bool stream_coefficients(const double* a, const double* b, const double* c, const double* d,
+                         const double* e, size_t count, double* sink, hipStream_t s)
+{
+    hipLaunchKernelGGL(stream_read_kernel, dim3(2048), dim3(256), 0, s, a, b, c, d, e, count, 0, sink);
+    return hipGetLastError() == hipSuccess;
+}
+
 uint64_t host_splitmix64(uint64_t z)
 {
     z += 0x9E3779B97F4A7C15ULL;
@@ -288,6 +314,7 @@ int pcrp_autotune(int precision, int n, int m, int layout, const double* a, cons
             for (int r = 0; r < reps && st == PCRP_OK; ++r) {
                 float t = 0.f;
                 if (hipMemcpyAsync(y_work, y, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    !stream_coefficients(a, b, c, d, e, bytes / sizeof(double), y_work, s) ||
                     hipEventRecord(e0, s) != hipSuccess || v->launch(A, s) != 0 ||
                     hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
@@ -359,6 +386,7 @@ int pcrp_split_autotune(int precision, int n, int m, int layout, const double* a
             for (int r = 0; r < reps; ++r) {
                 float t = 0.f;
                 if (hipMemcpyAsync(y_work, y, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    !stream_coefficients(a, b, c, d, e, bytes / sizeof(double), y_work, s) ||
                     hipEventRecord(e0, s) != hipSuccess ||
                     pcrp_solve_f64(a, b, c, d, e, y_work, n, m, layout, ws, ws_bytes, s) != PCRP_OK ||
                     hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -417,6 +445,7 @@ int pcrp_f32r_autotune(int n, int m, int layout, const double* a, const double* 
         for (int r = 0; r < reps; ++r) {
             float t = 0.f;
             if (hipMemcpyAsync(y_work, y, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                !stream_coefficients(a, b, c, d, e, bytes / sizeof(double), y_work, s) ||
                 hipEventRecord(e0, s) != hipSuccess ||
                 pcrp_solve_f32r_f64(a, b, c, d, e, y_work, n, m, layout, 8, 1e-12, nullptr, ws,
                                     ws_bytes, s) != PCRP_OK ||
