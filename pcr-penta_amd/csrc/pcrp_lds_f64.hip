@@ -485,9 +485,19 @@ __global__ __launch_bounds__(NT) void lds_f64_queue_kernel(
             if (idx >= len) break;
             const LdsJob J = jobs[begin + idx];
             const double* const* S = sets + 6 * J.src;
-            lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base,
-                                         1LL << J.stride_log2, J.xbase, 1LL << J.xstride_log2,
-                                         J.n, J.steps, sR, sE);
+            // the spaced form works through every row of its tile, so the
+            // spaced queue takes it for jobs that fill most of the tile and
+            // the redundant form (same LDS) for the rest (uniform per job)
+            bool redundant = false;
+            if constexpr (KIND == KIND_SPACED) redundant = J.n * 32 <= NT * RPT * 23;
+            if (redundant)
+                lds_solve_one<KIND == KIND_SPACED ? KIND_REDUNDANT : KIND, NT, RPT>(
+                    S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base, 1LL << J.stride_log2, J.xbase,
+                    1LL << J.xstride_log2, J.n, J.steps, sR, sE);
+            else
+                lds_solve_one<KIND, NT, RPT>(S[0], S[1], S[2], S[3], S[4], S[5], gx, J.base,
+                                             1LL << J.stride_log2, J.xbase, 1LL << J.xstride_log2,
+                                             J.n, J.steps, sR, sE);
             __syncthreads();
         }
     }
@@ -711,10 +721,8 @@ static int find_variant(int kind, int nt, int rpt)
 
 // The redundant form with 256 threads per workgroup and as many rows per
 // thread as n needs (tightest capacity first, then the workgroup size closest
-// to 256). Default of the uneven work queue, whose jobs have every size up to
-// its capacity: the spaced form works on all its rows whatever n is, and on
-// the uneven batch of 10000 systems (MI355X) it is 2.5% slower.
-int lds_f64_queue_default_variant(int n)
+// to 256).
+static int redundant_default_variant(int n)
 {
     int best = -1, best_score = 1 << 30;
     for (int id = 0; id < lds_f64_variant_count(); ++id) {
@@ -735,7 +743,18 @@ int lds_f64_default_variant(int n)
         const int id = find_variant(KIND_SPACED, 256, 4);
         if (id >= 0) return id;
     }
-    return lds_f64_queue_default_variant(n);
+    return redundant_default_variant(n);
+}
+
+// Default of the uneven work queue, whose jobs have every size up to its
+// capacity n: the spaced queue kernel, which runs the spaced form on the jobs
+// that fill most of its tile and the redundant form on the others (MI355X,
+// uneven batch of 10000 systems: 18.4-18.6 vs 18.7-18.8 ms per solve with
+// the redundant form on every job; the spaced form on every job was even
+// with it, as the small jobs lost what the large ones gained).
+int lds_f64_queue_default_variant(int n)
+{
+    return lds_f64_default_variant(n);
 }
 
 // One-system default for a whole batch of m systems (MI355X, FP64, every

@@ -578,6 +578,25 @@ def test_uneven_grouped_residue_layout_bit_exact(gpu, oracle_lib):
         gpu.split_tune_set(*saved[:3])
 
 
+def test_uneven_queue_step_forms_bit_exact(gpu, oracle_lib):
+    """The work queue pinned to the spaced queue kernel (spaced steps for jobs
+    above 736 rows, redundant below, chosen per job) and to the redundant
+    one: whole systems and residue sub-systems on both sides of the switch,
+    bit-identical to the oracle."""
+    vs = {v["kind"] + f"-{v['nt']}x{v['rpt']}": v["id"] for v in gpu.variants()}
+    rng = np.random.default_rng(77)
+    sizes = [1, 2, 700, 736, 737, 800, 1023, 1024, 1473, 1475, 2047, 2049, 2943, 5889]
+    sizes += list(rng.integers(1, 3000, 40))
+    try:
+        for name in ("spaced-256x4", "redundant-256x4"):
+            gpu.tune_set(1024, vs[name])
+            _say("uneven-queue", name, len(sizes))
+            x, ref, _, _ = _uneven_case(gpu, oracle_lib, sizes, 0, 91)
+            assert np.array_equal(x, ref), name
+    finally:
+        gpu.tune_set(1024, -1)
+
+
 def test_uneven_log_uniform_sample(gpu, oracle_lib):
     """First 400 systems of the uneven size draw (n log-uniform in [64, 131072], seed 1)."""
     sizes = gpu.uneven_log_uniform_sizes()[:400]
