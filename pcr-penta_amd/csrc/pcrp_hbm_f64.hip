@@ -36,6 +36,27 @@
 
 namespace pcrp {
 
+// Natural order of a spaced pass's last step: tile row i at i with the bits
+// [s, s+3) XORed with the low bits of i >> k (bits below k, so a permutation
+// of the tile rows). The grouped store phase reads rows 2^k apart in
+// consecutive lanes, which share LDS banks in plain natural order.
+struct PassNat {
+    int k, s, mask;
+    __device__ int operator()(int i) const { return i ^ (((i >> k) & mask) << s); }
+};
+
+// Store modes of a spaced pass (SM): 1, the output slot in a group of 4
+// residues is g mod 4 (G a multiple of 4); 2, it is j mod 4, j = l mod 2^k (one
+// sub-system per tile, at least 2 steps); 0, per-row stores from registers.
+template <int G, int SM>
+__device__ __forceinline__ PassNat pass_nat(int k)
+{
+    if (SM == 1 && G == 4 && k >= 3) return PassNat{k, 0, 7};
+    if (SM == 1 && G == 8 && k >= 2) return PassNat{k, 0, 3};
+    if (SM == 2 && k >= 5) return PassNat{k, 2, 7};
+    return PassNat{k, 0, 0};
+}
+
 // One tile of one HBM pass: rows l = tile*T - H + lr of current sub-system q
 // (valid when 0 <= l < L), k fused steps in LDS, interior rows stored.
 // Each thread owns RPT (row, sub-system) slots: slot tid + NTH*kk is row
@@ -46,11 +67,23 @@ namespace pcrp {
 // SP: the k steps in the spaced form (pcrp_spaced.h; FP64, RPT even, 2^k at
 // most the TB/RPT threads of a sub-system), else each row builds both its
 // pair equations (kernel.cu:66-140 as written).
-template <typename T, int TB, int G, int RPT, bool SP, typename InAt, typename OutAt>
+// Stores: a row of sub-system q goes to residue q' = q + j*Qstep (j = l mod
+// 2^k) at element l >> k, and the output keeps groups of 4 residues
+// row-interleaved. With SM != 0 (SP only) the interior rows are stored from
+// LDS in the output's own order: each wave writes runs of consecutive
+// elements of one group (slot fastest, then the element), instead of pieces
+// of 8 groups per instruction when lanes follow the rows (a block whose
+// output groups are smaller, SM 2 with one step, stores row by row from LDS).
+// out_g(g, l) and len_g(g) give the element index of row l of the tile's
+// sub-system g and that sub-system's length (0 when g is past the last one).
+template <typename T, int TB, int G, int RPT, bool SP, int SM, typename InAt, typename OutAt,
+          typename OutG, typename LenG>
 __device__ __forceinline__ void pass_tile(const T* const (&in)[6], T* const (&out)[6],
                                           bool qvalid, int l0, int L, int k, int H, int TI,
-                                          InAt in_at, OutAt out_at, T (&sR)[6][TB * G])
+                                          InAt in_at, OutAt out_at, T (&sR)[6][TB * G],
+                                          OutG out_g, LenG len_g)
 {
+    static_assert(SM == 0 || SP, "grouped stores need the spaced form");
     constexpr int NTH = TB * G / RPT;
     constexpr int RSTEP = TB / RPT;
     const int tid = threadIdx.x, lr0 = tid / G, g = tid - lr0 * G;
@@ -85,10 +118,53 @@ __device__ __forceinline__ void pass_tile(const T* const (&in)[6], T* const (&ou
         __syncthreads();
         const int lofs = l0 - lr0;   // sub-system row of tile row 0
         const int Lq = qvalid ? L : 0;
-        for (int s = 0; s < k; ++s) spaced_step<RSTEP, RPT, G>(s, s + 1 == k, lofs, Lq, sR);
-        // (rows outside the sub-system are not stored: any value will do)
+        const PassNat nat = pass_nat<G, SM>(k);
+        for (int s = 0; s < k; ++s) spaced_step<RSTEP, RPT, G>(s, s + 1 == k, lofs, Lq, sR, nat);
+        if constexpr (SM != 0) {
+            if (SM == 2 && k < 2) {
+#pragma unroll 1
+                for (int kk = 0; kk < RPT; ++kk) {
+                    const int l = l0 + RSTEP * kk, lr = lr0 + RSTEP * kk;
+                    if (valid[kk] && lr >= H && lr < H + TI) {
+                        const int a = nat(lr) * G + g;
+                        const long long go = out_at(l);
+                        out[0][go] = sR[0][a]; out[1][go] = sR[1][a]; out[2][go] = sR[2][a];
+                        out[3][go] = sR[3][a]; out[4][go] = sR[4][a]; out[5][go] = sR[5][a];
+                    }
+                }
+                return;
+            }
+            // items (slot s4, sub-system group gh, element e, residue j), slot
+            // fastest; E elements per residue cover the kept rows [kl0, kl0+TI)
+            const int K2 = 1 << k;
+            const int E = (TI + K2 - 1) >> k;
+            const int kl0 = lofs + H;   // = tile*TI >= 0
+            constexpr int GH = SM == 1 ? G / 4 : 1;
+            const int items = 4 * GH * E * (SM == 1 ? K2 : K2 / 4);
+#pragma unroll 1
+            for (int u = tid; u < items; u += NTH) {
+                const int s4 = u & 3;
+                int rest = u >> 2;
+                const int gh = rest % GH;
+                rest /= GH;
+                const int el = rest % E;
+                const int jj = rest / E;
+                const int gg = SM == 1 ? gh * 4 + s4 : 0;
+                const int j = SM == 1 ? jj : jj * 4 + s4;
+                const int l = (((kl0 - j + K2 - 1) >> k) + el) * K2 + j;
+                if (l < kl0 + TI && l < len_g(gg)) {
+                    const int a = nat(l - lofs) * G + gg;
+                    const long long go = out_g(gg, l);
+                    out[0][go] = sR[0][a]; out[1][go] = sR[1][a]; out[2][go] = sR[2][a];
+                    out[3][go] = sR[3][a]; out[4][go] = sR[4][a]; out[5][go] = sR[5][a];
+                }
+            }
+            return;
+        } else {
+            // (rows outside the sub-system are not stored: any value will do)
 #pragma unroll
-        for (int kk = 0; kk < RPT; ++kk) r[kk] = readR(tid + NTH * kk);
+            for (int kk = 0; kk < RPT; ++kk) r[kk] = readR(tid + NTH * kk);
+        }
     }
 #pragma unroll
     for (int kk = 0; kk < RPT; ++kk)
@@ -162,7 +238,7 @@ __host__ __forceinline__ long long blocks_for(int groups, int tiles)
 }
 
 // Uniform batch: m0 systems of n0 rows, current sub-systems q = R*m0 + j.
-template <typename T, int TB, int G, int RPT, bool SP>
+template <typename T, int TB, int G, int RPT, bool SP, int SM>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const T* __restrict__ ia, const T* __restrict__ ib, const T* __restrict__ ic,
     const T* __restrict__ id, const T* __restrict__ ie, const T* __restrict__ iy,
@@ -186,7 +262,7 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
     const long long Lout = ((long long)n0 + (1LL << (Sin + k)) - 1) >> (Sin + k);
     const T* const in[6] = {ia, ib, ic, id, ie, iy};
     T* const out[6] = {oa, ob, oc, od, oe, oy};
-    pass_tile<T, TB, G, RPT, SP>(
+    pass_tile<T, TB, G, RPT, SP, SM>(
         in, out, q < Qin, l0, L, k, H, TI,
         [&](int ll) {
             return Sin == 0 ? (long long)q * in_ss + (long long)ll * in_rs
@@ -195,7 +271,15 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
         [&](int ll) {
             return split_slot((long long)(ll & ((1 << k) - 1)) * Qin + q, ll >> k, Qout, Lout);
         },
-        sR);
+        sR,
+        [&](int gg, int ll) {
+            const int qq = grp * G + gg;
+            return split_slot((long long)(ll & ((1 << k) - 1)) * Qin + qq, ll >> k, Qout, Lout);
+        },
+        [&](int gg) {
+            const int qq = grp * G + gg;
+            return qq < Qin ? (n0 - qq / m0 + (1 << Sin) - 1) >> Sin : 0;
+        });
 }
 
 // Uneven batch: systems stored contiguously at their own offsets. The first
@@ -207,7 +291,7 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_kernel(
 // tab lists the systems that still need passes in this round; bmap gives
 // every block its entry of tab (a binary search over the entries, 13
 // dependent loads at 6000 systems, delayed every block's first loads).
-template <int TB, int G, int RPT, bool SP>
+template <int TB, int G, int RPT, bool SP, int SM>
 __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     const double* const* __restrict__ sets, int src, int dst, const PassSys* __restrict__ tab,
     const int* __restrict__ bmap, int Sin)
@@ -238,7 +322,16 @@ __global__ __launch_bounds__(TB* G / RPT) void hbm_pass_uneven_f64_kernel(
     auto out_at = [&](int ll) {
         return P.woff + res_slot(q + ((ll & ((1 << k) - 1)) << Sin), ll >> k, Sout, P.n);
     };
-    pass_tile<double, TB, G, RPT, SP>(in, out, q < Qin, l0, L, k, H, T, in_at, out_at, sR);
+    pass_tile<double, TB, G, RPT, SP, SM>(
+        in, out, q < Qin, l0, L, k, H, T, in_at, out_at, sR,
+        [&](int gg, int ll) {
+            const int qq = grp * G + gg;
+            return P.woff + res_slot(qq + ((ll & ((1 << k) - 1)) << Sin), ll >> k, Sout, P.n);
+        },
+        [&](int gg) {
+            const int qq = grp * G + gg;
+            return qq < Qin ? (P.n - qq + Qin - 1) >> Sin : 0;
+        });
 }
 
 // SP shapes run the spaced form when 2^k fits the TB/RPT threads of a
@@ -255,16 +348,19 @@ static int launch_pass(const T* const in[6], T* const out[6], int n0, int m0, in
     const int groups = (Qin + G - 1) / G;
     const long long blocks = blocks_for(groups, tiles);
     if (blocks > 0x7fffffffLL) return -2;
-    if (SP && (1 << k) <= TB / RPT)
-        hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT, SP>), dim3((unsigned)blocks),
-                           dim3(TB * G / RPT), 0, st, in[0], in[1], in[2], in[3], in[4], in[5],
-                           out[0], out[1], out[2], out[3], out[4], out[5], n0, m0, Sin, k, in_rs,
-                           in_ss, Qin, tiles);
-    else
-        hipLaunchKernelGGL((hbm_pass_kernel<T, TB, G, RPT, false>), dim3((unsigned)blocks),
-                           dim3(TB * G / RPT), 0, st, in[0], in[1], in[2], in[3], in[4], in[5],
-                           out[0], out[1], out[2], out[3], out[4], out[5], n0, m0, Sin, k, in_rs,
-                           in_ss, Qin, tiles);
+    const auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(TB * G / RPT), 0, st, in[0], in[1],
+                           in[2], in[3], in[4], in[5], out[0], out[1], out[2], out[3], out[4],
+                           out[5], n0, m0, Sin, k, in_rs, in_ss, Qin, tiles);
+    };
+    // grouped stores when the output slot is the tile's sub-system mod 4
+    constexpr bool SP_OK = SP && G % 4 == 0;
+    if (SP && (1 << k) <= TB / RPT) {
+        if (SP_OK && Qin % 4 == 0) go(hbm_pass_kernel<T, TB, G, RPT, SP, SP_OK ? 1 : 0>);
+        else go(hbm_pass_kernel<T, TB, G, RPT, SP, 0>);
+    } else {
+        go(hbm_pass_kernel<T, TB, G, RPT, false, 0>);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -276,12 +372,21 @@ static int launch_pass_uneven_t(const double* const* sets, int src, int dst, con
 {
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffffLL) return -2;
-    if (SP && (1 << kmax) <= TB / RPT)
-        hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT, SP>), dim3((unsigned)blocks),
-                           dim3(TB * G / RPT), 0, st, sets, src, dst, tab, bmap, Sin);
-    else
-        hipLaunchKernelGGL((hbm_pass_uneven_f64_kernel<TB, G, RPT, false>), dim3((unsigned)blocks),
-                           dim3(TB * G / RPT), 0, st, sets, src, dst, tab, bmap, Sin);
+    const auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(TB * G / RPT), 0, st, sets, src,
+                           dst, tab, bmap, Sin);
+    };
+    // grouped stores: the output slot is the tile's sub-system mod 4 (groups of
+    // 4 or 8 sub-systems, at least 4 residues in) or, in a round of whole
+    // systems (one sub-system per tile), the residue j mod 4
+    constexpr int SM_G = (SP && G % 4 == 0) ? 1 : (SP && G == 1) ? 2 : 0;
+    if (SP && (1 << kmax) <= TB / RPT) {
+        if (SM_G == 1 && Sin >= 2) go(hbm_pass_uneven_f64_kernel<TB, G, RPT, SP, SM_G>);
+        else if (SM_G == 2 && Sin == 0) go(hbm_pass_uneven_f64_kernel<TB, G, RPT, SP, SM_G>);
+        else go(hbm_pass_uneven_f64_kernel<TB, G, RPT, SP, 0>);
+    } else {
+        go(hbm_pass_uneven_f64_kernel<TB, G, RPT, false, 0>);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

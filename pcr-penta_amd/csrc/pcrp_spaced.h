@@ -21,7 +21,7 @@
 // reads one element per instruction and consecutive lanes read consecutive
 // elements (no bank conflicts). A step writes its new rows straight into the
 // layout of stride 2d, or, after the last step, in natural order (tile row i
-// at i*G + g).
+// at nat(i)*G + g, nat a permutation the caller picks for its own reads).
 //
 // Rows of the tile outside the system (global index l = lofs + i outside
 // [0, L)) hold the identity row (0, 0, 1, 0, 0, 0) in every layout, so every
@@ -47,14 +47,21 @@ __device__ __forceinline__ int spaced_addr(int i, int S)
     return (q >> 1) * NT + r + d * ((q & 1) + 2 * b);
 }
 
+// Natural order with no permutation (see spaced_step).
+struct NaturalOrder {
+    __device__ int operator()(int i) const { return i; }
+};
+
 // One step at stride 2^S (2^S <= NT/2) over a tile of NT*RPT rows of each of
 // G sub-systems, rows in the spaced layout of that stride. Tile row i is row
 // lofs + i of a sub-system of L rows (the guards use these global indices).
-// Writes the new rows in the layout of stride 2^(S+1), or in natural order
-// when `natural`. All threads of the workgroup call it (two barriers).
-template <int NT, int RPT, int G>
+// Writes the new rows in the layout of stride 2^(S+1), or, when `natural`, in
+// natural order: tile row i at slot-row nat(i), nat a permutation of the tile
+// rows (identity by default). All threads of the workgroup call it (two
+// barriers).
+template <int NT, int RPT, int G, typename Nat = NaturalOrder>
 __device__ __forceinline__ void spaced_step(int S, bool natural, int lofs, int L,
-                                            double (&sR)[6][NT * RPT * G])
+                                            double (&sR)[6][NT * RPT * G], Nat nat = Nat())
 {
     static_assert(RPT >= 2 && RPT % 2 == 0, "spaced steps need an even number of rows per thread");
     const int tid = threadIdx.x;
@@ -117,7 +124,7 @@ __device__ __forceinline__ void spaced_step(int S, bool natural, int lofs, int L
 #pragma unroll
         for (int k = 0; k < RPT; ++k) {
             const int l = l0 + 2 * d * k;
-            if (l >= 0 && l < L) wr(base + 2 * d * k, N[k]);
+            if (l >= 0 && l < L) wr(nat(base + 2 * d * k), N[k]);
         }
     } else {
         // row r + d*(P0 + 2k) at stride 2d: residue r + d*par, block b/2,
