@@ -290,14 +290,15 @@ int pcrp_solve_factored_f32r_f64(const void *factor, const double *a, const doub
  * central diffusion in v, explicit mixed term. Each step solves Nv S-lines
  * (contiguous, n = Ns) and Ns v-lines (interleaved, n = Nv) with
  * pcrp_solve_f64 or pcrp_solve_f32r_f64 (the S-line solve stores Y1 + W,
- * the v-line right-hand side, directly). reuse_factor = 1 factors the one
+ * the v-line right-hand side, directly; by default it also computes its own
+ * right-hand side, the explicit stage, from the previous step's grid). reuse_factor = 1 factors the one
  * matrix all v-lines share (the v-line coefficients depend on v only) once
  * before the first step, in both precisions; the S-lines keep the full solve
  * (their matrices differ per line, and reading a per-line factor measured
  * slower than recomputing it); needs Nv <= 1024 (FP64) or 2048
  * (FP32+refine), else PCRP_ERR_UNSUPPORTED.
  * u: DEVICE array of Ns*Nv doubles, u[j*Ns + i-1], receives the price at
- * tau = steps*dt. use_graph = 1 captures a sequence of up to 25 time steps
+ * tau = steps*dt. use_graph = 1 captures a sequence of up to 20 time steps
  * in a hipGraph and replays it (plus one graph for the remaining steps;
  * needs a non-NULL stream; the call then synchronises `stream`
  * before releasing the graph); use_graph = 0 enqueues every step.
@@ -317,6 +318,11 @@ typedef struct {
     int reuse_factor;           /* 1: factor the shared v-line matrix once
                                    (pcrp_factor_*), solve its right-hand sides
                                    each step; 0: full line solves every step */
+    int separate_explicit;      /* 0: the S-line solve kernels compute the
+                                   explicit stage while they load (two launches
+                                   per step; Ns <= 1024 FP64, <= 2048 FP32+refine,
+                                   else as 1); 1: a separate explicit-stage kernel
+                                   first. Both give the same bits. */
 } pcrp_heston_params;
 
 size_t pcrp_heston_workspace_size(const pcrp_heston_params *p, int precision);

@@ -2,30 +2,21 @@
 // paper (convection-dominated Heston PDE). The reference repository ships no
 // ADI code; this caller builds the line systems and steps the Douglas scheme.
 //
-// PDE in time-to-maturity tau for a European call:
-//   u_tau = 1/2 v S^2 u_SS + rho sigma v S u_Sv + 1/2 sigma^2 v u_vv
-//           + r S u_S + kappa (theta_v - v) u_v - r u
-// Unknowns: S_i = i hS, i = 1..Ns (hS = Smax/(Ns+1); u = 0 at S = 0 and
-// u = Smax - K e^{-r tau} at S = Smax), v_j = j hv, j = 0..Nv-1 (hv = vmax/Nv;
-// the PDE holds at v = 0; u = S at v = vmax). Storage U[j*Ns + (i-1)]:
+// The PDE, the grid and the difference operators are in pcrp_heston_ops.h.
 // S-lines are contiguous (m = Nv systems of n = Ns), v-lines interleaved
-// (m = Ns systems of n = Nv).
-//   A1 (S): 1/2 v S^2 u_SS + r S u_S - r u/2, fourth-order 5-point central
-//           differences (3-point second order at i = 1 and i = Ns);
-//   A2 (v): 1/2 sigma^2 v u_vv (3-point) + kappa(theta_v - v) u_v with
-//           second-order upwinding (first order next to v = vmax / at j = 1),
-//           - r u/2;
-//   A0    : rho sigma v S u_Sv, 4-point central cross difference (explicit).
-// Douglas (theta = 1/2), boundary terms b_k(tau) split off each A_k:
+// (m = Ns systems of n = Nv). Douglas (theta = 1/2):
 //   Y0 = U + dt (A0 U + A1 U + A2 U + b(tau_n))
 //   (I - th dt A1) Y1 = Y0 - th dt (A1 U + b1(tau_n)) + th dt b1(tau_n+1)
 //   (I - th dt A2) Y2 = Y1 - th dt (A2 U + b2(tau_n)) + th dt b2(tau_n+1)
 //   U_{n+1} = Y2.
-// A step is four launches on one stream with fixed pointers: the explicit
-// kernel, a one-thread kernel advancing the device-side step index, the
-// S-line solve, which stores Y1 + W (the v-line right-hand side) straight
-// into U, and the v-line solve in place. The step index lives in
-// device memory, so one captured hipGraph replays every step.
+// A step is two launches on one stream with fixed pointers: the S-line
+// solve, whose kernel computes its right-hand side R1 and the v-line term W
+// (the explicit stage) from the previous grid while it loads its rows and
+// stores Y1 + W into the other grid buffer, then the v-line solve in place
+// there. With separate_explicit (or line sizes the folded kernels do not
+// take) an explicit-stage kernel writes R1 and W first and the grid stays in
+// u. A one-thread kernel advances the device-side step index once per
+// captured chunk, so one captured hipGraph replays every chunk.
 // With reuse_factor the v-line matrix (constant over the run, and one matrix
 // for every v-line, since A2 depends on v only) is factored once before the
 // first step: FP64 keeps its PCR-Penta multipliers, FP32+refine its 2x2-block
@@ -34,6 +25,7 @@
 // per-line factor (43 doubles per row at n = 1024, 180 MB per S-line solve)
 // took longer than recomputing it (FP64: 36.2 vs 34.9 us per S-line solve).
 #include "pcr_penta.h"
+#include "pcrp_heston_ops.h"
 #include "pcrp_internal.h"
 
 #include <math.h>
@@ -42,83 +34,7 @@ namespace pcrp {
 
 namespace {
 
-struct Grid {
-    int ns, nv;
-    double hS, hv, smax, vmax, kappa, thv, sigma, rho, r, K, dt, th;
-    // reciprocals of the difference-quotient denominators (host-computed, so
-    // the stencil coefficients take multiplications, not FP64 divisions)
-    double r12hS2, r12hS, rhS2, r2hS, rhv2, r2hv, rhv, r4hShv;
-};
-
-__host__ __device__ inline double S_at(const Grid& g, int i) { return i * g.hS; }
-__host__ __device__ inline double v_at(const Grid& g, int j) { return j * g.hv; }
-
-// far-S boundary value Smax - K e^{-r tau} at time-to-maturity tau
-__device__ inline double far_s(const Grid& g, double tau) { return g.smax - g.K * exp(-g.r * tau); }
-
-// value of the grid function at node (i, j), i in 0..ns+1, j in 0..nv
-// (boundary nodes from the boundary conditions; bS = far_s(tau))
-__device__ inline double node(const Grid& g, const double* U, int i, int j, double bS)
-{
-    if (i == 0) return 0.0;
-    if (i == g.ns + 1) return bS;
-    if (j == g.nv) return S_at(g, i);
-    return U[j * g.ns + (i - 1)];   // ns * nv < 2^31 (checked at entry)
-}
-
-// stencil coefficients of A1 at row i (offsets -2..2)
-__host__ __device__ inline void a1_coef(const Grid& g, int i, int j, double w[5])
-{
-    const double S = S_at(g, i), v = v_at(g, j);
-    const double D = 0.5 * v * S * S, C = g.r * S;
-    if (i >= 2 && i <= g.ns - 1) {
-        const double d2 = D * g.r12hS2, d1 = C * g.r12hS;
-        w[0] = -d2 + d1;
-        w[1] = 16.0 * d2 - 8.0 * d1;
-        w[2] = -30.0 * d2 - 0.5 * g.r;
-        w[3] = 16.0 * d2 + 8.0 * d1;
-        w[4] = -d2 - d1;
-    } else {
-        const double d2 = D * g.rhS2, d1 = C * g.r2hS;
-        w[0] = 0.0;
-        w[1] = d2 - d1;
-        w[2] = -2.0 * d2 - 0.5 * g.r;
-        w[3] = d2 + d1;
-        w[4] = 0.0;
-    }
-}
-
-// stencil coefficients of A2 at row j (offsets -2..2)
-__host__ __device__ inline void a2_coef(const Grid& g, int j, double w[5])
-{
-    const double v = v_at(g, j);
-    const double diff = 0.5 * g.sigma * g.sigma * v * g.rhv2;
-    const double mu = g.kappa * (g.thv - v);
-    w[0] = w[1] = w[3] = w[4] = 0.0;
-    w[2] = -0.5 * g.r;
-    if (j >= 1) {   // diffusion (zero at v = 0)
-        w[1] += diff;
-        w[2] += -2.0 * diff;
-        w[3] += diff;
-    }
-    if (mu >= 0.0) {
-        if (j + 2 <= g.nv) {   // second-order forward
-            const double f = mu * g.r2hv;
-            w[2] += -3.0 * f; w[3] += 4.0 * f; w[4] += -f;
-        } else {               // first-order forward next to v = vmax
-            const double f = mu * g.rhv;
-            w[2] += -f; w[3] += f;
-        }
-    } else {
-        if (j >= 2) {          // second-order backward
-            const double f = mu * g.r2hv;
-            w[2] += 3.0 * f; w[1] += -4.0 * f; w[0] += f;
-        } else {               // first-order backward at j = 1
-            const double f = mu * g.rhv;
-            w[2] += f; w[1] += -f;
-        }
-    }
-}
+using Grid = HestonGrid;
 
 // (I - th dt A) row coefficients into the five diagonals, row-aligned:
 // entries that would couple to a boundary node or fall outside are zero.
@@ -164,75 +80,15 @@ __global__ void payoff_kernel(Grid g, double* U, int* step)
     U[idx] = s > 0.0 ? s : 0.0;
 }
 
-// A1 U (with boundary nodes at tau) at unknown (i, j); w: a1_coef(i, j)
-__device__ inline double apply_a1(const Grid& g, const double* U, int i, int j, double bS,
-                                  const double (&w)[5])
-{
-    double acc = w[2] * node(g, U, i, j, bS);
-    acc += w[1] * node(g, U, i - 1, j, bS);
-    acc += w[3] * node(g, U, i + 1, j, bS);
-    if (w[0] != 0.0) acc += w[0] * node(g, U, i - 2, j, bS);
-    if (w[4] != 0.0) acc += w[4] * node(g, U, i + 2, j, bS);
-    return acc;
-}
-
-// boundary part b1(tau) of A1 at (i, j): terms on boundary nodes only
-__device__ inline double bnd_a1(const Grid& g, int i, int j, double bS, const double (&w)[5])
-{
-    double acc = 0.0;
-    for (int o = -2; o <= 2; ++o) {
-        const int ii = i + o;
-        if (w[o + 2] != 0.0 && (ii == 0 || ii == g.ns + 1))
-            acc += w[o + 2] * node(g, nullptr, ii, j, bS);
-    }
-    return acc;
-}
-
-__device__ inline double apply_a2(const Grid& g, const double* U, int i, int j, double bS,
-                                  const double (&w)[5])
-{
-    double acc = w[2] * node(g, U, i, j, bS);
-    for (int o = -2; o <= 2; ++o) {
-        if (o == 0 || w[o + 2] == 0.0) continue;
-        acc += w[o + 2] * node(g, U, i, j + o, bS);
-    }
-    return acc;
-}
-
-__device__ inline double bnd_a2(const Grid& g, int i, int j, double bS, const double (&w)[5])
-{
-    double acc = 0.0;
-    for (int o = 1; o <= 2; ++o)
-        if (w[o + 2] != 0.0 && j + o == g.nv) acc += w[o + 2] * node(g, nullptr, i, j + o, bS);
-    return acc;
-}
-
-__device__ inline double apply_a0(const Grid& g, const double* U, int i, int j, double bS)
-{
-    if (j == 0) return 0.0;   // v = 0
-    const double c = g.rho * g.sigma * v_at(g, j) * S_at(g, i) * g.r4hShv;
-    const double x = node(g, U, i + 1, j + 1, bS) - node(g, U, i + 1, j - 1, bS) -
-                     node(g, U, i - 1, j + 1, bS) + node(g, U, i - 1, j - 1, bS);
-    return c * x;
-}
-
 // the explicit stage at unknown (i, j): R1 = RHS of the S-line system,
 // W = v-line correction (bS0, bS1: the far-S boundary at tau0, tau0 + dt)
 __device__ inline void explicit_node(const Grid& g, const double* U, double* R1, double* W, int i,
                                      int j, double bS0, double bS1)
 {
     const int idx = j * g.ns + (i - 1);
-    const double u = U[idx];
-    double w1[5], w2[5];   // each stencil's coefficients once per node
-    a1_coef(g, i, j, w1);
+    double w2[5];
     a2_coef(g, j, w2);
-    const double l0 = apply_a0(g, U, i, j, bS0);
-    const double l1 = apply_a1(g, U, i, j, bS0, w1);
-    const double l2 = apply_a2(g, U, i, j, bS0, w2);
-    const double y0 = u + g.dt * (l0 + l1 + l2);
-    const double th = g.th * g.dt;
-    R1[idx] = y0 - th * l1 + th * bnd_a1(g, i, j, bS1, w1);
-    W[idx] = -th * l2 + th * bnd_a2(g, i, j, bS1, w2);
+    heston_stage_node(g, U, i, j, bS0, bS1, w2, R1[idx], W[idx]);
 }
 
 // far-S boundary values of every step, once per run: bnd[2k] at
@@ -432,11 +288,46 @@ int solve_v_lines(const Grid& g, int precision, double* U, const HestonWs& h, hi
                                h.solver_ws, h.solver_bytes, st);
 }
 
-// One Douglas step (step index: device counter + off): three launches (four
-// when the S-line solve cannot store Y1 + W itself), fixed pointers.
-int enqueue_step(const Grid& g, int precision, double* U, const HestonWs& h, int off,
-                 hipStream_t st)
+// The explicit stage folded into the S-line solve (the default): the S-line
+// kernels compute R1 and W from the previous step's grid while they load
+// their rows, so a step is two launches. They read rows j-2..j+2 of that grid
+// while other lines' x is stored, so the grid alternates between u and a
+// second buffer (h.R1, idle in this form): step k reads buffer k & 1.
+bool stage_folded(const pcrp_heston_params* p, int precision)
 {
+    if (p->separate_explicit) return false;
+    if (precision == PCRP_PREC_F64) return p->ns <= lds_f64_max_n();
+    return p->ns <= lds_f32r_max_n() && !window_preferred(p->ns, p->nv);
+}
+
+// S-lines with the stage folded in: grid Uin (never written), Y1 + W to Uout
+int solve_s_lines_folded(const Grid& g, int precision, const double* Uin, double* Uout,
+                         const HestonWs& h, int off, hipStream_t st)
+{
+    const int n = g.ns, m = g.nv;
+    const HestonStage src{g, Uin, h.step, off, h.bnd};
+    if (precision == PCRP_PREC_F64)
+        return lds_f64_solve_stage(h.sa, h.sb, h.sc, h.sd, h.se, Uout, n, m, src, st) == 0
+                   ? PCRP_OK
+                   : PCRP_ERR_HIP;
+    SolveArgsF32R A{h.sa, h.sb, h.sc, h.sd, h.se, nullptr, n, m, 1, n, pcr_steps(n),
+                    PCRP_HESTON_MAX_SWEEPS, PCRP_HESTON_REFINE_TOL, nullptr};
+    A.out = Uout;
+    return lds_f32r_solve_stage(A, src, st) == 0 ? PCRP_OK : PCRP_ERR_HIP;
+}
+
+// One Douglas step (step index: device counter + off) from grid Uin to grid
+// Uout. Folded: two launches, Uin != Uout. Separate: the explicit kernel,
+// then the S-line solve (three launches; four when the S-line solve cannot
+// store Y1 + W itself), in place (Uin == Uout).
+int enqueue_step(const Grid& g, int precision, bool folded, const double* Uin, double* Uout,
+                 const HestonWs& h, int off, hipStream_t st)
+{
+    if (folded) {
+        const int rc = solve_s_lines_folded(g, precision, Uin, Uout, h, off, st);
+        return rc != PCRP_OK ? rc : solve_v_lines(g, precision, Uout, h, st);
+    }
+    double* const U = Uout;
     const long long N = (long long)g.ns * g.nv;
     const int bs = 256;
     const unsigned blocks = (unsigned)((N + bs - 1) / bs);
@@ -462,12 +353,16 @@ int enqueue_step(const Grid& g, int precision, double* U, const HestonWs& h, int
     return solve_v_lines(g, precision, U, h, st);
 }
 
-// `count` consecutive steps, then the counter moves past them
-int enqueue_steps(const Grid& g, int precision, double* U, const HestonWs& h, int count,
-                  hipStream_t st)
+// `count` consecutive steps starting from grid buffer `parity` (folded form:
+// buffer 0 is u, buffer 1 is h.R1; separate form: always u), then the counter
+// moves past them
+int enqueue_steps(const Grid& g, int precision, bool folded, double* u, const HestonWs& h,
+                  int count, int parity, hipStream_t st)
 {
+    double* const buf[2] = {u, folded ? h.R1 : u};
     for (int k = 0; k < count; ++k) {
-        const int rc = enqueue_step(g, precision, U, h, k, st);
+        const int b = (parity + k) & 1;
+        const int rc = enqueue_step(g, precision, folded, buf[b], buf[b ^ 1], h, k, st);
         if (rc != PCRP_OK) return rc;
     }
     hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, st, h.step, count);
@@ -476,18 +371,19 @@ int enqueue_steps(const Grid& g, int precision, double* U, const HestonWs& h, in
 
 // Steps per captured graph: one graph launch per chunk instead of per step
 // (each graph launch left ~8 us between steps on MI355X), with the counter
-// advanced once per chunk.
-constexpr int kStepsPerGraph = 25;
+// advanced once per chunk. Even, so that every replay of a folded-form chunk
+// starts and ends in u.
+constexpr int kStepsPerGraph = 20;
 
 // Captures `count` steps into an executable graph.
-int capture_steps(const Grid& g, int precision, double* U, const HestonWs& h, int count,
-                  hipStream_t st, hipGraphExec_t* exec)
+int capture_steps(const Grid& g, int precision, bool folded, double* u, const HestonWs& h,
+                  int count, hipStream_t st, hipGraphExec_t* exec)
 {
     hipGraph_t graph = nullptr;
     *exec = nullptr;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return PCRP_ERR_HIP;
-    const int rc = enqueue_steps(g, precision, U, h, count, st);
+    const int rc = enqueue_steps(g, precision, folded, u, h, count, 0, st);
     const hipError_t ec = hipStreamEndCapture(st, &graph);
     int out = rc != PCRP_OK ? rc : (ec != hipSuccess ? PCRP_ERR_HIP : PCRP_OK);
     if (out == PCRP_OK && hipGraphInstantiate(exec, graph, nullptr, nullptr, 0) != hipSuccess) {
@@ -519,6 +415,7 @@ int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph,
     if (precision != PCRP_PREC_F64 && precision != PCRP_PREC_F32_REFINE) return PCRP_ERR_ARG;
     if (!(p->dt > 0.0) || !(p->smax > 0.0) || !(p->vmax > 0.0)) return PCRP_ERR_ARG;
     if (p->reuse_factor != 0 && p->reuse_factor != 1) return PCRP_ERR_ARG;
+    if (p->separate_explicit != 0 && p->separate_explicit != 1) return PCRP_ERR_ARG;
     if (!reuse_supported(p, precision)) return PCRP_ERR_UNSUPPORTED;
     if (ws_bytes < heston_ws_bytes(p, precision)) return PCRP_ERR_WORKSPACE;
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -536,12 +433,21 @@ int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph,
     if (hipGetLastError() != hipSuccess) return PCRP_ERR_HIP;
     const int frc = factor_lines(g, precision, h, st);
     if (frc != PCRP_OK) return frc;
+    const bool folded = stage_folded(p, precision);
+    // folded form, odd step count: the last step's grid is in h.R1
+    const auto finish = [&]() {
+        if (!folded || (p->steps & 1) == 0) return PCRP_OK;
+        return hipMemcpyAsync(u, h.R1, (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, st) ==
+                       hipSuccess
+                   ? PCRP_OK
+                   : PCRP_ERR_HIP;
+    };
     if (!use_graph) {
         for (int n = 0; n < p->steps; ++n) {
-            const int rc = enqueue_steps(g, precision, u, h, 1, st);
+            const int rc = enqueue_steps(g, precision, folded, u, h, 1, n & 1, st);
             if (rc != PCRP_OK) return rc;
         }
-        return PCRP_OK;
+        return finish();
     }
     if (p->steps == 0) return PCRP_OK;
     if (st == nullptr) return PCRP_ERR_ARG;   // the null stream cannot be captured
@@ -549,12 +455,14 @@ int pcrp_heston_solve(const pcrp_heston_params* p, int precision, int use_graph,
     const int K = p->steps < kStepsPerGraph ? p->steps : kStepsPerGraph;
     const int full = p->steps / K, rest = p->steps % K;
     hipGraphExec_t exec = nullptr, exec_rest = nullptr;
-    int out = capture_steps(g, precision, u, h, K, st, &exec);
-    if (out == PCRP_OK && rest > 0) out = capture_steps(g, precision, u, h, rest, st, &exec_rest);
+    int out = capture_steps(g, precision, folded, u, h, K, st, &exec);
+    if (out == PCRP_OK && rest > 0)
+        out = capture_steps(g, precision, folded, u, h, rest, st, &exec_rest);
     for (int c = 0; c < full && out == PCRP_OK; ++c)
         if (hipGraphLaunch(exec, st) != hipSuccess) out = PCRP_ERR_HIP;
     if (out == PCRP_OK && exec_rest && hipGraphLaunch(exec_rest, st) != hipSuccess)
         out = PCRP_ERR_HIP;
+    if (out == PCRP_OK) out = finish();
     // replays finish before the executable graphs are released
     if ((exec || exec_rest) && hipStreamSynchronize(st) != hipSuccess) out = PCRP_ERR_HIP;
     if (exec && hipGraphExecDestroy(exec) != hipSuccess) out = PCRP_ERR_HIP;

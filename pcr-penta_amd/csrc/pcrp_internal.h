@@ -196,6 +196,21 @@ int pass_shape_g(int shape);
 int pass_shape_rpt(int shape);
 int pass_shape_single(int shape);
 
+// Right-hand side of a solve read from y, and x (+ add[]) stored: every
+// solve except the Heston S-lines with the explicit stage folded in
+// (HestonStage, pcrp_heston_ops.h).
+struct PlainRhs {
+    static constexpr bool kFused = false;
+};
+struct HestonStage;
+// Heston S-lines with the explicit stage computed while the rows load:
+// A.m contiguous systems of A.n rows (a..e at j*n + i), x + W stored to A.out
+// (FP32+refine) or x (FP64). -2: n beyond the one-workgroup kernels.
+int lds_f32r_solve_stage(const SolveArgsF32R& A, const HestonStage& src, hipStream_t st);
+int lds_f64_solve_stage(const double* a, const double* b, const double* c, const double* d,
+                        const double* e, double* x, int n, int m, const HestonStage& src,
+                        hipStream_t st);
+
 int lds_f32r_max_n();
 int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st);
 // Writes the FP32 block factor of A's systems (of the one system when

@@ -29,6 +29,7 @@
 // A block determinant below 2^-100 gives a zero inverse (the refinement then
 // flags the system if it cannot converge).
 #include "pcrp_device.h"
+#include "pcrp_heston_ops.h"
 #include "pcrp_internal.h"
 
 namespace pcrp {
@@ -89,14 +90,17 @@ __host__ __device__ constexpr long long f32r_factor_bytes_nb(int NB, int LMAX)
     return 16 + 16LL * NB * (2 * LMAX + 1);
 }
 
-template <int NT, int BR, int LMAX, int G, int MODE, int MINW = 1>
+// Src: PlainRhs (y read from gy, x + add[] stored) or HestonStage (G = 1,
+// kSolve: y computed from the previous Douglas step's grid, x + W stored).
+template <int NT, int BR, int LMAX, int G, int MODE, int MINW = 1, class Src = PlainRhs>
 __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
     const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
     const double* __restrict__ gd, const double* __restrict__ ge, const double* gy,
     double* gx, const double* __restrict__ add, int n, int m, long long rs, long long ss,
     long long ars, long long ass, int max_sweeps, double tol, int* __restrict__ sweeps_out,
-    float4* __restrict__ fct)
+    float4* __restrict__ fct, Src src)
 {
+    static_assert(!Src::kFused || (G == 1 && MODE == kSolve), "folded stage: one system, solve");
     constexpr int NB = NT * BR;        // block rows
     constexpr int NMAX = 2 * NB;       // rows
     static_assert(G == 1 || G % 2 == 0, "systems are loaded in pairs");
@@ -124,6 +128,10 @@ __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
     const long long base = (long long)sys * ss;
     const long long abase = (long long)sys * ass;
     const int nb = (n + 1) >> 1;
+    [[maybe_unused]] const auto rhs = [&] {
+        if constexpr (Src::kFused) return src.bind(sys);
+        else return 0;
+    }();
     // this system's factor block (the shared one when ass == 0)
     float4* const Fc = reinterpret_cast<float4*>(reinterpret_cast<char*>(fct) +
                                                  (ass == 0 ? 0LL : (long long)sys) *
@@ -196,8 +204,12 @@ __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
                 const double2 cv = *reinterpret_cast<const double2*>(gc + g);
                 const double2 dv = *reinterpret_cast<const double2*>(gd + g);
                 const double2 ev = *reinterpret_cast<const double2*>(ge + g);
-                const double2 yv = MODE == kFactor ? make_double2(0.0, 0.0)
-                                                   : *reinterpret_cast<const double2*>(gy + base + 2LL * k);
+                double2 yv;
+                if constexpr (MODE == kFactor) yv = make_double2(0.0, 0.0);
+                else if constexpr (Src::kFused)   // W parked in x's elements until the store
+                    yv = make_double2(rhs.r1(2 * k, gx + base + 2LL * k),
+                                      rhs.r1(2 * k + 1, gx + base + 2LL * k + 1));
+                else yv = *reinterpret_cast<const double2*>(gy + base + 2LL * k);
                 const double ra[2] = {av.x, av.y}, rb[2] = {bv.x, bv.y}, rc[2] = {cv.x, cv.y};
                 const double rd[2] = {dv.x, dv.y}, re[2] = {ev.x, ev.y}, ry[2] = {yv.x, yv.y};
 #pragma unroll
@@ -228,7 +240,9 @@ __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
                     const long long g = abase + (long long)i * ars;
                     a = band(0, i, ga[g]); b = band(1, i, gb[g]); c = gc[g];
                     d = band(3, i, gd[g]); e = band(4, i, ge[g]);
-                    y = MODE == kFactor ? 0.0 : gy[base + (long long)i * rs];
+                    if constexpr (MODE == kFactor) y = 0.0;
+                    else if constexpr (Src::kFused) y = rhs.r1(i, gx + base + (long long)i * rs);
+                    else y = gy[base + (long long)i * rs];
                     A[0][p] = a; A[1][p] = b; A[2][p] = c; A[3][p] = d; A[4][p] = e;
                     A[5][p] = y;
                 } else {
@@ -466,7 +480,8 @@ __global__ __launch_bounds__(NT * G, MINW) void lds_f32r_kernel(
             const int i = 2 * k + h;
             if (i < n) {
                 const long long g = base + (long long)i * rs;
-                gx[g] = add ? X[q][h] + add[g] : X[q][h];
+                if constexpr (Src::kFused) gx[g] = X[q][h] + gx[g];   // this thread's W
+                else gx[g] = add ? X[q][h] + add[g] : X[q][h];
             }
         }
     }
@@ -494,17 +509,17 @@ static int launch_f32r(const SolveArgsF32R& A, hipStream_t st, int mode = kSolve
             hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, 1, kFactor>),
                                dim3(A.shared ? 1 : A.m), dim3(NT), 0, st, A.a, A.b, A.c, A.d,
                                A.e, A.y, gx, A.add, A.n, A.shared ? 1 : A.m, A.rs, A.ss, ars, ass,
-                               0, 0.0, nullptr, f4p);
+                               0, 0.0, nullptr, f4p, PlainRhs{});
         else
             return -2;
     } else if (A.fct) {
         hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, G, kCached>), dim3(A.m / G),
                            dim3(NT * G), 0, st, A.a, A.b, A.c, A.d, A.e, A.y, gx, A.add, A.n,
-                           A.m, A.rs, A.ss, ars, ass, A.max_sweeps, A.tol, A.sweeps_out, f4p);
+                           A.m, A.rs, A.ss, ars, ass, A.max_sweeps, A.tol, A.sweeps_out, f4p, PlainRhs{});
     } else {
         hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, G, kSolve>), dim3(A.m / G),
                            dim3(NT * G), 0, st, A.a, A.b, A.c, A.d, A.e, A.y, gx, A.add, A.n,
-                           A.m, A.rs, A.ss, ars, ass, A.max_sweeps, A.tol, A.sweeps_out, f4p);
+                           A.m, A.rs, A.ss, ars, ass, A.max_sweeps, A.tol, A.sweeps_out, f4p, PlainRhs{});
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -565,6 +580,30 @@ int lds_f32r_solve(const SolveArgsF32R& A, hipStream_t st)
         return groupable(A, 2) && A.m > cus && A.m <= 2 * cus ? launch_f32r<256, 2, 2>(A, st)
                                                               : launch_f32r<256, 2, 1>(A, st);
     if (n <= 2048) return launch_f32r<512, 2, 1>(A, st);
+    return -2;
+}
+
+template <int NT, int BR>
+static int launch_f32r_stage(const SolveArgsF32R& A, const HestonStage& src, hipStream_t st)
+{
+    constexpr int LMAX = lmax_for<NT * BR>();
+    hipLaunchKernelGGL((lds_f32r_kernel<NT, BR, LMAX, 1, kSolve, 1, HestonStage>), dim3(A.m),
+                       dim3(NT), 0, st, A.a, A.b, A.c, A.d, A.e, nullptr, A.out, nullptr, A.n,
+                       A.m, A.rs, A.ss, A.rs, A.ss, A.max_sweeps, A.tol, A.sweeps_out, nullptr,
+                       src);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// The one-system shapes of lds_f32r_solve (contiguous lines never group).
+int lds_f32r_solve_stage(const SolveArgsF32R& A, const HestonStage& src, hipStream_t st)
+{
+    const int n = A.n;
+    if (A.rs != 1 || A.ss != n || !A.out) return -2;
+    if (n <= 128) return launch_f32r_stage<64, 1>(A, src, st);
+    if (n <= 256) return launch_f32r_stage<128, 1>(A, src, st);
+    if (n <= 512) return launch_f32r_stage<128, 2>(A, src, st);
+    if (n <= 1024) return launch_f32r_stage<256, 2>(A, src, st);
+    if (n <= 2048) return launch_f32r_stage<512, 2>(A, src, st);
     return -2;
 }
 

@@ -36,6 +36,7 @@
 #include "pcrp_device.h"
 #include "pcrp_internal.h"
 #include "pcrp_spaced.h"
+#include "pcrp_heston_ops.h"
 
 namespace pcrp {
 
@@ -419,6 +420,39 @@ __global__ __launch_bounds__(NT) void lds_f64_kernel(
                                  split.S > 0 ? nullptr : split.add);
 }
 
+// Heston S-lines with the explicit stage folded in: system sys is S-line j
+// (contiguous rows base + i), its right-hand side computed from the previous
+// step's grid while the rows load, x + W stored to gx (never the grid read).
+template <int KIND, int NT, int RPT>
+__global__ __launch_bounds__(NT) void lds_f64_stage_kernel(
+    const double* __restrict__ ga, const double* __restrict__ gb, const double* __restrict__ gc,
+    const double* __restrict__ gd, const double* __restrict__ ge, double* gx, int n, int m,
+    int steps, HestonStage src)
+{
+    constexpr int NMAX = NT * RPT;
+    __shared__ double sR[6][NMAX];
+    __shared__ double sE[KIND == KIND_SHARED ? 8 : 1][KIND == KIND_SHARED ? NMAX : 1];
+    const int sys = xcd_remap(blockIdx.x, m);
+    const long long base = (long long)sys * n;
+    const auto rhs = src.bind(sys);
+    const int t = threadIdx.x;
+    Row<double> r[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int i = t + NT * k;
+        if (i < n) {
+            const long long g = base + i;
+            r[k].a = ga[g]; r[k].b = gb[g]; r[k].c = gc[g];
+            r[k].d = gd[g]; r[k].e = ge[g]; r[k].y = rhs.r1(i, gx + g);   // W parked in x's element
+        } else {
+            r[k] = Row<double>{0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+        }
+    }
+    lds_solve_regs_f<KIND, NT, RPT>(r, n, steps, sR, sE, [&](int i, double x) {
+        gx[base + i] = x + gx[base + i];   // behind the solve's barriers
+    });
+}
+
 // Persistent batch form: a resident grid walks the systems, and while it
 // solves one system the rows of its next system are already being loaded
 // into registers, so the strided HBM loads overlap the LDS steps. Virtual
@@ -694,6 +728,28 @@ static const LdsVariantF64 kVariants[] = {
 #undef PCRP_V
 #undef PCRP_P
 #undef PCRP_2
+
+template <int KIND, int NT, int RPT>
+static int launch_stage(const double* a, const double* b, const double* c, const double* d,
+                        const double* e, double* x, int n, int m, const HestonStage& src,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL((lds_f64_stage_kernel<KIND, NT, RPT>), dim3(m), dim3(NT), 0, st, a, b, c,
+                       d, e, x, n, m, pcr_steps(n), src);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Shapes as the batch defaults for these sizes: the redundant form up to
+// n = 512, spaced 256 x 4 above.
+int lds_f64_solve_stage(const double* a, const double* b, const double* c, const double* d,
+                        const double* e, double* x, int n, int m, const HestonStage& src,
+                        hipStream_t st)
+{
+    if (n <= 256) return launch_stage<KIND_REDUNDANT, 256, 1>(a, b, c, d, e, x, n, m, src, st);
+    if (n <= 512) return launch_stage<KIND_REDUNDANT, 256, 2>(a, b, c, d, e, x, n, m, src, st);
+    if (n <= 1024) return launch_stage<KIND_SPACED, 256, 4>(a, b, c, d, e, x, n, m, src, st);
+    return -2;
+}
 
 int lds_f64_variant_count() { return (int)(sizeof(kVariants) / sizeof(kVariants[0])); }
 

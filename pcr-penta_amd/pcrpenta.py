@@ -578,7 +578,8 @@ class HestonParams(ctypes.Structure):
                 ("sigma", ctypes.c_double), ("rho", ctypes.c_double),
                 ("r", ctypes.c_double), ("strike", ctypes.c_double),
                 ("dt", ctypes.c_double), ("steps", ctypes.c_int),
-                ("theta_scheme", ctypes.c_double), ("reuse_factor", ctypes.c_int)]
+                ("theta_scheme", ctypes.c_double), ("reuse_factor", ctypes.c_int),
+                ("separate_explicit", ctypes.c_int)]
 
 
 # refinement settings of the FP32 line solves in the Heston driver
@@ -589,7 +590,7 @@ HESTON_MAX_SWEEPS = 8
 # Heston ADI problem: Ns = 1024, Nv = 512 unknowns, 200 steps of 1/200
 HESTON_GRID = dict(ns=1024, nv=512, smax=800.0, vmax=5.0, kappa=3.0, theta_v=0.04, sigma=0.15,
                rho=-0.7, r=0.03, strike=100.0, dt=1.0 / 200, steps=200, theta_scheme=0.5,
-               reuse_factor=0)
+               reuse_factor=0, separate_explicit=0)
 
 
 def heston_params(**kw) -> HestonParams:

@@ -951,6 +951,26 @@ def test_heston_graph_replay_equals_eager(gpu):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_heston_folded_stage_equals_separate(gpu, prec):
+    """The explicit stage computed by the S-line solve kernels while they load
+    (two launches per step, the grid alternating between two buffers) gives
+    the separate explicit-stage kernel's price grid bit for bit: both evaluate
+    the same stencil expressions, so the solves see the same right-hand sides.
+    Odd step counts (the last grid copied back), several replays of a captured
+    chunk plus a shorter one, eager steps, both reuse forms, and the full grid."""
+    cases = (dict(ns=128, nv=64, steps=45), dict(ns=96, nv=40, steps=7), dict(steps=6))
+    for kw in cases:
+        for use_graph in (False, True):
+            for reuse in (0, 1):
+                _say("heston-folded", prec, kw, use_graph, reuse)
+                U0, _ = _heston_gpu(gpu, prec, use_graph, reuse_factor=reuse,
+                                    separate_explicit=1, **kw)
+                U1, _ = _heston_gpu(gpu, prec, use_graph, reuse_factor=reuse,
+                                    separate_explicit=0, **kw)
+                assert np.array_equal(U0, U1), (kw, use_graph, reuse, np.abs(U0 - U1).max())
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_heston_factor_reuse_equals_full_solves(gpu, prec):
     """The Heston grid, all 200 steps: factoring the shared v-line matrix once
     gives the same price grid bit for bit as solving every line in full each
