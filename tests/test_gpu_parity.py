@@ -957,8 +957,10 @@ def test_heston_folded_stage_equals_separate(gpu, prec):
     the separate explicit-stage kernel's price grid bit for bit: both evaluate
     the same stencil expressions, so the solves see the same right-hand sides.
     Odd step counts (the last grid copied back), several replays of a captured
-    chunk plus a shorter one, eager steps, both reuse forms, and the full grid."""
-    cases = (dict(ns=128, nv=64, steps=45), dict(ns=96, nv=40, steps=7), dict(steps=6))
+    chunk plus a shorter one, odd line lengths (rows loaded one at a time),
+    eager steps, both reuse forms, and the full grid."""
+    cases = (dict(ns=128, nv=64, steps=45), dict(ns=96, nv=40, steps=7),
+             dict(ns=301, nv=37, steps=5), dict(steps=6))
     for kw in cases:
         for use_graph in (False, True):
             for reuse in (0, 1):
