@@ -153,6 +153,19 @@ def measured_valu(workload):
         return json.load(f).get(workload, {}).get("valu_insts_per_step")
 
 
+def rocprof_summary(workload, solve_s):
+    """This workload's rocprofv3 figures (profiles/hbm_traffic.json): HBM bytes
+    of one solve over this run's solve time, and LDS bank-conflict cycles
+    (SQ_LDS_BANK_CONFLICT summed over the solver kernels of one solve)."""
+    e = _traffic_entry(workload)
+    if not e.get("bytes_per_step"):
+        return None
+    return {"hbm_gb_per_solve": e["bytes_per_step"] / 1e9,
+            "hbm_gb_per_s": e["bytes_per_step"] / solve_s / 1e9,
+            "lds_bank_conflict_cycles_per_solve": e.get("lds_bank_conflicts_per_step"),
+            "profile": e.get("profile")}
+
+
 def roofline(unknowns, solve_s, workload, note):
     achieved = BYTES_PER_UNKNOWN * unknowns / solve_s
     out = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -167,6 +180,9 @@ def roofline(unknowns, solve_s, workload, note):
                                   "dispatches_per_solve": dom["dispatches_per_step"],
                                   "share_of_solve": dom["avg_us"] * dom["dispatches_per_step"]
                                   / 1e3 / (solve_s * 1e3)}
+    rp = rocprof_summary(workload, solve_s)
+    if rp:
+        out["rocprof"] = rp
     insts = measured_valu(workload)
     if insts:
         # issue bound: one wave64 VALU instruction per CU per cycle (4 SIMDs,
@@ -595,13 +611,17 @@ def also_workloads(args):
             "ratio_f32r_vs_f64": s64["ms_per_run"] / s32["ms_per_run"],
             "note": "a separate explicit-stage kernel before each S-line solve (three launches "
                     "per step); the same bits as the folded stage"},
+        "rocprof": {f: rocprof_summary("heston_adi_" + f, r["ms_per_run"] / 1e3)
+                    for f, r in (("f64", h64), ("f32r", h32), ("f64_reuse", h64r),
+                                 ("f32r_reuse", h32r))},
         "f32r_vs_f64_grid_relerr": float(np.abs(U32 - U64).max() / np.abs(U64).max()),
         "price_at_S100_v0.04_f64": float(_interp_price(U64, p, hS, hv)),
         "refine": {"tol": 1e-12, "max_sweeps": 8},
     }
-    for label, n, m, kind in (("lds_n1024_m16384", 1024, 16384, 0),
-                              ("n2048_m16384", 2048, 16384, 0),
-                              ("hbm_n1048576_m32", 1 << 20, 32, 1)):
+    for label, n, m, kind, wl in (
+            ("lds_n1024_m16384", 1024, 16384, 0, ("lds_f64_m16384_n1024", "lds_f32r_m16384_n1024")),
+            ("n2048_m16384", 2048, 16384, 0, ("hbm_f64_m16384_n2048", "lds_f32r_m16384_n2048")),
+            ("hbm_n1048576_m32", 1 << 20, 32, 1, ("hbm_f64_m32_n1048576", "hbm_f32r_m32_n1048576"))):
         r64 = run_uniform(n, m, "f64", kind, K, W)
         r32 = run_uniform(n, m, "f32r", kind, K, W)
         p32 = plan_execute_ms(r32["batch"], "f32r", 5)
@@ -619,6 +639,8 @@ def also_workloads(args):
             "ratio_f32r_plan_vs_f64": r64["ms_per_step"] / p32["ms"],
             "f32r_stage_path": s32 and {**s32, "ratio_vs_f64": r64["ms_per_step"] / s32["ms"]},
             "f64_tuned": (r64["tuned"] or {}).get("variant", (r64["tuned"] or {}).get("split")),
+            "rocprof": {"f64": rocprof_summary(wl[0], r64["solve_s"]),
+                        "f32r": rocprof_summary(wl[1], r32["solve_s"])},
         }
         del r64, r32
     return out
