@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define PCRP_ABI_VERSION 1
+#define PCRP_ABI_VERSION 2   /* 2: pcrp_heston_params gained separate_explicit */
 
 /* Status codes. */
 enum {

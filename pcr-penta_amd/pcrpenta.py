@@ -133,13 +133,21 @@ def _declare(lib):
     return lib
 
 
+# PCRP_ABI_VERSION of include/pcr_penta.h this binding mirrors
+ABI_VERSION = 2
+
+
 def load():
     """Load the in-tree libpcrpenta.so; raises if it has not been built."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FileNotFoundError(f"{LIB_PATH} missing (run __graft_entry__.build())")
-        _lib = _declare(ctypes.CDLL(LIB_PATH))
+        lib = _declare(ctypes.CDLL(LIB_PATH))
+        if lib.pcrp_abi_version() != ABI_VERSION:   # the structures mirrored below
+            raise RuntimeError(f"{LIB_PATH}: ABI version {lib.pcrp_abi_version()}, "
+                               f"this binding expects {ABI_VERSION} (rebuild the library)")
+        _lib = lib
     return _lib
 
 

@@ -35,7 +35,7 @@ def test_exports_every_declared_symbol(lib):
 
 def test_abi_version_and_strerror(lib):
     import pcrpenta
-    assert lib.pcrp_abi_version() == 1
+    assert lib.pcrp_abi_version() == pcrpenta.ABI_VERSION == 2
     for st, name in pcrpenta.STATUS.items():
         assert pcrpenta.strerror(st)
     assert pcrpenta.strerror(999) == "unknown status"
